@@ -1,0 +1,187 @@
+"""Benchmark of the midint HIP kernels on one MI355X.
+
+Headline (``value``): big-integer multiplications per second at 8192 bits --
+16384 pairs of 128-limb operands, full 256-limb products (classical kernel),
+inputs resident in HBM, one step = one batch. The same JSON line also
+carries the batched-addition bandwidth (65536 x 64 limbs, HBM roofline) and
+the NTT multiplication rate (1024 x 2048 limbs).
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+MIB = 1 << 20
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed(torch, fn, steps, warmup):
+    """Mean ms per call of fn() over `steps` calls after `warmup` untimed
+    calls, HIP events on torch's current stream (the stream the kernels are
+    launched on)."""
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def to_dev(torch, x):
+    return torch.from_numpy(np.ascontiguousarray(x).view(np.int64)).to("cuda:0")
+
+
+def cpu_baseline_mul(a, b, m, threads, budget_s=1.0):
+    """Oracle restatement (port of the reference algorithm) and GMP mpn_mul on
+    the host cores, repeated over the batch until ~budget_s of wall time."""
+    import oracle
+
+    out = {}
+    for kind, fn in (("port", lambda x, y: oracle.mul_full(x, y, m, threads=threads)),
+                     ("gmp", lambda x, y: oracle.gmp_mul_full(x, y, m, threads=threads))):
+        if kind == "gmp" and not oracle.gmp_available():
+            continue
+        reps, t0 = 0, time.perf_counter()
+        while True:
+            fn(a, b)
+            reps += 1
+            el = time.perf_counter() - t0
+            if el >= budget_s:
+                break
+        out[kind] = (reps * (a.size // m)) / el
+    return out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: every instance is independent, so N GPUs would each run "
+              f"this same kernel on 1/N of the batch with no exchange (see DESIGN.md); --gpus {args.gpus} "
+              f"is not measured here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from __graft_entry__ import load_package
+    from workloads import make_inputs
+
+    mi = load_package()
+    mi.lib()
+    ctx = mi.Context(0)
+
+    # ---------------------------------------------------------- headline: mul
+    n, m = 16384, 128
+    a, b = make_inputs("mul_16384x128", mi.gen_limbs)
+    da, db = to_dev(torch, a), to_dev(torch, b)
+    dr = torch.empty(2 * n * m, dtype=torch.int64, device="cuda:0")
+    log(f"mul_full {n} x {m} limbs: warmup {args.warmup}, steps {args.steps}")
+    ms = timed(torch, lambda: ctx.mul_full(da, db, dr, m), args.steps, args.warmup)
+    muls_per_s = n / (ms * 1e-3)
+    pp_per_launch = n * (2 * m) ** 2  # 32x32->64 partial products of the full product
+    mad_rate, add_rate = mi.probe_valu(0)
+    achieved_pp = pp_per_launch / (ms * 1e-3)
+    log(f"mul_full: {ms * 1e3:.1f} us/batch, {muls_per_s / 1e6:.2f} M muls/s; "
+        f"probe v_mad_u64_u32 {mad_rate / 1e12:.2f} T/s")
+
+    # classical low-half product (reference semantics) at 256 limbs = 128x128 padded, like the reference harness
+    # ---------------------------------------------------------- NTT
+    nn, mn = 1024, 2048
+    an, bn = make_inputs("ntt_1024x2048", mi.gen_limbs)
+    dan, dbn = to_dev(torch, an), to_dev(torch, bn)
+    drn = torch.empty(2 * nn * mn, dtype=torch.int64, device="cuda:0")
+    ctx.ntt_prepare(mn)
+    ms_ntt = timed(torch, lambda: ctx.mul_ntt(dan, dbn, drn, mn), max(5, args.steps // 20), 2)
+    log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch")
+    del dan, dbn, drn
+
+    # ---------------------------------------------------------- add (HBM)
+    na, ma = 65536, 64
+    aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
+    sets = []
+    for _ in range(4):  # 4 x 96 MiB > the 256 MiB Infinity Cache
+        sets.append((to_dev(torch, aa), to_dev(torch, ba), torch.empty(na * ma, dtype=torch.int64, device="cuda:0")))
+    it = [0]
+
+    def add_step():
+        x, y, z = sets[it[0] & 3]
+        it[0] += 1
+        ctx.add(x, y, z, ma)
+
+    ms_add = timed(torch, add_step, 400, 40)
+    add_bytes = 3 * na * ma * 8
+    add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
+    log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
+
+    # ---------------------------------------------------------- CPU baseline
+    cpu = None
+    if not args.no_cpu:
+        threads = min(16, os.cpu_count() or 1)
+        k = 2048
+        base = cpu_baseline_mul(a[: k * m], b[: k * m], m, threads)
+        cpu = {"value": base.get("port"), "unit": "muls/s", "cores": threads, "kind": "port",
+               "sample": f"{k} of the 16384 pairs (128 x 128 limbs, full product), repeated for ~1 s wall "
+                         f"on {threads} threads; oracle C restatement",
+               "gmp_mpn_mul_muls_per_s": base.get("gmp")}
+        log(f"cpu baseline: {cpu}")
+
+    out = {
+        "metric": "big-int muls/s at 8192-bit",
+        "value": muls_per_s,
+        "unit": "muls/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u64",
+        "data": "synthetic (xoshiro256** limbs, seed 0x6D6964696E74 + 1)",
+        "config": {"workload": "mul_16384x128: 16384 pairs of 8192-bit ints, full 256-limb products, "
+                               "classical kernel", "instances": n, "limbs": m},
+        "roofline": {"bound": "valu", "achieved": achieved_pp / 1e12, "peak": mad_rate / 1e12,
+                     "unit": "T partial-products/s (32x32->64 v_mad_u64_u32)",
+                     "frac": achieved_pp / mad_rate, "traffic": None,
+                     "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"},
+        "cpu_baseline": cpu,
+        "extra": {
+            "add_65536x64": {"us_per_launch": ms_add * 1e3, "GB_per_s": add_gbs,
+                             "roofline": {"bound": "hbm", "achieved": add_gbs, "peak": HBM_PEAK_GBS,
+                                          "unit": "GB/s", "frac": add_gbs / HBM_PEAK_GBS,
+                                          "traffic": None},
+                             "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)"},
+            "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3)},
+            "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
+        },
+    }
+    ctx.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,149 @@
+/*
+ * midint -- batched midsize big-integer arithmetic on AMD Instinct MI355X
+ * (gfx950). Public C ABI of libmidint.so.
+ *
+ * Data layout (same contract as the reference, cuda/ker-helpers.cu.h:12-19,
+ * cuda/helper.h:116): an integer of m limbs is m uint64_t, limb 0 least
+ * significant (a limb is two little-endian u32 words); a batch of n integers
+ * is n*m contiguous limbs, instance i at [i*m, (i+1)*m).
+ *
+ * The reference's interface is C++ templates with compile-time sizes:
+ * kernels baddKer/a6pb10Ker/bmulKerQ/polyKerQ<Base,IPB,M,Q>(num_instances,
+ * ass, bss, rss) and bmulFFT<P,M,Q>(...), launched by host drivers
+ * gpuAdd/gpuMultiply/gpuMulFFT<Base,m>(num_instances, h_as, h_bs, h_rs)
+ * (cuda/main.cu). Here sizes are runtime arguments and every entry point is
+ * plain C. Each declaration names the reference interface it replaces.
+ *
+ * Device-pointer entry points take pointers to device memory (8-byte
+ * aligned; 16-byte alignment enables vector loads) and a hipStream_t passed
+ * as `void*` (NULL = default stream). They enqueue work and return without
+ * synchronising. Inputs are read-only; the output must not overlap the
+ * inputs unless stated. Errors are returned as midint_status; nothing is
+ * printed.
+ */
+#ifndef MIDINT_H
+#define MIDINT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    MIDINT_OK = 0,
+    MIDINT_EINVAL = 1,       /* bad argument (NULL pointer, m == 0, ...)        */
+    MIDINT_EUNSUPPORTED = 2, /* size outside what the kernels support          */
+    MIDINT_EHIP = 3,         /* HIP runtime error (launch, allocation, copy)   */
+    MIDINT_ENOMEM = 4,       /* host or device allocation failed              */
+    MIDINT_EBUSY = 5,        /* resource temporarily unavailable (admission)  */
+    MIDINT_ENOTFOUND = 6     /* unknown request / handle                       */
+} midint_status;
+
+typedef struct midint_ctx midint_ctx;
+
+/* Human-readable text of a status code (static string). */
+const char* midint_strerror(midint_status s);
+/* Library version string. */
+const char* midint_version(void);
+/* HIP error code of the last MIDINT_EHIP returned on this thread. */
+int midint_last_hip_error(void);
+
+/* Context: device binding, NTT twiddle cache, scratch. Replaces the
+ * reference's hard-coded cudaSetDevice(1) (cuda/main.cu:757) and the
+ * per-call twiddle construction in gpuMulFFT (cuda/main.cu:458-470). */
+midint_status midint_init(midint_ctx** ctx, int device);
+midint_status midint_destroy(midint_ctx* ctx);
+
+/* Largest m (limbs) accepted by each operation. */
+uint32_t midint_max_limbs_add(void);
+uint32_t midint_max_limbs_mul(void);
+uint32_t midint_max_limbs_ntt(void);
+
+/* ---------------------------------------------------------------- add */
+/* r = (a + b) mod B^m per instance; carry (optional, n bytes) receives the
+ * dropped carry-out. In-place (r == a or r == b) is allowed.
+ * Replaces baddKer<Base,IPB,M,Q> (cuda/ker-addition.cu.h:227-260) and its
+ * launch geometry in gpuAdd (cuda/main.cu:63-82). */
+midint_status midint_add(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                         uint64_t* d_r, uint8_t* d_carry, void* stream);
+
+/* r = (6a + 10b) mod B^m per instance.
+ * Replaces a6pb10Ker (cuda/ker-addition.cu.h:266-328). */
+midint_status midint_add6(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                          uint64_t* d_r, void* stream);
+
+/* ---------------------------------------------------------------- mul */
+/* r = (a * b) mod B^m per instance (m limbs out).
+ * Replaces bmulKerQ<Base,IPB,M,Q> (cuda/ker-classic-mul.cu.h:468-498) and
+ * the geometry in gpuMultiply (cuda/main.cu:243-261). */
+midint_status midint_mul_lo(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                            uint64_t* d_r, void* stream);
+
+/* r = a * b per instance (2m limbs out), classical algorithm. The reference
+ * obtains a full product by calling bmulKerQ at 2m limbs with the top half
+ * of each operand zero (cuda/helper.h:93-114, ourMkRandom<m,nz>). */
+midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a,
+                              const uint64_t* d_b, uint64_t* d_r, void* stream);
+
+/* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance.
+ * Replaces polyKerQ (cuda/ker-classic-mul.cu.h:500-575). Uses 4*n*m limbs
+ * of context scratch. */
+midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                          uint64_t* d_r, void* stream);
+
+/* r = a * b per instance (2m limbs out) through a Goldilocks NTT on 16-bit
+ * coefficients; exact. Replaces bmulFFT<P,M,Q> (cuda/ker-fft-mul.cu.h:
+ * 427-454) and gpuMulFFT's twiddle setup (cuda/main.cu:444-476). */
+midint_status midint_mul_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a,
+                             const uint64_t* d_b, uint64_t* d_r, void* stream);
+
+/* Raw cyclic convolution of n instances of 2^lg values (each < p =
+ * 2^64 - 2^32 + 1): r = INTT(NTT(a) . NTT(b)). Replaces bmulFFTvalid
+ * (cuda/ker-fft-mul.cu.h:512-554), the kernel the reference's FFT
+ * known-answer test (cuda/main.cu:631-655) checks. */
+midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const uint64_t* d_a,
+                              const uint64_t* d_b, uint64_t* d_r, void* stream);
+
+/* Precomputes the NTT twiddle tables for operands of m limbs (otherwise done
+ * on first use). */
+midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);
+
+/* ------------------------------------------------- host-pointer drop-ins */
+/* Mirror the reference host drivers: allocate device buffers, copy inputs,
+ * run the operation `runs` times back to back (after one untimed run),
+ * copy the result back and free. *usec (optional) receives the mean time of
+ * one run in microseconds, measured with HIP events.
+ *   midint_gpu_add_host      <- gpuAdd<Base,m>      (cuda/main.cu:39-162)
+ *   midint_gpu_mul_host      <- gpuMultiply<Base,m> (cuda/main.cu:221-356);
+ *                               full != 0 selects the 2m-limb product
+ *   midint_gpu_mul_fft_host  <- gpuMulFFT<P,m>      (cuda/main.cu:415-579),
+ *                               2m-limb exact product */
+midint_status midint_gpu_add_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                  uint64_t* h_r, uint32_t runs, double* usec);
+midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, const uint64_t* h_a,
+                                  const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
+midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
+                                      const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
+
+/* ------------------------------------------------------ microbenchmark */
+/* Issue throughput of the VALU operations the kernels are built from,
+ * measured on `device`: 32x32->64 multiply-adds per second (the price of
+ * one partial product) and 32-bit adds per second. Replaces the reference's
+ * basic-op latency kernels testBasicOps / multiplyKer
+ * (cuda/ker-helpers.cu.h:189-317). Synchronous. */
+midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_per_s);
+
+/* ------------------------------------------------------ data generation */
+/* xoshiro256** stream (state seeded by four splitmix64 draws of `seed`)
+ * filling `count` limbs; the generator of the benchmark workloads. */
+void midint_gen_limbs(uint64_t seed, uint64_t* out, size_t count);
+/* The reference's ourMkRandom<m,nz> (cuda/helper.h:99-114) with glibc
+ * srand(seed)/rand(): m_u32 words per instance, first nz random, rest 0. */
+void midint_gen_ref_rand(uint32_t seed, size_t num_instances, size_t m_u32, size_t nz, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MIDINT_H */

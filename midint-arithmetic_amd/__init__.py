@@ -1,0 +1,239 @@
+"""Python view of libmidint's C ABI (include/midint.h).
+
+This is a thin ctypes layer used by the test-suite and bench.py; the product
+is the C ABI itself. Device buffers are torch tensors (dtype int64/uint64,
+contiguous, on a HIP device): torch provides allocation and streams only.
+
+The directory name is not a valid Python identifier; load it with
+``load_package()`` from ``__graft_entry__`` or ``importlib``:
+
+    pkg = importlib.util.spec_from_file_location(
+        "midint_arithmetic_amd", ".../midint-arithmetic_amd/__init__.py",
+        submodule_search_locations=[".../midint-arithmetic_amd"])
+
+Every call goes to the HIP kernels; there is no CPU fallback. If the shared
+library is missing, ``lib()`` raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmidint.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "midint.h")
+
+OK = 0
+STATUS = {0: "OK", 1: "EINVAL", 2: "EUNSUPPORTED", 3: "EHIP", 4: "ENOMEM", 5: "EBUSY", 6: "ENOTFOUND"}
+
+_lib = None
+_u64p = ctypes.POINTER(ctypes.c_uint64)
+_u32p = ctypes.POINTER(ctypes.c_uint32)
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_int = ctypes.c_int
+_dblp = ctypes.POINTER(ctypes.c_double)
+
+_SIGS = {
+    "midint_strerror": (ctypes.c_char_p, [_int]),
+    "midint_version": (ctypes.c_char_p, []),
+    "midint_last_hip_error": (_int, []),
+    "midint_init": (_int, [ctypes.POINTER(_vp), _int]),
+    "midint_destroy": (_int, [_vp]),
+    "midint_max_limbs_add": (_u32, []),
+    "midint_max_limbs_mul": (_u32, []),
+    "midint_max_limbs_ntt": (_u32, []),
+    "midint_add": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "midint_add6": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_lo": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_full": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_poly": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_ntt_conv": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_ntt_prepare": (_int, [_vp, _u32]),
+    "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_mul_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_probe_valu": (_int, [_int, _dblp, _dblp]),
+    "midint_gen_limbs": (None, [ctypes.c_uint64, _vp, ctypes.c_size_t]),
+    "midint_gen_ref_rand": (None, [_u32, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _vp]),
+}
+
+
+class MidintError(RuntimeError):
+    def __init__(self, status: int, what: str):
+        self.status = status
+        name = STATUS.get(status, str(status))
+        extra = ""
+        if status == 3 and _lib is not None:
+            extra = f" (hip error {_lib.midint_last_hip_error()})"
+        super().__init__(f"{what}: {name}{extra}")
+
+
+def lib():
+    """The loaded libmidint.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it first (make -C {HERE})")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def declared_symbols(header: str = HEADER):
+    """Function names declared in include/midint.h."""
+    text = open(header).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(midint_[a-z0-9_]+)\s*\(", text)))
+
+
+def check(status: int, what: str) -> None:
+    if status != OK:
+        raise MidintError(status, what)
+
+
+def _ptr(t):
+    if t is None:
+        return None
+    if hasattr(t, "data_ptr"):
+        if not t.is_contiguous():
+            raise ValueError("tensor must be contiguous")
+        return ctypes.c_void_p(t.data_ptr())
+    return ctypes.c_void_p(t.ctypes.data)
+
+
+def _stream(stream):
+    if stream is None:
+        try:
+            import torch
+
+            return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+        except Exception:  # pragma: no cover - no torch / no device
+            return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)
+
+
+class Context:
+    """Owns a midint_ctx (device binding, NTT twiddle cache, scratch)."""
+
+    def __init__(self, device: int = 0):
+        self._h = ctypes.c_void_p()
+        check(lib().midint_init(ctypes.byref(self._h), device), "midint_init")
+        self.device = device
+
+    def close(self):
+        if self._h:
+            lib().midint_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ------------------------------------------------------------ ops
+    def add(self, a, b, r, m, carry=None, stream=None):
+        n = a.numel() // m
+        check(lib().midint_add(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _ptr(carry), _stream(stream)),
+              "midint_add")
+
+    def add6(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_add6(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_add6")
+
+    def mul_lo(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_mul_lo(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_mul_lo")
+
+    def mul_full(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_mul_full(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_mul_full")
+
+    def poly(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_poly(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_poly")
+
+    def mul_ntt(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_mul_ntt(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_mul_ntt")
+
+    def ntt_conv(self, a, b, r, lg, stream=None):
+        n = a.numel() >> lg
+        check(lib().midint_ntt_conv(self._h, n, lg, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_ntt_conv")
+
+    def ntt_prepare(self, m):
+        check(lib().midint_ntt_prepare(self._h, m), "midint_ntt_prepare")
+
+    @property
+    def handle(self):
+        return self._h
+
+
+# ---------------------------------------------------------------- host side
+def gen_limbs(seed: int, count: int):
+    import numpy as np
+
+    out = np.empty(count, dtype=np.uint64)
+    lib().midint_gen_limbs(seed, _ptr(out), count)
+    return out
+
+
+def gen_ref_rand(seed: int, n: int, m_u32: int, nz: int):
+    import numpy as np
+
+    out = np.empty(n * m_u32, dtype=np.uint32)
+    lib().midint_gen_ref_rand(seed, n, m_u32, nz, _ptr(out))
+    return out
+
+
+def probe_valu(device=0):
+    """(v_mad_u64_u32 per second, 32-bit adds per second) measured on device."""
+    mad = ctypes.c_double(0.0)
+    add = ctypes.c_double(0.0)
+    check(lib().midint_probe_valu(device, ctypes.byref(mad), ctypes.byref(add)), "midint_probe_valu")
+    return mad.value, add.value
+
+
+def gpu_add_host(a, b, m, runs=0, device=0):
+    """Host-pointer drop-in of the reference gpuAdd (cuda/main.cu:39-162)."""
+    import numpy as np
+
+    r = np.empty_like(a)
+    us = ctypes.c_double(0.0)
+    check(lib().midint_gpu_add_host(device, a.size // m, m, _ptr(a), _ptr(b), _ptr(r), runs, ctypes.byref(us)),
+          "midint_gpu_add_host")
+    return r, us.value
+
+
+def gpu_mul_host(a, b, m, full=True, runs=0, device=0):
+    """Host-pointer drop-in of the reference gpuMultiply (cuda/main.cu:221-356)."""
+    import numpy as np
+
+    r = np.empty(a.size * (2 if full else 1), dtype=np.uint64)
+    us = ctypes.c_double(0.0)
+    check(lib().midint_gpu_mul_host(device, a.size // m, m, int(full), _ptr(a), _ptr(b), _ptr(r), runs,
+                                    ctypes.byref(us)), "midint_gpu_mul_host")
+    return r, us.value
+
+
+def gpu_mul_fft_host(a, b, m, runs=0, device=0):
+    """Host-pointer drop-in of the reference gpuMulFFT (cuda/main.cu:415-579)."""
+    import numpy as np
+
+    r = np.empty(a.size * 2, dtype=np.uint64)
+    us = ctypes.c_double(0.0)
+    check(lib().midint_gpu_mul_fft_host(device, a.size // m, m, _ptr(a), _ptr(b), _ptr(r), runs,
+                                        ctypes.byref(us)), "midint_gpu_mul_fft_host")
+    return r, us.value

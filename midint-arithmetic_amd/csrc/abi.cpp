@@ -1,0 +1,336 @@
+// C ABI of libmidint (declared in include/midint.h): argument checking,
+// per-size dispatch to the kernels, the context (twiddle cache, scratch) and
+// the host-pointer drop-ins that mirror the reference's gpuAdd / gpuMultiply /
+// gpuMulFFT drivers (cuda/main.cu:39-579).
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+
+#include "midint.h"
+#include "midint_internal.h"
+
+using namespace midint;
+
+struct midint_ctx {
+    int device;
+    std::mutex mu;
+    std::map<uint32_t, NttTables> ntt;  // keyed by log2 length
+    uint64_t* scratch = nullptr;
+    size_t scratch_limbs = 0;
+};
+
+static thread_local int g_last_hip = 0;
+
+static midint_status hip_status(hipError_t e) {
+    if (e == hipSuccess) return MIDINT_OK;
+    g_last_hip = (int)e;
+    return MIDINT_EHIP;
+}
+
+static midint_status bind_device(const midint_ctx* ctx) {
+    int cur = -1;
+    if (hipGetDevice(&cur) != hipSuccess) return hip_status(hipGetLastError());
+    if (cur != ctx->device) return hip_status(hipSetDevice(ctx->device));
+    return MIDINT_OK;
+}
+
+static bool aligned8(const void* p) { return (((uintptr_t)p) & 7u) == 0; }
+
+extern "C" {
+
+const char* midint_strerror(midint_status s) {
+    switch (s) {
+        case MIDINT_OK: return "success";
+        case MIDINT_EINVAL: return "invalid argument";
+        case MIDINT_EUNSUPPORTED: return "operand size not supported";
+        case MIDINT_EHIP: return "HIP runtime error";
+        case MIDINT_ENOMEM: return "out of memory";
+        case MIDINT_EBUSY: return "resource busy";
+        case MIDINT_ENOTFOUND: return "not found";
+    }
+    return "unknown status";
+}
+
+const char* midint_version(void) { return "midint-mi355x 0.1 (gfx950)"; }
+
+int midint_last_hip_error(void) { return g_last_hip; }
+
+midint_status midint_init(midint_ctx** out, int device) {
+    if (!out) return MIDINT_EINVAL;
+    *out = nullptr;
+    int count = 0;
+    hipError_t e = hipGetDeviceCount(&count);
+    if (e != hipSuccess) return hip_status(e);
+    if (device < 0 || device >= count) return MIDINT_EINVAL;
+    if ((e = hipSetDevice(device)) != hipSuccess) return hip_status(e);
+    midint_ctx* c = new (std::nothrow) midint_ctx;
+    if (!c) return MIDINT_ENOMEM;
+    c->device = device;
+    *out = c;
+    return MIDINT_OK;
+}
+
+midint_status midint_destroy(midint_ctx* ctx) {
+    if (!ctx) return MIDINT_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    for (auto& kv : ctx->ntt) {
+        (void)hipFree(kv.second.d_fwd);
+        (void)hipFree(kv.second.d_inv);
+    }
+    if (ctx->scratch) (void)hipFree(ctx->scratch);
+    delete ctx;
+    return MIDINT_OK;
+}
+
+uint32_t midint_max_limbs_add(void) { return 16u * 16u * 128u; }
+uint32_t midint_max_limbs_mul(void) { return 2048u; }
+uint32_t midint_max_limbs_ntt(void) { return (1u << kNttMaxLg) / 8u; }
+
+static midint_status check_binary(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
+                                  const uint64_t* r) {
+    if (!ctx || m == 0) return MIDINT_EINVAL;
+    if (n == 0) return MIDINT_OK;
+    if (!a || !b || !r) return MIDINT_EINVAL;
+    if (!aligned8(a) || !aligned8(b) || !aligned8(r)) return MIDINT_EINVAL;
+    return MIDINT_OK;
+}
+
+midint_status midint_add(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                         uint64_t* d_r, uint8_t* d_carry, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    return hip_status(launch_add(n, m, d_a, d_b, d_r, d_carry, false, (hipStream_t)stream));
+}
+
+midint_status midint_add6(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                          uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    return hip_status(launch_add(n, m, d_a, d_b, d_r, nullptr, true, (hipStream_t)stream));
+}
+
+midint_status midint_mul_lo(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                            uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    return hip_status(launch_mul(n, m, d_a, d_b, d_r, false, (hipStream_t)stream));
+}
+
+midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a,
+                              const uint64_t* d_b, uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    return hip_status(launch_mul(n, m, d_a, d_b, d_r, true, (hipStream_t)stream));
+}
+
+static midint_status ensure_scratch(midint_ctx* ctx, size_t limbs) {
+    if (ctx->scratch_limbs >= limbs) return MIDINT_OK;
+    if (ctx->scratch) {
+        hipError_t e = hipFree(ctx->scratch);
+        ctx->scratch = nullptr;
+        ctx->scratch_limbs = 0;
+        if (e != hipSuccess) return hip_status(e);
+    }
+    hipError_t e = hipMalloc((void**)&ctx->scratch, limbs * sizeof(uint64_t));
+    if (e != hipSuccess) {
+        ctx->scratch = nullptr;
+        (void)hipGetLastError();
+        return MIDINT_ENOMEM;
+    }
+    ctx->scratch_limbs = limbs;
+    return MIDINT_OK;
+}
+
+midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                          uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
+    return hip_status(launch_poly(n, m, d_a, d_b, d_r, ctx->scratch, (hipStream_t)stream));
+}
+
+// Returns the (cached) twiddle tables of length 2^lg.
+static midint_status get_tables(midint_ctx* ctx, uint32_t lg, NttTables* out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    auto it = ctx->ntt.find(lg);
+    if (it != ctx->ntt.end()) {
+        *out = it->second;
+        return MIDINT_OK;
+    }
+    const size_t half = (size_t)1 << (lg - 1);
+    uint64_t* hf = (uint64_t*)malloc(half * sizeof(uint64_t));
+    uint64_t* hi = (uint64_t*)malloc(half * sizeof(uint64_t));
+    if (!hf || !hi) {
+        free(hf);
+        free(hi);
+        return MIDINT_ENOMEM;
+    }
+    NttTables t;
+    t.lg = lg;
+    ntt_host_tables(lg, hf, hi, &t.inv_len);
+    hipError_t e = hipMalloc((void**)&t.d_fwd, half * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&t.d_inv, half * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpy(t.d_fwd, hf, half * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.d_inv, hi, half * sizeof(uint64_t), hipMemcpyHostToDevice);
+    free(hf);
+    free(hi);
+    if (e != hipSuccess) return hip_status(e);
+    ctx->ntt[lg] = t;
+    *out = t;
+    return MIDINT_OK;
+}
+
+midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m) {
+    if (!ctx || m == 0) return MIDINT_EINVAL;
+    if (m > midint_max_limbs_ntt()) return MIDINT_EUNSUPPORTED;
+    midint_status st = bind_device(ctx);
+    if (st != MIDINT_OK) return st;
+    NttTables t;
+    return get_tables(ctx, ntt_lg_for_limbs(m), &t);
+}
+
+midint_status midint_mul_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a,
+                             const uint64_t* d_b, uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_ntt()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    NttTables t;
+    if ((st = get_tables(ctx, ntt_lg_for_limbs(m), &t)) != MIDINT_OK) return st;
+    return hip_status(launch_mul_ntt(n, m, d_a, d_b, d_r, t, (hipStream_t)stream));
+}
+
+midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const uint64_t* d_a,
+                              const uint64_t* d_b, uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, 1, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (lg < 6 || lg > kNttMaxLg) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    NttTables t;
+    if ((st = get_tables(ctx, lg, &t)) != MIDINT_OK) return st;
+    return hip_status(launch_ntt_conv(n, d_a, d_b, d_r, t, (hipStream_t)stream));
+}
+
+// ----------------------------------------------------------- host drop-ins
+enum class HostOp { Add, MulLo, MulFull, MulNtt };
+
+static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, const uint64_t* h_a,
+                              const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
+    if (m == 0 || (n > 0 && (!h_a || !h_b || !h_r))) return MIDINT_EINVAL;
+    if (n == 0) return MIDINT_OK;
+    midint_ctx* ctx = nullptr;
+    midint_status st = midint_init(&ctx, device);
+    if (st != MIDINT_OK) return st;
+    const size_t in_bytes = (size_t)n * m * sizeof(uint64_t);
+    const size_t out_limbs = (op == HostOp::Add || op == HostOp::MulLo) ? (size_t)n * m : (size_t)2 * n * m;
+    uint64_t *d_a = nullptr, *d_b = nullptr, *d_r = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t he = hipMalloc((void**)&d_a, in_bytes);
+    if (he == hipSuccess) he = hipMalloc((void**)&d_b, in_bytes);
+    if (he == hipSuccess) he = hipMalloc((void**)&d_r, out_limbs * sizeof(uint64_t));
+    if (he == hipSuccess) he = hipMemcpy(d_a, h_a, in_bytes, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipMemcpy(d_b, h_b, in_bytes, hipMemcpyHostToDevice);
+    if (he == hipSuccess) he = hipEventCreate(&e0);
+    if (he == hipSuccess) he = hipEventCreate(&e1);
+    if (he != hipSuccess) st = hip_status(he);
+
+    auto once = [&]() -> midint_status {
+        switch (op) {
+            case HostOp::Add: return midint_add(ctx, n, m, d_a, d_b, d_r, nullptr, nullptr);
+            case HostOp::MulLo: return midint_mul_lo(ctx, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::MulFull: return midint_mul_full(ctx, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::MulNtt: return midint_mul_ntt(ctx, n, m, d_a, d_b, d_r, nullptr);
+        }
+        return MIDINT_EINVAL;
+    };
+    if (st == MIDINT_OK) st = once();  // dry run (cuda/main.cu:80-85)
+    if (st == MIDINT_OK) st = hip_status(hipDeviceSynchronize());
+    if (st == MIDINT_OK && runs > 0) {
+        st = hip_status(hipEventRecord(e0, nullptr));
+        for (uint32_t i = 0; st == MIDINT_OK && i < runs; i++) st = once();
+        if (st == MIDINT_OK) st = hip_status(hipEventRecord(e1, nullptr));
+        if (st == MIDINT_OK) st = hip_status(hipEventSynchronize(e1));
+        float ms = 0.f;
+        if (st == MIDINT_OK) st = hip_status(hipEventElapsedTime(&ms, e0, e1));
+        if (st == MIDINT_OK && usec) *usec = 1000.0 * ms / runs;
+    }
+    if (st == MIDINT_OK) st = hip_status(hipMemcpy(h_r, d_r, out_limbs * sizeof(uint64_t), hipMemcpyDeviceToHost));
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_a);
+    (void)hipFree(d_b);
+    (void)hipFree(d_r);
+    midint_destroy(ctx);
+    return st;
+}
+
+midint_status midint_gpu_add_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                  uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, HostOp::Add, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, const uint64_t* h_a,
+                                  const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, full ? HostOp::MulFull : HostOp::MulLo, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
+                                      const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, HostOp::MulNtt, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+// ----------------------------------------------------------- generators
+static uint64_t splitmix64(uint64_t* s) {
+    uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+static inline uint64_t rotl64(uint64_t x, int k) { return (x << k) | (x >> (64 - k)); }
+
+void midint_gen_limbs(uint64_t seed, uint64_t* out, size_t count) {
+    uint64_t sm = seed;
+    uint64_t s0 = splitmix64(&sm), s1 = splitmix64(&sm), s2 = splitmix64(&sm), s3 = splitmix64(&sm);
+    for (size_t i = 0; i < count; i++) {
+        out[i] = rotl64(s1 * 5, 7) * 9;
+        const uint64_t t = s1 << 17;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = rotl64(s3, 45);
+    }
+}
+
+void midint_gen_ref_rand(uint32_t seed, size_t num_instances, size_t m_u32, size_t nz, uint32_t* out) {
+    srand(seed);
+    for (size_t i = 0; i < num_instances; i++)
+        for (size_t k = 0; k < m_u32; k++) {
+            uint32_t v = 0;
+            if (k < nz) {
+                const uint32_t lo = (uint32_t)rand() * 2u;
+                const uint32_t hi = (uint32_t)rand() * 2u;
+                v = (hi << 16) + lo;
+            }
+            out[i * m_u32 + k] = v;
+        }
+}
+
+}  // extern "C"

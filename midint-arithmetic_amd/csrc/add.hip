@@ -1,0 +1,247 @@
+// Batched big-integer addition kernels for gfx950.
+//
+// add_kernel  : r = (a + b) mod B^m per instance    (reference baddKer,
+//               cuda/ker-addition.cu.h:227-260)
+// add6_kernel : r = (6a + 10b) mod B^m per instance (reference a6pb10Ker,
+//               cuda/ker-addition.cu.h:266-328), the same chain of six
+//               register-resident additions.
+//
+// A block owns `ipb` whole instances (a window of ipb*m limbs); the window is
+// split into rows of 128 limbs, R rows per wave. Loads and stores are 16-byte
+// per lane and contiguous per wave when the window is 16-byte aligned (VEC),
+// 8-byte otherwise.
+#include "add_core.hpp"
+#include "midint_internal.h"
+
+namespace midint {
+
+template <bool VEC>
+__device__ __forceinline__ void load_pair(const uint64_t* __restrict__ p, uint32_t k, uint32_t L,
+                                          uint64_t& v0, uint64_t& v1) {
+    if (VEC && k + 1 < L) {
+        ulonglong2 t = *reinterpret_cast<const ulonglong2*>(p + k);
+        v0 = t.x;
+        v1 = t.y;
+    } else {
+        v0 = k < L ? p[k] : 0ull;
+        v1 = k + 1 < L ? p[k + 1] : 0ull;
+    }
+}
+
+template <bool VEC>
+__device__ __forceinline__ void store_pair(uint64_t* __restrict__ p, uint32_t k, uint32_t L, uint64_t v0,
+                                           uint64_t v1) {
+    if (VEC && k + 1 < L) {
+        ulonglong2 t;
+        t.x = v0;
+        t.y = v1;
+        *reinterpret_cast<ulonglong2*>(p + k) = t;
+    } else {
+        if (k < L) p[k] = v0;
+        if (k + 1 < L) p[k + 1] = v1;
+    }
+}
+
+// Loads both operands of the block window into rows and marks segment starts.
+template <int R, bool VEC>
+__device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restrict__ a,
+                                            const uint64_t* __restrict__ b, uint32_t L, uint32_t m,
+                                            uint32_t step_q, uint32_t step_r, uint32_t* off0,
+                                            uint32_t* inst0) {
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t k_first = wave * R * kRowLimbs + 2 * lane_id();
+    SegPos sp = segpos_init(k_first, m);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        load_pair<VEC>(a, k, L, t.x0[v], t.x1[v]);
+        load_pair<VEC>(b, k, L, t.y0[v], t.y1[v]);
+        t.s0[v] = (sp.off == 0) && (k < L);
+        t.s1[v] = (sp.off + 1 == m) && (k + 1 < L);
+        off0[v] = sp.off;
+        inst0[v] = sp.inst;
+        segpos_next(&sp, m, step_q, step_r);
+    }
+}
+
+template <int R, bool VEC>
+__global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
+                                                   uint32_t step_r, const uint64_t* __restrict__ a,
+                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                   uint8_t* __restrict__ carry) {
+    __shared__ uint32_t lds_gp[16];
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+    const uint32_t L = ninst * m;
+    const uint64_t base = inst_base * m;
+
+    Rows<R> t;
+    uint32_t off0[R], inst0[R];
+    load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+
+    uint64_t ax0[R], ax1[R];
+    if (carry) {
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            ax0[v] = t.x0[v];
+            ax1[v] = t.x1[v];
+        }
+    }
+
+    block_add_rows<R>(t, lds_gp);
+
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int v = 0; v < R; v++) store_pair<VEC>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
+
+    if (carry) {
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t k = k_first + v * kRowLimbs;
+            if (k < L && off0[v] == m - 1) {
+                uint64_t pre = t.x0[v] - (uint64_t)t.c0[v];
+                carry[inst_base + inst0[v]] = (uint8_t)limb_cout(ax0[v], pre, t.c0[v]);
+            }
+            const uint32_t off1 = (off0[v] + 1 == m) ? 0u : off0[v] + 1;
+            const uint32_t i1 = inst0[v] + ((off0[v] + 1 == m) ? 1u : 0u);
+            if (k + 1 < L && off1 == m - 1) {
+                uint64_t pre = t.x1[v] - (uint64_t)t.c1[v];
+                carry[inst_base + i1] = (uint8_t)limb_cout(ax1[v], pre, t.c1[v]);
+            }
+        }
+    }
+}
+
+template <int R, bool VEC>
+__global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
+                                                    uint32_t step_r, const uint64_t* __restrict__ a,
+                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r) {
+    __shared__ uint32_t lds_gp[16];
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+    const uint32_t L = ninst * m;
+    const uint64_t base = inst_base * m;
+
+    Rows<R> t;
+    uint32_t off0[R], inst0[R];
+    load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+    uint64_t b0[R], b1[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        b0[v] = t.y0[v];
+        b1[v] = t.y1[v];
+    }
+    uint64_t u0[R], u1[R];
+    // t1 = a + b
+    block_add_rows<R>(t, lds_gp);
+    // t2 = t1 + t1
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        t.y0[v] = t.x0[v];
+        t.y1[v] = t.x1[v];
+    }
+    block_add_rows<R>(t, lds_gp);
+    // t3 = t2 + b
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        t.y0[v] = b0[v];
+        t.y1[v] = b1[v];
+    }
+    block_add_rows<R>(t, lds_gp);
+    // t4 = t3 + t3 (keep t3)
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        u0[v] = t.x0[v];
+        u1[v] = t.x1[v];
+        t.y0[v] = t.x0[v];
+        t.y1[v] = t.x1[v];
+    }
+    block_add_rows<R>(t, lds_gp);
+    // t5 = t4 + t3
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        t.y0[v] = u0[v];
+        t.y1[v] = u1[v];
+    }
+    block_add_rows<R>(t, lds_gp);
+    // r = t5 + b
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        t.y0[v] = b0[v];
+        t.y1[v] = b1[v];
+    }
+    block_add_rows<R>(t, lds_gp);
+
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int v = 0; v < R; v++) store_pair<VEC>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
+}
+
+// ---------------------------------------------------------------------------
+// Host-side launch geometry.
+
+// Chooses rows per wave R and waves per block so that one block holds at
+// least one whole instance; small instances are packed `ipb` per block.
+AddGeom add_geometry(uint32_t m) {
+    AddGeom g;
+    if (m <= 256) {
+        g.rows = 2;
+        g.threads = 256;
+    } else {
+        g.rows = 0;
+        const uint32_t rows_opts[] = {2, 4, 8, 16};
+        for (uint32_t R : rows_opts) {
+            uint32_t waves = (m + R * kRowLimbs - 1) / (R * kRowLimbs);
+            if (waves <= 16) {
+                g.rows = R;
+                g.threads = (waves < 4 ? 4 : waves) * kWave;
+                break;
+            }
+        }
+        if (g.rows == 0) {
+            g.threads = 0;
+            g.ipb = 0;
+            return g;
+        }
+    }
+    uint32_t cap = (g.threads / kWave) * g.rows * kRowLimbs;
+    g.ipb = cap / m;
+    if ((m & 1u) && g.ipb > 1) g.ipb &= ~1u;  // keep windows 16-byte aligned
+    return g;
+}
+
+template <int R>
+static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n, uint32_t m, const uint64_t* a,
+                               const uint64_t* b, uint64_t* r, uint8_t* carry, hipStream_t s) {
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + g.ipb - 1) / g.ipb);
+    const uint32_t sq = kRowLimbs / m, sr = kRowLimbs % m;
+    if (six) {
+        if (vec)
+            add6_kernel<R, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r);
+        else
+            add6_kernel<R, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r);
+    } else {
+        if (vec)
+            add_kernel<R, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+        else
+            add_kernel<R, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, uint8_t* carry,
+                      bool six, hipStream_t s) {
+    AddGeom g = add_geometry(m);
+    if (g.ipb == 0) return hipErrorInvalidValue;
+    const bool aligned = ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 15u) == 0;
+    const bool vec = aligned && ((uint64_t)g.ipb * m) % 2 == 0;
+    switch (g.rows) {
+        case 2: return launch_add_r<2>(g, vec, six, n, m, a, b, r, carry, s);
+        case 4: return launch_add_r<4>(g, vec, six, n, m, a, b, r, carry, s);
+        case 8: return launch_add_r<8>(g, vec, six, n, m, a, b, r, carry, s);
+        case 16: return launch_add_r<16>(g, vec, six, n, m, a, b, r, carry, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+}  // namespace midint

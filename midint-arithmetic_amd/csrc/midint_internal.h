@@ -1,0 +1,48 @@
+// Internal host-side interface between the C ABI (abi.cpp) and the kernel
+// translation units. Not installed; the public interface is include/midint.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace midint {
+
+struct AddGeom {
+    uint32_t rows;     // rows of 128 limbs per wave
+    uint32_t threads;  // threads per block
+    uint32_t ipb;      // instances per block (0: size unsupported)
+};
+AddGeom add_geometry(uint32_t m);
+
+// r = (a + b) mod B^m (six == false) or (6a + 10b) mod B^m (six == true).
+hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                      uint8_t* carry, bool six, hipStream_t s);
+
+// Largest operand size (limbs) of the classical multiplication kernels.
+constexpr uint32_t kMulMaxLimbs = 4096;
+// full: r[0..2m) = a*b ; otherwise r[0..m) = a*b mod B^m.
+hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                      hipStream_t s);
+// r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ).
+hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                       uint64_t* scratch, hipStream_t s);
+
+// NTT multiplication.
+struct NttTables {
+    uint32_t lg;          // log2 of the transform length
+    uint64_t* d_fwd;      // w^i, i < L/2
+    uint64_t* d_inv;      // w^-i, i < L/2
+    uint64_t inv_len;     // L^-1 mod p
+};
+constexpr uint32_t kNttMaxLg = 14;  // 16384-point transform: 128 KiB of LDS
+uint32_t ntt_lg_for_limbs(uint32_t m);
+// Host computation of twiddle tables (allocated by the caller).
+void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_len);
+hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                          const NttTables& t, hipStream_t s);
+// Raw cyclic convolution of n instances of L = 2^lg values (< p each).
+hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
+                           hipStream_t s);
+
+}  // namespace midint

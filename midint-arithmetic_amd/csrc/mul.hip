@@ -1,0 +1,310 @@
+// Classical (schoolbook) batched multiplication for gfx950.
+//
+// mul_kernel<FULL=true>  : r[0..2m) = a*b             (full product)
+// mul_kernel<FULL=false> : r[0..m)  = a*b mod B^m     (reference bmulKerQ,
+//                          cuda/ker-classic-mul.cu.h:468-498)
+//
+// Operands are viewed as n = 2m u32 words (zero-padded to n_pad, a multiple
+// of 2Q). Column c of the product is sum_{i+j=c} a_i b_j. As in the
+// reference (wrapperConvQ, cuda/ker-classic-mul.cu.h:406-423) each lane owns
+// two groups of Q consecutive columns chosen so that their term counts add up
+// to the same total for every lane. Unlike the reference, the two groups are
+// walked in ONE loop of equal trip count for all lanes: lane t works on its
+// first group for blocks 0..t and on its second group afterwards, so no lane
+// of a wavefront idles while another finishes a longer column range. At the
+// switch the lane parks its accumulators in LDS (one masked LDS store) and
+// keeps accumulating; the second group is recovered by subtraction.
+//
+// Each loop block processes Q a-words x Q columns = Q^2 32x32->64 products
+// from one 16-byte broadcast LDS read of a and two 16-byte LDS reads of a
+// sliding window of b (lane-contiguous, bank-conflict free). Column sums are
+// 64-bit accumulators plus a 32-bit overflow count (the reference's
+// computeIter64, :11-26, at u32 granularity).
+//
+// Column sums are split into low words L and high words H shifted by one
+// group (reference from4Reg2ShmQ, :299-350) and normalised with one
+// carry-propagating addition L + H (reference baddRegs, :462) done by the
+// wave-ballot row adder.
+#include "add_core.hpp"
+#include "midint_internal.h"
+
+namespace midint {
+
+constexpr uint32_t kMulQ = 4;
+
+struct MulLayout {
+    uint32_t n_pad;   // padded operand length in u32 words (multiple of 2Q)
+    uint32_t T;       // lanes per instance
+    uint32_t P;       // product limbs computed per instance
+    uint32_t words;   // LDS u32 words per instance slot (conv phase)
+    uint32_t ipb;     // instances per block
+    uint32_t threads; // block size
+};
+
+__host__ __device__ inline uint32_t mul_npad(uint32_t m) {
+    uint32_t n = 2 * m;
+    return (n + 2 * kMulQ - 1) / (2 * kMulQ) * (2 * kMulQ);
+}
+
+template <bool FULL>
+__global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
+                                                   uint32_t P, uint32_t words, uint32_t ipb, uint32_t out_len,
+                                                   const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
+                                                   uint64_t* __restrict__ r) {
+    constexpr uint32_t Q = kMulQ;
+    extern __shared__ __align__(16) uint64_t lds64[];
+    uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+    __shared__ uint32_t lds_gp[16];
+
+    const uint32_t tid = threadIdx.x;
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+
+    // ---- stage operands into LDS: A[n_pad], B[(Q-1) + j] with zero pads
+    const uint32_t zero_words = ipb * words;
+    for (uint32_t f = tid; f < zero_words; f += blockDim.x) lds[f] = 0u;
+    __syncthreads();
+    {
+        const uint32_t total = ninst * m;
+        const uint64_t* ga = a + inst_base * m;
+        const uint64_t* gb = b + inst_base * m;
+        for (uint32_t f = tid; f < total; f += blockDim.x) {
+            const uint32_t s = f / m;
+            const uint32_t i = f - s * m;
+            const uint64_t va = ga[f], vb = gb[f];
+            uint32_t* A = lds + s * words;
+            uint32_t* Bw = A + n_pad + (Q - 1);
+            A[2 * i] = (uint32_t)va;
+            A[2 * i + 1] = (uint32_t)(va >> 32);
+            Bw[2 * i] = (uint32_t)vb;
+            Bw[2 * i + 1] = (uint32_t)(vb >> 32);
+        }
+    }
+    __syncthreads();
+
+    // ---- convolution
+    const uint32_t slot = tid / T;
+    const uint32_t t = tid - slot * T;
+    const bool active = slot < ipb;
+
+    uint64_t acc[Q];
+    uint32_t ovf[Q];
+    uint64_t sacc[Q];
+    uint32_t sovf[Q];
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        acc[q] = 0;
+        ovf[q] = 0;
+    }
+    uint32_t k1 = Q * t, k2, i2, NB;
+    if (FULL) {
+        k2 = n_pad + Q * t;
+        i2 = Q * t;
+        NB = T + 1;
+    } else {
+        k2 = n_pad - Q - Q * t;
+        i2 = 0;
+        NB = 2 * T + 1;
+    }
+    const uint32_t sw = t + 1;
+    if (active) {
+        const uint32_t* A = lds + slot * words;
+        const uint32_t* Bw = A + n_pad;
+        uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
+        for (uint32_t beta = 0; beta < NB; beta++) {
+            uint32_t k, i0;
+            if (beta < sw) {
+                k = k1;
+                i0 = Q * beta;
+            } else {
+                k = k2;
+                i0 = i2 + Q * (beta - sw);
+            }
+            if (beta == sw) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    snap[3 * q] = (uint32_t)acc[q];
+                    snap[3 * q + 1] = (uint32_t)(acc[q] >> 32);
+                    snap[3 * q + 2] = ovf[q];
+                }
+            }
+            const uint4 av = *reinterpret_cast<const uint4*>(A + i0);
+            const uint32_t bidx = k - i0;
+            const uint4 bw0 = *reinterpret_cast<const uint4*>(Bw + bidx);
+            const uint4 bw1 = *reinterpret_cast<const uint4*>(Bw + bidx + 4);
+            const uint32_t ax[4] = {av.x, av.y, av.z, av.w};
+            const uint32_t bx[8] = {bw0.x, bw0.y, bw0.z, bw0.w, bw1.x, bw1.y, bw1.z, bw1.w};
+#pragma unroll
+            for (int u = 0; u < Q; u++) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    const uint64_t prev = acc[q];
+                    acc[q] = (uint64_t)ax[u] * (uint64_t)bx[q - u + Q - 1] + prev;
+                    ovf[q] += acc[q] < prev;
+                }
+            }
+        }
+        // group 1 parked at the switch, group 2 = total - group 1
+#pragma unroll
+        for (int q = 0; q < Q; q++) {
+            sacc[q] = (uint64_t)snap[3 * q] | ((uint64_t)snap[3 * q + 1] << 32);
+            sovf[q] = snap[3 * q + 2];
+        }
+    }
+    __syncthreads();  // everyone is done with A/B/snap; reuse LDS for L/H
+
+    uint64_t* Lb = lds64;
+    uint64_t* Hb = lds64 + ipb * P;
+    if (active) {
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            uint64_t va[Q];
+            uint32_t vo[Q];
+            uint32_t kcol;
+            if (g == 0) {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    va[q] = sacc[q];
+                    vo[q] = sovf[q];
+                }
+                kcol = k1;
+            } else {
+#pragma unroll
+                for (int q = 0; q < Q; q++) {
+                    va[q] = acc[q] - sacc[q];
+                    vo[q] = ovf[q] - sovf[q] - (acc[q] < sacc[q] ? 1u : 0u);
+                }
+                kcol = k2;
+            }
+            // group value sum_q (va + vo*2^64) * 2^(32q) -> Q low words + H
+            uint64_t lo = 0, hi = 0;  // running 128-bit value
+            uint32_t w[Q];
+#pragma unroll
+            for (int q = 0; q < Q; q++) {
+                uint64_t nlo = lo + va[q];
+                hi += (uint64_t)vo[q] + (nlo < lo ? 1ull : 0ull);
+                lo = nlo;
+                w[q] = (uint32_t)lo;
+                lo = (lo >> 32) | (hi << 32);
+                hi >>= 32;
+            }
+            const uint32_t limb = kcol / 2;
+            uint64_t* L = Lb + slot * P;
+            uint64_t* H = Hb + slot * P;
+#pragma unroll
+            for (int i = 0; i < Q / 2; i++) L[limb + i] = (uint64_t)w[2 * i] | ((uint64_t)w[2 * i + 1] << 32);
+            if (limb + Q / 2 < P) H[limb + Q / 2] = lo;
+            if (limb + Q / 2 + 1 < P) H[limb + Q / 2 + 1] = hi;
+            if (kcol == 0) {
+#pragma unroll
+                for (int i = 0; i < Q / 2; i++) H[i] = 0;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- normalisation: L + H over ipb segments of P limbs (2 rows per wave)
+    constexpr int R = 2;
+    const uint32_t Lw = ipb * P;
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<R> rows;
+    uint32_t off0[R], inst0[R];
+    {
+        SegPos sp = segpos_init(k_first, P);
+        const uint32_t sq = kRowLimbs / P, sr = kRowLimbs % P;
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t k = k_first + v * kRowLimbs;
+            rows.x0[v] = k < Lw ? Lb[k] : 0ull;
+            rows.x1[v] = k + 1 < Lw ? Lb[k + 1] : 0ull;
+            rows.y0[v] = k < Lw ? Hb[k] : 0ull;
+            rows.y1[v] = k + 1 < Lw ? Hb[k + 1] : 0ull;
+            rows.s0[v] = (sp.off == 0) && (k < Lw);
+            rows.s1[v] = (sp.off + 1 == P) && (k + 1 < Lw);
+            off0[v] = sp.off;
+            inst0[v] = sp.inst;
+            segpos_next(&sp, P, sq, sr);
+        }
+    }
+    block_add_rows<R>(rows, lds_gp);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        const uint32_t o0 = off0[v];
+        const uint32_t i0 = inst0[v];
+        if (k < Lw && i0 < ninst && o0 < out_len) r[(inst_base + i0) * out_len + o0] = rows.x0[v];
+        const bool wrap = (o0 + 1 == P);
+        const uint32_t o1 = wrap ? 0u : o0 + 1;
+        const uint32_t i1 = i0 + (wrap ? 1u : 0u);
+        if (k + 1 < Lw && i1 < ninst && o1 < out_len) r[(inst_base + i1) * out_len + o1] = rows.x1[v];
+    }
+}
+
+static bool mul_layout(uint32_t m, bool full, MulLayout* L) {
+    constexpr uint32_t Q = kMulQ;
+    if (m == 0 || m > kMulMaxLimbs) return false;
+    L->n_pad = mul_npad(m);
+    L->T = full ? L->n_pad / Q : L->n_pad / (2 * Q);
+    L->P = full ? L->n_pad : L->n_pad / 2;
+    L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
+    if (L->T <= 256) {
+        L->threads = 256;
+        L->ipb = 256 / L->T;
+    } else {
+        L->threads = (L->T + 63) / 64 * 64;
+        L->ipb = 1;
+    }
+    if (L->threads > 1024) return false;
+    // normalisation: R=2 rows of 128 limbs per wave must cover ipb*P limbs
+    if ((uint64_t)(L->threads / 64) * 2 * 128 < (uint64_t)L->ipb * L->P) return false;
+    return true;
+}
+
+static size_t mul_lds_bytes(const MulLayout& L) {
+    size_t conv = (size_t)L.ipb * L.words * 4;
+    size_t norm = (size_t)L.ipb * L.P * 16;
+    return conv > norm ? conv : norm;
+}
+
+hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                      hipStream_t s) {
+    MulLayout L;
+    if (!mul_layout(m, full, &L)) return hipErrorInvalidValue;
+    const size_t lds = mul_lds_bytes(L);
+    if (lds > 160 * 1024) return hipErrorInvalidValue;
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
+    const uint32_t out_len = full ? 2 * m : m;
+    const void* fn = full ? (const void*)mul_kernel<true> : (const void*)mul_kernel<false>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (full) {
+        mul_kernel<true><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len, a, b, r);
+    } else {
+        mul_kernel<false><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len, a, b,
+                                                         r);
+    }
+    return hipGetLastError();
+}
+
+// Reference polyKerQ (cuda/ker-classic-mul.cu.h:533-559) as a chain of
+// launches over caller-provided scratch of 4*n*m limbs.
+hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                       uint64_t* scratch, hipStream_t s) {
+    const size_t nm = (size_t)n * m;
+    uint64_t* t1 = scratch;
+    uint64_t* t2 = scratch + nm;
+    uint64_t* t3 = scratch + 2 * nm;
+    uint64_t* t4 = scratch + 3 * nm;
+    hipError_t e;
+    if ((e = launch_mul(n, m, a, a, t1, false, s)) != hipSuccess) return e;          // t1 = a^2
+    if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s)) != hipSuccess) return e; // t2 = a^2 + b
+    if ((e = launch_mul(n, m, b, b, t3, false, s)) != hipSuccess) return e;          // t3 = b^2
+    if ((e = launch_add(n, m, t3, b, t4, nullptr, false, s)) != hipSuccess) return e; // t4 = b^2 + b
+    if ((e = launch_mul(n, m, t2, t4, t1, false, s)) != hipSuccess) return e;        // t5 = t2 * t4
+    if ((e = launch_mul(n, m, a, b, t3, false, s)) != hipSuccess) return e;          // t6 = a * b
+    return launch_add(n, m, t1, t3, r, nullptr, false, s);                            // r = t5 + t6
+}
+
+}  // namespace midint

@@ -1,0 +1,256 @@
+// NTT multiplication over the Goldilocks prime p = 2^64 - 2^32 + 1 (gfx950).
+//
+// Reference: bmulFFT / bmulFftReg (cuda/ker-fft-mul.cu.h:375-454) multiply
+// one integer per block with a radix-2 NTT over a 32-bit prime and base-2^15
+// digits, then recombine with a carry pass (cuda/ker-fft-help.cu.h). That
+// path is not exact for arbitrary 16-bit digits; this one is:
+//
+//  * each operand of m u64 limbs is split into 4m 16-bit coefficients;
+//  * the linear convolution (length < 8m) is computed with an L-point NTT,
+//    L = 2^lg >= 8m, over Goldilocks -- every coefficient of the product is
+//    below 4m * 2^32 < p, so the field result is the integer result;
+//  * the forward transform is decimation-in-frequency (natural order in,
+//    bit-reversed out) and the inverse is decimation-in-time (bit-reversed
+//    in, natural out), so no bit-reversal permutation pass is needed
+//    (the reference permutes in place, permuteIP :179-189);
+//  * coefficients are recombined exactly: limb j gets
+//    c_{4j} + c_{4j+1} 2^16 + c_{4j+2} 2^32 + c_{4j+3} 2^48 (< 2^94), split
+//    into a low limb and a high part one limb up, and the two vectors are
+//    added with the wave-ballot row adder.
+//
+// One instance per block, the whole transform in LDS (8L bytes, 128 KiB at
+// L = 16384). Twiddles w^i and w^-i (i < L/2) come from a per-length table
+// built once on the host.
+#include "add_core.hpp"
+#include "midint_internal.h"
+
+namespace midint {
+
+__device__ __forceinline__ void ntt_dif(uint64_t* x, uint32_t lg, const uint64_t* __restrict__ tw) {
+    const uint32_t half = 1u << (lg - 1);
+    const uint32_t T = blockDim.x;
+    for (uint32_t s = 0; s < lg; s++) {
+        const uint32_t lgh = lg - 1 - s;
+        const uint32_t h = 1u << lgh;
+        for (uint32_t bfl = threadIdx.x; bfl < half; bfl += T) {
+            const uint32_t j = bfl & (h - 1);
+            const uint32_t blk = bfl >> lgh;
+            const uint32_t i0 = (blk << (lgh + 1)) + j;
+            const uint32_t i1 = i0 + h;
+            const uint64_t u = x[i0], v = x[i1];
+            x[i0] = gl_add(u, v);
+            x[i1] = gl_mul(gl_sub(u, v), tw[j << s]);
+        }
+        __syncthreads();
+    }
+}
+
+__device__ __forceinline__ void ntt_dit(uint64_t* x, uint32_t lg, const uint64_t* __restrict__ tw) {
+    const uint32_t half = 1u << (lg - 1);
+    const uint32_t T = blockDim.x;
+    for (uint32_t lgh = 0; lgh < lg; lgh++) {
+        const uint32_t h = 1u << lgh;
+        const uint32_t sh = lg - 1 - lgh;
+        for (uint32_t bfl = threadIdx.x; bfl < half; bfl += T) {
+            const uint32_t j = bfl & (h - 1);
+            const uint32_t blk = bfl >> lgh;
+            const uint32_t i0 = (blk << (lgh + 1)) + j;
+            const uint32_t i1 = i0 + h;
+            const uint64_t u = x[i0];
+            const uint64_t v = gl_mul(x[i1], tw[j << sh]);
+            x[i0] = gl_add(u, v);
+            x[i1] = gl_sub(u, v);
+        }
+        __syncthreads();
+    }
+}
+
+constexpr uint32_t kNttMaxPerThread = 16;  // L / blockDim for every supported L
+constexpr uint32_t kNttMaxLimbsPerThread = 4;  // 2m / blockDim, blockDim >= m/2
+
+// MODE 0: integer product (r has 2m limbs per instance).
+// MODE 1: raw cyclic convolution of L values (reference bmulFFTvalid).
+template <int MODE>
+__global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t lg, const uint64_t* __restrict__ a,
+                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                   const uint64_t* __restrict__ twf,
+                                                   const uint64_t* __restrict__ twi, uint64_t inv_len) {
+    extern __shared__ __align__(16) uint64_t x[];
+    __shared__ uint32_t lds_gp[16];
+    const uint32_t L = 1u << lg;
+    const uint32_t T = blockDim.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t E = L / T;
+
+    // ---- forward transform of a, kept in registers (positions tid + T*e)
+    const uint64_t* ga = a + (size_t)blockIdx.x * (MODE == 0 ? m : L);
+    const uint64_t* gb = b + (size_t)blockIdx.x * (MODE == 0 ? m : L);
+    for (uint32_t k = tid; k < L; k += T) {
+        uint64_t v;
+        if (MODE == 0)
+            v = (k < 4 * m) ? ((ga[k >> 2] >> (16 * (k & 3))) & 0xFFFFull) : 0ull;
+        else
+            v = ga[k];
+        x[k] = v;
+    }
+    __syncthreads();
+    ntt_dif(x, lg, twf);
+    uint64_t ah[kNttMaxPerThread];
+#pragma unroll
+    for (uint32_t e = 0; e < kNttMaxPerThread; e++)
+        if (e < E) ah[e] = x[tid + T * e];
+    __syncthreads();
+
+    // ---- forward transform of b, pointwise product (scaled by L^-1)
+    for (uint32_t k = tid; k < L; k += T) {
+        uint64_t v;
+        if (MODE == 0)
+            v = (k < 4 * m) ? ((gb[k >> 2] >> (16 * (k & 3))) & 0xFFFFull) : 0ull;
+        else
+            v = gb[k];
+        x[k] = v;
+    }
+    __syncthreads();
+    ntt_dif(x, lg, twf);
+#pragma unroll
+    for (uint32_t e = 0; e < kNttMaxPerThread; e++) {
+        if (e < E) {
+            const uint32_t pos = tid + T * e;
+            x[pos] = gl_mul(gl_mul(x[pos], ah[e]), inv_len);
+        }
+    }
+    __syncthreads();
+    ntt_dit(x, lg, twi);
+
+    if (MODE == 1) {
+        uint64_t* gr = r + (size_t)blockIdx.x * L;
+        for (uint32_t k = tid; k < L; k += T) gr[k] = x[k];
+        return;
+    }
+
+    // ---- exact recombination: limb j = sum_s c_{4j+s} 2^(16 s), split low/high
+    const uint32_t outl = 2 * m;
+    uint64_t lo_v[kNttMaxLimbsPerThread], hi_v[kNttMaxLimbsPerThread];
+#pragma unroll
+    for (uint32_t e = 0; e < kNttMaxLimbsPerThread; e++) {
+        const uint32_t j = tid + T * e;
+        uint64_t lo = 0, hi = 0;
+        if (j < outl) {
+            const uint64_t c0 = x[4 * j], c1 = x[4 * j + 1], c2 = x[4 * j + 2], c3 = x[4 * j + 3];
+            unsigned __int128 acc = (unsigned __int128)c0 + ((unsigned __int128)c1 << 16) +
+                                    ((unsigned __int128)c2 << 32) + ((unsigned __int128)c3 << 48);
+            lo = (uint64_t)acc;
+            hi = (uint64_t)(acc >> 64);
+        }
+        lo_v[e] = lo;
+        hi_v[e] = hi;
+    }
+    __syncthreads();
+    uint64_t* Lb = x;
+    uint64_t* Hb = x + outl;
+    if (tid == 0) Hb[0] = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < kNttMaxLimbsPerThread; e++) {
+        const uint32_t j = tid + T * e;
+        if (j < outl) {
+            Lb[j] = lo_v[e];
+            if (j + 1 < outl) Hb[j + 1] = hi_v[e];
+        }
+    }
+    __syncthreads();
+
+    constexpr int R = 2;
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<R> rows;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        rows.x0[v] = k < outl ? Lb[k] : 0ull;
+        rows.x1[v] = k + 1 < outl ? Lb[k + 1] : 0ull;
+        rows.y0[v] = k < outl ? Hb[k] : 0ull;
+        rows.y1[v] = k + 1 < outl ? Hb[k + 1] : 0ull;
+        rows.s0[v] = (k == 0);
+        rows.s1[v] = false;
+    }
+    block_add_rows<R>(rows, lds_gp);
+    uint64_t* gr = r + (size_t)blockIdx.x * outl;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        if (k < outl) gr[k] = rows.x0[v];
+        if (k + 1 < outl) gr[k + 1] = rows.x1[v];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Host side.
+
+static inline uint64_t h_mulmod(uint64_t a, uint64_t b) {
+    unsigned __int128 t = (unsigned __int128)a * b;
+    return (uint64_t)(t % kGlP);
+}
+static uint64_t h_pow(uint64_t a, uint64_t e) {
+    uint64_t r = 1;
+    while (e) {
+        if (e & 1) r = h_mulmod(r, a);
+        a = h_mulmod(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_len) {
+    const uint64_t L = 1ull << lg;
+    const uint64_t w = h_pow(7, (kGlP - 1) / L);  // 7 generates the multiplicative group
+    const uint64_t wi = h_pow(w, kGlP - 2);
+    uint64_t f = 1, g = 1;
+    for (uint64_t i = 0; i < L / 2; i++) {
+        fwd[i] = f;
+        inv[i] = g;
+        f = h_mulmod(f, w);
+        g = h_mulmod(g, wi);
+    }
+    *inv_len = h_pow(L, kGlP - 2);
+}
+
+uint32_t ntt_lg_for_limbs(uint32_t m) {
+    uint32_t lg = 6;
+    while ((1ull << lg) < 8ull * m) lg++;
+    return lg;
+}
+
+static uint32_t ntt_threads(uint32_t lg) {
+    uint32_t L = 1u << lg;
+    uint32_t T = L / 16;
+    if (T < 64) T = 64;
+    if (T > 1024) T = 1024;
+    return T;
+}
+
+hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                          const NttTables& t, hipStream_t s) {
+    if (t.lg > kNttMaxLg || t.lg < 6) return hipErrorInvalidValue;
+    const uint32_t T = ntt_threads(t.lg);
+    const size_t lds = ((size_t)1 << t.lg) * 8;
+    if ((1u << t.lg) / T > kNttMaxPerThread) return hipErrorInvalidValue;
+    if (2ull * m > (uint64_t)kNttMaxLimbsPerThread * T) return hipErrorInvalidValue;
+    hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    ntt_kernel<0><<<n, T, lds, s>>>(m, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    return hipGetLastError();
+}
+
+hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
+                           hipStream_t s) {
+    if (t.lg > kNttMaxLg || t.lg < 6) return hipErrorInvalidValue;
+    const uint32_t T = ntt_threads(t.lg);
+    const size_t lds = ((size_t)1 << t.lg) * 8;
+    hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds);
+    if (e != hipSuccess) return e;
+    ntt_kernel<1><<<n, T, lds, s>>>(0, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    return hipGetLastError();
+}
+
+}  // namespace midint

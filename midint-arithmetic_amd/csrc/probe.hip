@@ -1,0 +1,93 @@
+// VALU throughput probes (reference: the basic-op latency kernels
+// additionKer / multiplyKer / testBasicOps, cuda/ker-helpers.cu.h:189-317).
+//
+// Each lane runs 8 independent dependency chains of one operation for
+// `iters` iterations; enough waves are launched to fill every SIMD, so the
+// measured rate is the issue throughput of that instruction on gfx950. The
+// result is the price of one 32x32->64 partial product used as the
+// multiplication roofline.
+#include "midint.h"
+#include "midint_internal.h"
+
+namespace midint {
+
+constexpr int kChains = 8;
+
+// 32x32 -> 64 multiply-add (one v_mad_u64_u32 per step and chain).
+__global__ __launch_bounds__(256) void probe_mad64(uint32_t iters, uint32_t seed, uint64_t* out) {
+    uint64_t acc[kChains];
+    uint32_t x = seed ^ threadIdx.x;
+    uint32_t y = seed * 2654435761u + blockIdx.x;
+#pragma unroll
+    for (int c = 0; c < kChains; c++) acc[c] = (uint64_t)(x + c);
+    for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < kChains; c++) acc[c] = (uint64_t)(uint32_t)acc[c] * (uint64_t)y + acc[c];
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int c = 0; c < kChains; c++) s ^= acc[c];
+    if (s == 0x123456789ull) out[0] = s;  // keeps the chains live
+}
+
+// 32-bit integer xor + add (full-rate reference point: two VALU ops per step).
+__global__ __launch_bounds__(256) void probe_add32(uint32_t iters, uint32_t seed, uint64_t* out) {
+    uint32_t acc[kChains];
+    uint32_t y = seed * 2654435761u + blockIdx.x;
+#pragma unroll
+    for (int c = 0; c < kChains; c++) acc[c] = seed ^ (threadIdx.x + c);
+    for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < kChains; c++) acc[c] = (acc[c] ^ y) + (uint32_t)c + 0x9E3779B9u;
+    }
+    uint32_t s = 0;
+#pragma unroll
+    for (int c = 0; c < kChains; c++) s ^= acc[c];
+    if (s == 0x12345678u) out[0] = s;
+}
+
+}  // namespace midint
+
+using namespace midint;
+
+extern "C" midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_per_s) {
+    if (!mad64_per_s || !add32_per_s) return MIDINT_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MIDINT_EHIP;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return MIDINT_EHIP;
+    uint64_t* d_out = nullptr;
+    if (hipMalloc((void**)&d_out, 64) != hipSuccess) return MIDINT_ENOMEM;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const uint32_t blocks = (uint32_t)cus * 32;  // 8 waves per SIMD
+    const uint32_t threads = 256;
+    const uint32_t iters = 4096;
+    double rates[2] = {0, 0};
+    for (int which = 0; which < 2; which++) {
+        for (int rep = 0; rep < 3; rep++) {
+            (void)hipEventRecord(e0, nullptr);
+            if (which == 0)
+                probe_mad64<<<blocks, threads>>>(iters, 12345u + rep, d_out);
+            else
+                probe_add32<<<blocks, threads>>>(iters, 12345u + rep, d_out);
+            (void)hipEventRecord(e1, nullptr);
+            (void)hipEventSynchronize(e1);
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            // probe_add32 issues two VALU ops (xor, add) per chain step
+            const double ops = (double)blocks * threads * iters * kChains * (which == 0 ? 1 : 2);
+            const double r = ops / (ms * 1e-3);
+            if (r > rates[which]) rates[which] = r;
+        }
+    }
+    hipError_t e = hipGetLastError();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return MIDINT_EHIP;
+    *mad64_per_s = rates[0];
+    *add32_per_s = rates[1];
+    return MIDINT_OK;
+}

@@ -1,0 +1,152 @@
+/*
+ * midint-harness: C host harness over libmidint's C ABI, mirroring the
+ * reference's benchmark driver (cuda/main.cu:661-776):
+ *
+ *   runAdditions  -> run_additions   (cuda/main.cu:661-681)
+ *   runNaiveMuls  -> run_naive_muls  (cuda/main.cu:683-709)
+ *   runFFTMuls    -> run_fft_muls    (cuda/main.cu:711-746), validated here
+ *
+ * For each size (in u32 words, as the reference counts them) the inputs are
+ * generated with the reference's ourMkRandom<m,m> (glibc rand, seed 1), the
+ * operation runs through the host drop-ins (midint_gpu_*_host: dry run, then
+ * `runs` timed launches), and the result is compared word for word with
+ * GMP's mpn functions loaded from libgmp.so.10 (cuda/helper.h:146-170,
+ * validateExact :71-81). Output: the reference's printf lines
+ * (cuda/main.cu:112-115, 307-310, 513-519) followed by one JSON line each.
+ *
+ * usage: midint-harness [total_work_u32_words] [device]
+ *        default total_work = 2^27 (cuda/Makefile:25)
+ */
+#define _GNU_SOURCE
+#include <dlfcn.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "midint.h"
+
+typedef unsigned long (*gmpn_add_n_fn)(unsigned long*, const unsigned long*, const unsigned long*, long);
+typedef unsigned long (*gmpn_mul_fn)(unsigned long*, const unsigned long*, long, const unsigned long*, long);
+static gmpn_add_n_fn g_add_n;
+static gmpn_mul_fn g_mul;
+
+static int load_gmp(void) {
+    void* h = dlopen("libgmp.so.10", RTLD_NOW | RTLD_LOCAL);
+    if (!h) return 0;
+    g_add_n = (gmpn_add_n_fn)dlsym(h, "__gmpn_add_n");
+    g_mul = (gmpn_mul_fn)dlsym(h, "__gmpn_mul");
+    return g_add_n && g_mul;
+}
+
+/* Reference numAd32OpsOfMultInst (cuda/helper.h:36-43): 300 * n * log2 n for
+ * n = u32 words of one operand. */
+static double ops_of_mul(uint32_t m_u64) {
+    uint32_t n = 2 * m_u64, lg = 0;
+    for (uint32_t x = n; x > 1; x >>= 1) lg++;
+    return 300.0 * n * lg;
+}
+
+static int validate_exact(const uint64_t* want, const uint64_t* got, size_t count) {
+    for (size_t i = 0; i < count; i++)
+        if (want[i] != got[i]) {
+            printf("INVALID RESULT at flat index %zu: %lu vs %lu\n", i, (unsigned long)want[i],
+                   (unsigned long)got[i]);
+            return 0;
+        }
+    printf("VALID RESULT!\n");
+    return 1;
+}
+
+enum { OP_ADD, OP_MUL_LO, OP_MUL_FFT };
+
+static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int gmp) {
+    const uint32_t m = m_u32 / 2;                    /* u64 limbs */
+    const uint32_t n = (uint32_t)(total_work / m_u32);
+    const size_t in_limbs = (size_t)n * m;
+    const size_t out_limbs = (op == OP_MUL_FFT) ? 2 * in_limbs : in_limbs;
+    uint64_t* a = malloc(in_limbs * 8);
+    uint64_t* b = malloc(in_limbs * 8);
+    uint64_t* r = malloc(out_limbs * 8);
+    uint64_t* want = malloc(out_limbs * 8);
+    if (!a || !b || !r || !want) {
+        printf("host allocation failed\n");
+        return 0;
+    }
+    /* both operands from one glibc stream, as mkRandArrays (cuda/main.cu:18-33) */
+    midint_gen_ref_rand(1, n, m_u32, m_u32, (uint32_t*)a);
+    midint_gen_ref_rand(2, n, m_u32, m_u32, (uint32_t*)b);
+
+    double usec = 0.0;
+    midint_status st;
+    const uint32_t runs = (op == OP_ADD) ? 50 : 25; /* GPU_RUNS_ADD / GPU_RUNS_MUL (cuda/main.cu:11-12) */
+    if (op == OP_ADD)
+        st = midint_gpu_add_host(device, n, m, a, b, r, runs, &usec);
+    else if (op == OP_MUL_LO)
+        st = midint_gpu_mul_host(device, n, m, 0, a, b, r, runs, &usec);
+    else
+        st = midint_gpu_mul_fft_host(device, n, m, a, b, r, runs, &usec);
+    if (st != MIDINT_OK) {
+        printf("midint error: %s (hip %d)\n", midint_strerror(st), midint_last_hip_error());
+        free(a), free(b), free(r), free(want);
+        return 0;
+    }
+    const unsigned bits = m_u32 * 32;
+    const char* name = op == OP_ADD ? "add" : op == OP_MUL_LO ? "mul_lo" : "mul_ntt";
+    double metric;
+    if (op == OP_ADD) {
+        metric = 3.0 * in_limbs * 8 / (usec * 1000.0);
+        printf("Our Addition of %u-bit Big-Numbers (base u64) runs %u instances in: %.0f microsecs, "
+               "GB/sec: %.2f, Mil-Instances/sec: %.2f\n",
+               bits, n, usec, metric, n / usec);
+    } else {
+        metric = (double)n * ops_of_mul(m) / (usec * 1000.0);
+        printf("%s Multiplication of %u-bits Big-Numbers (in base = 64 bits) runs %u instances in: %.0f "
+               "microsecs, Gu32ops/sec: %.2f, Mil-Instances/sec: %.2f\n",
+               op == OP_MUL_LO ? "N^2" : "Our FFT", bits, n, usec, metric, n / usec);
+    }
+    int ok = -1;
+    if (gmp) {
+        if (op == OP_ADD) {
+            for (uint32_t i = 0; i < n; i++)
+                g_add_n((unsigned long*)(want + (size_t)i * m), (const unsigned long*)(a + (size_t)i * m),
+                        (const unsigned long*)(b + (size_t)i * m), (long)m);
+        } else {
+            uint64_t* full = malloc(2 * (size_t)m * 8);
+            for (uint32_t i = 0; i < n; i++) {
+                g_mul((unsigned long*)full, (const unsigned long*)(a + (size_t)i * m), (long)m,
+                      (const unsigned long*)(b + (size_t)i * m), (long)m);
+                if (op == OP_MUL_LO)
+                    memcpy(want + (size_t)i * m, full, (size_t)m * 8);
+                else
+                    memcpy(want + (size_t)i * 2 * m, full, 2 * (size_t)m * 8);
+            }
+            free(full);
+        }
+        ok = validate_exact(want, r, out_limbs);
+    }
+    printf("{\"op\": \"%s\", \"bits\": %u, \"instances\": %u, \"usec\": %.3f, \"%s\": %.3f, \"valid\": %s}\n",
+           name, bits, n, usec, op == OP_ADD ? "GB_per_s" : "Gu32ops_per_s", metric,
+           ok < 0 ? "null" : ok ? "true" : "false");
+    fflush(stdout);
+    free(a), free(b), free(r), free(want);
+    return ok != 0;
+}
+
+int main(int argc, char** argv) {
+    uint64_t total_work = 1ull << 27;
+    int device = 0;
+    if (argc > 1) total_work = strtoull(argv[1], NULL, 10);
+    if (argc > 2) device = atoi(argv[2]);
+    const int gmp = load_gmp();
+    if (!gmp) printf("libgmp.so.10 not found: results are not validated\n");
+    const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64, 32, 16};
+    int ok = 1;
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
+        ok &= run_one(OP_ADD, device, total_work, sizes[i], gmp);
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
+        ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], gmp);
+    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
+        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp);
+    return ok ? 0 : 1;
+}

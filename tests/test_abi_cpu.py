@@ -1,0 +1,67 @@
+"""CPU-side checks of the C ABI library: it loads, exports every symbol that
+include/midint.h declares, reports errors without a GPU call, and its
+host-side generators agree with the oracle's."""
+import ctypes
+import subprocess
+
+import numpy as np
+import pytest
+
+
+def test_exports_every_declared_symbol(mi):
+    declared = mi.declared_symbols()
+    assert len(declared) >= 20
+    L = mi.lib()
+    missing = [s for s in declared if not hasattr(L, s)]
+    assert not missing, missing
+
+
+def test_declared_matches_binding_table(mi):
+    # every declared function has a ctypes signature in the binding layer
+    assert set(mi.declared_symbols()) == set(mi._SIGS)
+
+
+def test_nm_dynamic_symbols(mi):
+    out = subprocess.run(["nm", "-D", "--defined-only", mi.LIB_PATH], capture_output=True, text=True,
+                         check=True).stdout
+    names = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    for s in mi.declared_symbols():
+        assert s in names, s
+
+
+def test_strerror_and_version(mi):
+    L = mi.lib()
+    assert L.midint_strerror(0) == b"success"
+    assert L.midint_strerror(2) == b"operand size not supported"
+    assert L.midint_strerror(99) == b"unknown status"
+    assert b"gfx950" in L.midint_version()
+
+
+def test_argument_errors_need_no_gpu(mi):
+    L = mi.lib()
+    # NULL context and m == 0 are rejected before any HIP call
+    assert L.midint_add(None, 4, 8, None, None, None, None, None) == 1
+    assert L.midint_mul_full(None, 4, 0, None, None, None, None) == 1
+    assert L.midint_init(None, 0) == 1
+    assert L.midint_destroy(None) == 1
+    assert L.midint_gpu_add_host(0, 4, 0, None, None, None, 0, None) == 1
+
+
+def test_size_limits(mi):
+    L = mi.lib()
+    assert L.midint_max_limbs_add() >= 4096
+    assert L.midint_max_limbs_mul() >= 2048
+    assert L.midint_max_limbs_ntt() >= 2048
+
+
+def test_generators_match_oracle(mi, orc):
+    seed = 0x6D6964696E74
+    assert np.array_equal(mi.gen_limbs(seed, 1000), orc.gen_limbs(seed, 1000))
+    assert np.array_equal(mi.gen_ref_rand(1, 3, 16, 16), orc.gen_ref_rand(1, 3, 16, 16))
+    assert np.array_equal(mi.gen_ref_rand(7, 2, 8, 5), orc.gen_ref_rand(7, 2, 8, 5))
+
+
+def test_harness_binary_built(mi):
+    import os
+
+    assert os.access(os.path.join(mi.HERE, "bin", "midint-harness"), os.X_OK)
