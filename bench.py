@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 256 MiB int64 negation (2 x 256 MiB of HBM traffic per launch) x5, "
+                         "the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: every instance is independent, so N GPUs would each run "
@@ -132,10 +135,18 @@ def main():
         it[0] += 1
         ctx.add(x, y, z, ma)
 
-    ms_add = timed(torch, add_step, 400, 40)
+    ms_add = timed(torch, add_step, 2 * args.steps, max(4, args.warmup))
     add_bytes = 3 * na * ma * 8
     add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
     log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
+
+    if args.calibrate:
+        src = torch.ones(32 * MIB, dtype=torch.int64, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            torch.neg(src, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
 
     # ---------------------------------------------------------- CPU baseline
     cpu = None

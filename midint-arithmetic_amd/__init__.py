@@ -77,6 +77,14 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it first (make -C {HERE})")
+        # torch bundles its own HIP runtime (SONAME libamdhip64.so.7, loaded by
+        # torch under the unversioned name). Loading torch first makes
+        # libmidint's dependency bind to that same runtime instead of a second
+        # copy from /opt/rocm, which would not see the device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         for name, (res, args) in _SIGS.items():
             f = getattr(L, name)

@@ -46,6 +46,28 @@ __host__ __device__ inline uint32_t mul_npad(uint32_t m) {
     return (n + 2 * kMulQ - 1) / (2 * kMulQ) * (2 * kMulQ);
 }
 
+// acc[q] += a * b_q (32x32->64) and ovf[q] += carry-out, for q = 0..3.
+// One v_mad_u64_u32 per product writes the 64-bit sum and its carry-out lane
+// mask; one v_addc_co_u32 folds the mask into the overflow count. Each mask
+// is read three VALU instructions after it is written.
+__device__ __forceinline__ void mac4(uint32_t a, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
+                                     uint64_t (&acc)[4], uint32_t (&ovf)[4]) {
+    uint64_t k0, k1, k2, k3;
+    asm volatile(
+        "v_mad_u64_u32 %[c0], %[k0], %[a], %[b0], %[c0]\n\t"
+        "v_mad_u64_u32 %[c1], %[k1], %[a], %[b1], %[c1]\n\t"
+        "v_mad_u64_u32 %[c2], %[k2], %[a], %[b2], %[c2]\n\t"
+        "v_mad_u64_u32 %[c3], %[k3], %[a], %[b3], %[c3]\n\t"
+        "v_addc_co_u32_e64 %[o0], %[k0], %[o0], 0, %[k0]\n\t"
+        "v_addc_co_u32_e64 %[o1], %[k1], %[o1], 0, %[k1]\n\t"
+        "v_addc_co_u32_e64 %[o2], %[k2], %[o2], 0, %[k2]\n\t"
+        "v_addc_co_u32_e64 %[o3], %[k3], %[o3], 0, %[k3]"
+        : [c0] "+v"(acc[0]), [c1] "+v"(acc[1]), [c2] "+v"(acc[2]), [c3] "+v"(acc[3]), [o0] "+v"(ovf[0]),
+          [o1] "+v"(ovf[1]), [o2] "+v"(ovf[2]), [o3] "+v"(ovf[3]), [k0] "=&s"(k0), [k1] "=&s"(k1),
+          [k2] "=&s"(k2), [k3] "=&s"(k3)
+        : [a] "v"(a), [b0] "v"(b0), [b1] "v"(b1), [b2] "v"(b2), [b3] "v"(b3));
+}
+
 template <bool FULL>
 __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
                                                    uint32_t P, uint32_t words, uint32_t ipb, uint32_t out_len,
@@ -111,15 +133,10 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
         const uint32_t* A = lds + slot * words;
         const uint32_t* Bw = A + n_pad;
         uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
+        // a-window start advances by Q per block, b-window start retreats by Q
+        const uint32_t* pa = A;
+        const uint32_t* pb = Bw + k1;
         for (uint32_t beta = 0; beta < NB; beta++) {
-            uint32_t k, i0;
-            if (beta < sw) {
-                k = k1;
-                i0 = Q * beta;
-            } else {
-                k = k2;
-                i0 = i2 + Q * (beta - sw);
-            }
             if (beta == sw) {
 #pragma unroll
                 for (int q = 0; q < Q; q++) {
@@ -127,22 +144,19 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
                     snap[3 * q + 1] = (uint32_t)(acc[q] >> 32);
                     snap[3 * q + 2] = ovf[q];
                 }
+                pa = A + i2;
+                pb = Bw + (k2 - i2);
             }
-            const uint4 av = *reinterpret_cast<const uint4*>(A + i0);
-            const uint32_t bidx = k - i0;
-            const uint4 bw0 = *reinterpret_cast<const uint4*>(Bw + bidx);
-            const uint4 bw1 = *reinterpret_cast<const uint4*>(Bw + bidx + 4);
-            const uint32_t ax[4] = {av.x, av.y, av.z, av.w};
-            const uint32_t bx[8] = {bw0.x, bw0.y, bw0.z, bw0.w, bw1.x, bw1.y, bw1.z, bw1.w};
-#pragma unroll
-            for (int u = 0; u < Q; u++) {
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    const uint64_t prev = acc[q];
-                    acc[q] = (uint64_t)ax[u] * (uint64_t)bx[q - u + Q - 1] + prev;
-                    ovf[q] += acc[q] < prev;
-                }
-            }
+            const uint4 av = *reinterpret_cast<const uint4*>(pa);
+            const uint4 bw0 = *reinterpret_cast<const uint4*>(pb);
+            const uint4 bw1 = *reinterpret_cast<const uint4*>(pb + 4);
+            // column q, a-word u uses b-window word q - u + Q - 1
+            mac4(av.x, bw0.w, bw1.x, bw1.y, bw1.z, acc, ovf);
+            mac4(av.y, bw0.z, bw0.w, bw1.x, bw1.y, acc, ovf);
+            mac4(av.z, bw0.y, bw0.z, bw0.w, bw1.x, acc, ovf);
+            mac4(av.w, bw0.x, bw0.y, bw0.z, bw0.w, acc, ovf);
+            pa += Q;
+            pb -= Q;
         }
         // group 1 parked at the switch, group 2 = total - group 1
 #pragma unroll
