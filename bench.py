@@ -51,6 +51,16 @@ def to_dev(torch, x):
     return torch.from_numpy(np.ascontiguousarray(x).view(np.int64)).to("cuda:0")
 
 
+def measured_traffic(name):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 PMC
+    summary (profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)["kernels"][name]["hbm_bytes_per_launch"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_baseline_mul(a, b, m, threads, budget_s=1.0):
     """Oracle restatement (port of the reference algorithm) and GMP mpn_mul on
     the host cores, repeated over the batch until ~budget_s of wall time."""
@@ -177,14 +187,15 @@ def main():
                                "classical kernel", "instances": n, "limbs": m},
         "roofline": {"bound": "valu", "achieved": achieved_pp / 1e12, "peak": mad_rate / 1e12,
                      "unit": "T partial-products/s (32x32->64 v_mad_u64_u32)",
-                     "frac": achieved_pp / mad_rate, "traffic": None,
+                     "frac": achieved_pp / mad_rate, "traffic": measured_traffic("mul_full_16384x128"),
+                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"},
         "cpu_baseline": cpu,
         "extra": {
             "add_65536x64": {"us_per_launch": ms_add * 1e3, "GB_per_s": add_gbs,
                              "roofline": {"bound": "hbm", "achieved": add_gbs, "peak": HBM_PEAK_GBS,
                                           "unit": "GB/s", "frac": add_gbs / HBM_PEAK_GBS,
-                                          "traffic": None},
+                                          "traffic": measured_traffic("add_65536x64")},
                              "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)"},
             "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3)},
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
