@@ -150,6 +150,47 @@ def main():
     add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
     log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
 
+    # ---------------------------------------------------------- ragged batch
+    from workloads import make_ragged, size_class
+
+    ops, ra, rb, rlimbs = make_ragged(mi.gen_limbs)
+    dra, drb = to_dev(torch, ra), to_dev(torch, rb)
+    drr = torch.zeros(rlimbs, dtype=torch.int64, device="cuda:0")
+    n_mul = sum(1 for o in ops if o[0] == 1)
+    plan = mi.RaggedPlan(ctx, ops, dra, drb, drr)
+    rsteps = max(5, args.steps // 20)
+    ms_rag = timed(torch, plan.launch, rsteps, 2)
+    pstats = plan.stats()
+    # plain per-size launches: one fixed-size launch per (op, power-of-two class)
+    classes = {}
+    for i, (kind, na, nb, oa, ob, orr) in enumerate(ops):
+        classes.setdefault((kind, size_class(max(na, nb))), []).append(i)
+    plain = []
+    for (kind, M), idx in sorted(classes.items()):
+        n_c = len(idx)
+        pa = np.zeros((n_c, M), dtype=np.uint64)
+        pb = np.zeros((n_c, M), dtype=np.uint64)
+        for j, i in enumerate(idx):
+            _, na, nb, oa, ob, _ = ops[i]
+            pa[j, :na] = ra[oa:oa + na]
+            pb[j, :nb] = rb[ob:ob + nb]
+        out_l = 2 * n_c * M if kind == 1 else n_c * M
+        plain.append((kind, M, to_dev(torch, pa), to_dev(torch, pb),
+                      torch.empty(out_l, dtype=torch.int64, device="cuda:0"),
+                      torch.empty(n_c, dtype=torch.uint8, device="cuda:0")))
+
+    def plain_step():
+        for kind, M, x, y, z, c in plain:
+            if kind == 1:
+                ctx.mul_full(x, y, z, M)
+            else:
+                ctx.add(x, y, z, M, carry=c)
+
+    ms_plain = timed(torch, plain_step, rsteps, 2)
+    log(f"ragged {len(ops)} ops: persistent {ms_rag:.3f} ms ({pstats}), plain per-size {ms_plain:.3f} ms, "
+        f"speedup {ms_plain / ms_rag:.2f}x")
+    del plan, plain, dra, drb, drr
+
     if args.calibrate:
         src = torch.ones(32 * MIB, dtype=torch.int64, device="cuda:0")
         dst = torch.empty_like(src)
@@ -198,6 +239,11 @@ def main():
                                           "traffic": measured_traffic("add_65536x64")},
                              "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)"},
             "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3)},
+            "ragged_50000": {"ops": len(ops), "muls": n_mul, "persistent_ms": ms_rag, "plain_per_size_ms": ms_plain,
+                             "speedup": ms_plain / ms_rag, "muls_per_s_persistent": n_mul / (ms_rag * 1e-3),
+                             "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plan": pstats,
+                             "note": "one hipGraph launch (counter reset + persistent kernel) vs one fixed-size "
+                                     "launch per (op, power-of-two size class) on zero-padded operands"},
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
         },
     }

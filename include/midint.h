@@ -110,6 +110,46 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
  * on first use). */
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);
 
+/* ------------------------------------------------------- ragged batches */
+/* One op of a ragged batch. Operands live in two packed device buffers
+ * (d_a, d_b) at limb offsets off_a / off_b; the result is written to d_r at
+ * off_r: n_a + n_b limbs for a product, max(n_a, n_b) + 1 limbs for a sum
+ * (both exact, no truncation). Sizes are independent per operand. */
+typedef enum { MIDINT_OP_ADD = 0, MIDINT_OP_MUL = 1 } midint_op_kind;
+typedef struct {
+    uint32_t kind;     /* midint_op_kind */
+    uint32_t n_a;      /* limbs of a, 1 .. midint_max_limbs_ragged() */
+    uint32_t n_b;      /* limbs of b */
+    uint32_t reserved; /* must be 0 */
+    uint64_t off_a;
+    uint64_t off_b;
+    uint64_t off_r;
+} midint_op;
+
+/* Launch parameters of the persistent kernel (0 = library default / tuned). */
+typedef struct {
+    uint32_t threads;       /* 256 or 512 threads per block */
+    uint32_t blocks_per_cu; /* persistent blocks per CU (0: occupancy limit) */
+} midint_ragged_tuning;
+
+typedef struct midint_ragged_plan midint_ragged_plan;
+
+uint32_t midint_max_limbs_ragged(void);
+
+/* Plans a ragged batch over fixed buffers: packs the ops into work items,
+ * uploads them, and builds a hipGraph (counter reset + ONE persistent
+ * kernel). No reference counterpart: the reference launches one
+ * compile-time specialisation per size (cuda/main.cu:661-709). */
+midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, size_t n_ops, const uint64_t* d_a,
+                                        const uint64_t* d_b, uint64_t* d_r, const midint_ragged_tuning* tuning,
+                                        midint_ragged_plan** plan);
+/* Replays the plan's graph on `stream` (one graph launch). */
+midint_status midint_ragged_launch(midint_ragged_plan* plan, void* stream);
+/* Work items, persistent grid size (blocks) and LDS bytes per block. */
+midint_status midint_ragged_plan_stats(const midint_ragged_plan* plan, uint64_t* n_items, uint64_t* grid,
+                                       uint64_t* lds_bytes);
+midint_status midint_ragged_plan_destroy(midint_ragged_plan* plan);
+
 /* ------------------------------------------------- host-pointer drop-ins */
 /* Mirror the reference host drivers: allocate device buffers, copy inputs,
  * run the operation `runs` times back to back (after one untimed run),

@@ -56,6 +56,12 @@ _SIGS = {
     "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_probe_valu": (_int, [_int, _dblp, _dblp]),
+    "midint_max_limbs_ragged": (_u32, []),
+    "midint_ragged_plan_create": (_int, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
+    "midint_ragged_launch": (_int, [_vp, _vp]),
+    "midint_ragged_plan_stats": (_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "midint_ragged_plan_destroy": (_int, [_vp]),
     "midint_gen_limbs": (None, [ctypes.c_uint64, _vp, ctypes.c_size_t]),
     "midint_gen_ref_rand": (None, [_u32, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _vp]),
 }
@@ -187,6 +193,56 @@ class Context:
     @property
     def handle(self):
         return self._h
+
+
+# ---------------------------------------------------------------- ragged
+OP_ADD = 0
+OP_MUL = 1
+
+
+class Op(ctypes.Structure):
+    """midint_op: one operation of a ragged batch."""
+    _fields_ = [("kind", ctypes.c_uint32), ("n_a", ctypes.c_uint32), ("n_b", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32), ("off_a", ctypes.c_uint64), ("off_b", ctypes.c_uint64),
+                ("off_r", ctypes.c_uint64)]
+
+
+class RaggedTuning(ctypes.Structure):
+    _fields_ = [("threads", ctypes.c_uint32), ("blocks_per_cu", ctypes.c_uint32)]
+
+
+class RaggedPlan:
+    """midint_ragged_plan over fixed device buffers (torch tensors)."""
+
+    def __init__(self, ctx, ops, a, b, r, threads=0, blocks_per_cu=0):
+        arr = (Op * len(ops))()
+        for i, (kind, na, nb, oa, ob, orr) in enumerate(ops):
+            arr[i] = Op(kind, na, nb, 0, oa, ob, orr)
+        self._keep = (a, b, r)
+        tun = RaggedTuning(threads, blocks_per_cu)
+        self._h = ctypes.c_void_p()
+        check(lib().midint_ragged_plan_create(ctx.handle, arr, len(ops), _ptr(a), _ptr(b), _ptr(r),
+                                              ctypes.byref(tun), ctypes.byref(self._h)), "midint_ragged_plan_create")
+
+    def launch(self, stream=None):
+        check(lib().midint_ragged_launch(self._h, _stream(stream)), "midint_ragged_launch")
+
+    def stats(self):
+        x, y, z = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        check(lib().midint_ragged_plan_stats(self._h, ctypes.byref(x), ctypes.byref(y), ctypes.byref(z)),
+              "midint_ragged_plan_stats")
+        return {"items": x.value, "grid": y.value, "lds_bytes": z.value}
+
+    def close(self):
+        if self._h:
+            lib().midint_ragged_plan_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- host side

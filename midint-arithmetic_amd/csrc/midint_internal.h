@@ -45,4 +45,31 @@ hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint6
 hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
                            hipStream_t s);
 
+// Ragged batches.
+constexpr uint32_t kRgWindow = 2048;   // output limbs per work item
+constexpr uint32_t kRgMaxLanes = 512;  // virtual lanes per multiplication item
+constexpr uint32_t kRgMaxLimbs = 1024; // operand limbs
+constexpr uint32_t kRgAdd = 0;
+constexpr uint32_t kRgMul = 1;
+
+struct RgOp {
+    uint64_t off_s, off_l, off_r;  // limb offsets: shorter operand, longer operand, result
+    uint32_t ns, nl;               // limbs (additions: a and b in order)
+    uint32_t ns_w, nl_w;           // padded u32 word counts (multiplications)
+    uint32_t lanes, lane_base;     // virtual lanes and first lane within the item
+    uint32_t lds_base;             // u32 word offset of the operand staging area
+    uint32_t seg_base, seg_len;    // output window segment within the item
+    uint32_t out_len;              // limbs written to the result
+    uint32_t swap;                 // 1: S comes from b, L from a
+    uint32_t pad;
+};
+
+struct RgItem {
+    uint32_t kind, first, count, lanes, window, conv_words;
+};
+
+size_t ragged_lds_size(uint32_t threads, uint32_t conv_cap);
+const void* ragged_kernel_ptr(uint32_t threads);
+const void* ragged_reset_ptr();
+
 }  // namespace midint

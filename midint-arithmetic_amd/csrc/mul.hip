@@ -27,10 +27,9 @@
 // wave-ballot row adder.
 #include "add_core.hpp"
 #include "midint_internal.h"
+#include "mul_core.hpp"
 
 namespace midint {
-
-constexpr uint32_t kMulQ = 4;
 
 struct MulLayout {
     uint32_t n_pad;   // padded operand length in u32 words (multiple of 2Q)
@@ -44,28 +43,6 @@ struct MulLayout {
 __host__ __device__ inline uint32_t mul_npad(uint32_t m) {
     uint32_t n = 2 * m;
     return (n + 2 * kMulQ - 1) / (2 * kMulQ) * (2 * kMulQ);
-}
-
-// acc[q] += a * b_q (32x32->64) and ovf[q] += carry-out, for q = 0..3.
-// One v_mad_u64_u32 per product writes the 64-bit sum and its carry-out lane
-// mask; one v_addc_co_u32 folds the mask into the overflow count. Each mask
-// is read three VALU instructions after it is written.
-__device__ __forceinline__ void mac4(uint32_t a, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
-                                     uint64_t (&acc)[4], uint32_t (&ovf)[4]) {
-    uint64_t k0, k1, k2, k3;
-    asm volatile(
-        "v_mad_u64_u32 %[c0], %[k0], %[a], %[b0], %[c0]\n\t"
-        "v_mad_u64_u32 %[c1], %[k1], %[a], %[b1], %[c1]\n\t"
-        "v_mad_u64_u32 %[c2], %[k2], %[a], %[b2], %[c2]\n\t"
-        "v_mad_u64_u32 %[c3], %[k3], %[a], %[b3], %[c3]\n\t"
-        "v_addc_co_u32_e64 %[o0], %[k0], %[o0], 0, %[k0]\n\t"
-        "v_addc_co_u32_e64 %[o1], %[k1], %[o1], 0, %[k1]\n\t"
-        "v_addc_co_u32_e64 %[o2], %[k2], %[o2], 0, %[k2]\n\t"
-        "v_addc_co_u32_e64 %[o3], %[k3], %[o3], 0, %[k3]"
-        : [c0] "+v"(acc[0]), [c1] "+v"(acc[1]), [c2] "+v"(acc[2]), [c3] "+v"(acc[3]), [o0] "+v"(ovf[0]),
-          [o1] "+v"(ovf[1]), [o2] "+v"(ovf[2]), [o3] "+v"(ovf[3]), [k0] "=&s"(k0), [k1] "=&s"(k1),
-          [k2] "=&s"(k2), [k3] "=&s"(k3)
-        : [a] "v"(a), [b0] "v"(b0), [b1] "v"(b1), [b2] "v"(b2), [b3] "v"(b3));
 }
 
 template <bool FULL>
@@ -109,15 +86,6 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     const uint32_t t = tid - slot * T;
     const bool active = slot < ipb;
 
-    uint64_t acc[Q];
-    uint32_t ovf[Q];
-    uint64_t sacc[Q];
-    uint32_t sovf[Q];
-#pragma unroll
-    for (int q = 0; q < Q; q++) {
-        acc[q] = 0;
-        ovf[q] = 0;
-    }
     uint32_t k1 = Q * t, k2, i2, NB;
     if (FULL) {
         k2 = n_pad + Q * t;
@@ -128,93 +96,17 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
         i2 = 0;
         NB = 2 * T + 1;
     }
-    const uint32_t sw = t + 1;
+    ColGroups g;
     if (active) {
         const uint32_t* A = lds + slot * words;
-        const uint32_t* Bw = A + n_pad;
         uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
-        // a-window start advances by Q per block, b-window start retreats by Q
-        const uint32_t* pa = A;
-        const uint32_t* pb = Bw + k1;
-        for (uint32_t beta = 0; beta < NB; beta++) {
-            if (beta == sw) {
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    snap[3 * q] = (uint32_t)acc[q];
-                    snap[3 * q + 1] = (uint32_t)(acc[q] >> 32);
-                    snap[3 * q + 2] = ovf[q];
-                }
-                pa = A + i2;
-                pb = Bw + (k2 - i2);
-            }
-            const uint4 av = *reinterpret_cast<const uint4*>(pa);
-            const uint4 bw0 = *reinterpret_cast<const uint4*>(pb);
-            const uint4 bw1 = *reinterpret_cast<const uint4*>(pb + 4);
-            // column q, a-word u uses b-window word q - u + Q - 1
-            mac4(av.x, bw0.w, bw1.x, bw1.y, bw1.z, acc, ovf);
-            mac4(av.y, bw0.z, bw0.w, bw1.x, bw1.y, acc, ovf);
-            mac4(av.z, bw0.y, bw0.z, bw0.w, bw1.x, acc, ovf);
-            mac4(av.w, bw0.x, bw0.y, bw0.z, bw0.w, acc, ovf);
-            pa += Q;
-            pb -= Q;
-        }
-        // group 1 parked at the switch, group 2 = total - group 1
-#pragma unroll
-        for (int q = 0; q < Q; q++) {
-            sacc[q] = (uint64_t)snap[3 * q] | ((uint64_t)snap[3 * q + 1] << 32);
-            sovf[q] = snap[3 * q + 2];
-        }
+        conv_lane(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
     }
     __syncthreads();  // everyone is done with A/B/snap; reuse LDS for L/H
 
     uint64_t* Lb = lds64;
     uint64_t* Hb = lds64 + ipb * P;
-    if (active) {
-#pragma unroll
-        for (int g = 0; g < 2; g++) {
-            uint64_t va[Q];
-            uint32_t vo[Q];
-            uint32_t kcol;
-            if (g == 0) {
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    va[q] = sacc[q];
-                    vo[q] = sovf[q];
-                }
-                kcol = k1;
-            } else {
-#pragma unroll
-                for (int q = 0; q < Q; q++) {
-                    va[q] = acc[q] - sacc[q];
-                    vo[q] = ovf[q] - sovf[q] - (acc[q] < sacc[q] ? 1u : 0u);
-                }
-                kcol = k2;
-            }
-            // group value sum_q (va + vo*2^64) * 2^(32q) -> Q low words + H
-            uint64_t lo = 0, hi = 0;  // running 128-bit value
-            uint32_t w[Q];
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                uint64_t nlo = lo + va[q];
-                hi += (uint64_t)vo[q] + (nlo < lo ? 1ull : 0ull);
-                lo = nlo;
-                w[q] = (uint32_t)lo;
-                lo = (lo >> 32) | (hi << 32);
-                hi >>= 32;
-            }
-            const uint32_t limb = kcol / 2;
-            uint64_t* L = Lb + slot * P;
-            uint64_t* H = Hb + slot * P;
-#pragma unroll
-            for (int i = 0; i < Q / 2; i++) L[limb + i] = (uint64_t)w[2 * i] | ((uint64_t)w[2 * i + 1] << 32);
-            if (limb + Q / 2 < P) H[limb + Q / 2] = lo;
-            if (limb + Q / 2 + 1 < P) H[limb + Q / 2 + 1] = hi;
-            if (kcol == 0) {
-#pragma unroll
-                for (int i = 0; i < Q / 2; i++) H[i] = 0;
-            }
-        }
-    }
+    if (active) lane_store(g, true, k1, k2, Lb + slot * P, Hb + slot * P, P);
     __syncthreads();
 
     // ---- normalisation: L + H over ipb segments of P limbs (2 rows per wave)

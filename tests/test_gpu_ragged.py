@@ -1,0 +1,123 @@
+"""Ragged batches in one persistent graph launch: every op's result equals
+the oracle's, alone and in the mix, and the plain per-size-class launches
+(operands zero-padded to the next power of two, the reference's way of
+serving mixed sizes) agree with it."""
+import numpy as np
+import pytest
+
+from workloads import make_ragged, ragged_reference, size_class
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def torch_dev():
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch
+
+
+@pytest.fixture(scope="module")
+def ctx(mi, torch_dev):
+    c = mi.Context(0)
+    yield c
+    c.close()
+
+
+def dev(torch, x):
+    return torch.from_numpy(np.ascontiguousarray(x).view(np.int64)).to("cuda:0")
+
+
+def run_plan(mi, ctx, torch, ops, a, b, rl, threads=0, reps=1):
+    da, db = dev(torch, a), dev(torch, b)
+    dr = torch.zeros(max(rl, 1), dtype=torch.int64, device="cuda:0")
+    plan = mi.RaggedPlan(ctx, ops, da, db, dr, threads=threads)
+    for _ in range(reps):
+        plan.launch()
+    torch.cuda.synchronize()
+    out = dr.cpu().numpy().view(np.uint64)[:rl]
+    plan.close()
+    return out
+
+
+@pytest.mark.parametrize("threads", [256, 512])
+def test_ragged_small_vs_oracle(mi, ctx, torch_dev, orc, threads):
+    ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=600, seed=11, lo_bits=6, hi_bits=14)
+    got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, threads=threads, reps=2)
+    want = ragged_reference(orc, ops, a, b, rl)
+    assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
+
+
+def test_ragged_edge_sizes(mi, ctx, torch_dev, orc):
+    MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+    shapes = [(1, 1), (1, 1024), (1024, 1), (1024, 1024), (3, 5), (2, 2), (1023, 1021), (4, 4), (511, 513)]
+    ops, a_parts, b_parts = [], [], []
+    oa = ob = orr = 0
+    rng = np.random.default_rng(3)
+    for i, (na, nb) in enumerate(shapes * 2):
+        kind = i % 2 if i >= len(shapes) else 1
+        x = np.full(na, MASK, dtype=np.uint64) if i % 3 == 0 else rng.integers(0, 2**64, na, dtype=np.uint64)
+        y = np.full(nb, MASK, dtype=np.uint64) if i % 3 == 0 else rng.integers(0, 2**64, nb, dtype=np.uint64)
+        rlen = na + nb if kind == 1 else max(na, nb) + 1
+        ops.append((kind, na, nb, oa, ob, orr))
+        a_parts.append(x)
+        b_parts.append(y)
+        oa += na
+        ob += nb
+        orr += rlen
+    a, b = np.concatenate(a_parts), np.concatenate(b_parts)
+    got = run_plan(mi, ctx, torch_dev, ops, a, b, orr)
+    assert np.array_equal(got, ragged_reference(orc, ops, a, b, orr))
+
+
+def test_ragged_alone_equals_mixed(mi, ctx, torch_dev, orc):
+    ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=300, seed=21, lo_bits=6, hi_bits=13)
+    mixed = run_plan(mi, ctx, torch_dev, ops, a, b, rl)
+    for i in (0, 7, 150, 299):
+        kind, na, nb, oa, ob, orr = ops[i]
+        rlen = na + nb if kind == 1 else max(na, nb) + 1
+        alone = run_plan(mi, ctx, torch_dev, [(kind, na, nb, 0, 0, 0)], a[oa:oa + na], b[ob:ob + nb], rlen)
+        assert np.array_equal(alone, mixed[orr:orr + rlen])
+
+
+def test_ragged_full_workload(mi, ctx, torch_dev, orc):
+    """The 50000-op batch: persistent launch vs oracle, and the plain
+    per-size-class launches vs the same oracle (bit-exact)."""
+    torch = torch_dev
+    ops, a, b, rl = make_ragged(mi.gen_limbs)
+    got = run_plan(mi, ctx, torch, ops, a, b, rl, reps=2)
+    want = ragged_reference(orc, ops, a, b, rl)
+    assert np.array_equal(got, want)
+    # plain per-size launches (mul_full at class M, add at class M + carry)
+    classes = {}
+    for i, (kind, na, nb, oa, ob, orr) in enumerate(ops):
+        classes.setdefault((kind, size_class(max(na, nb))), []).append(i)
+    for (kind, M), idx in sorted(classes.items()):
+        n = len(idx)
+        pa = np.zeros((n, M), dtype=np.uint64)
+        pb = np.zeros((n, M), dtype=np.uint64)
+        for j, i in enumerate(idx):
+            _, na, nb, oa, ob, _ = ops[i]
+            pa[j, :na] = a[oa:oa + na]
+            pb[j, :nb] = b[ob:ob + nb]
+        da, db = dev(torch, pa), dev(torch, pb)
+        if kind == 1:
+            dr = torch.empty(2 * n * M, dtype=torch.int64, device="cuda:0")
+            ctx.mul_full(da, db, dr, M)
+            res = dr.cpu().numpy().view(np.uint64).reshape(n, 2 * M)
+        else:
+            dr = torch.empty(n * M, dtype=torch.int64, device="cuda:0")
+            dc = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+            ctx.add(da, db, dr, M, carry=dc)
+            res = np.concatenate([dr.cpu().numpy().view(np.uint64).reshape(n, M),
+                                  dc.cpu().numpy().astype(np.uint64)[:, None]], axis=1)
+        for j, i in enumerate(idx):
+            _, na, nb, _, _, orr = ops[i]
+            rlen = na + nb if kind == 1 else max(na, nb) + 1
+            if kind == 0 and rlen - 1 < M:
+                # carry sits at limb max(na,nb) of the M-limb sum, or in the carry byte
+                assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])
+            else:
+                assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])

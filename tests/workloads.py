@@ -37,3 +37,50 @@ def make_inputs(name, gen):
         b2[::100, :] = np.uint64(0)
         b2[::100, 0] = np.uint64(1)
     return a, b
+
+
+# ---------------------------------------------------------------- ragged batch
+def make_ragged(gen, n_ops=50000, seed=SEED + 2, mul_frac=0.7, lo_bits=8, hi_bits=16):
+    """Ragged batch: each op is a multiplication with probability mul_frac,
+    else an addition; each operand size is max(4, round(2^U(lo_bits, hi_bits)
+    / 64)) limbs, drawn independently per operand (256..65536 bits for the
+    defaults). Kinds and sizes come from numpy's PCG64 seeded with `seed`,
+    limbs from `gen(seed, count)`.
+
+    Returns (ops, a, b, r_limbs) where ops is a list of
+    (kind, n_a, n_b, off_a, off_b, off_r) with kind 0 = add, 1 = mul, a and b
+    the packed operand buffers, r_limbs the size of the packed result."""
+    rng = np.random.default_rng(seed)
+    kinds = (rng.random(n_ops) < mul_frac).astype(np.uint32)
+    sizes = np.maximum(4, np.rint(2.0 ** rng.uniform(lo_bits, hi_bits, size=(n_ops, 2)) / 64)).astype(np.int64)
+    na, nb = sizes[:, 0], sizes[:, 1]
+    off_a = np.concatenate([[0], np.cumsum(na)[:-1]])
+    off_b = np.concatenate([[0], np.cumsum(nb)[:-1]])
+    rl = np.where(kinds == 1, na + nb, np.maximum(na, nb) + 1)
+    off_r = np.concatenate([[0], np.cumsum(rl)[:-1]])
+    limbs = gen(seed, int(na.sum() + nb.sum()))
+    a = np.ascontiguousarray(limbs[: na.sum()])
+    b = np.ascontiguousarray(limbs[na.sum():])
+    ops = [(int(kinds[i]), int(na[i]), int(nb[i]), int(off_a[i]), int(off_b[i]), int(off_r[i]))
+           for i in range(n_ops)]
+    return ops, a, b, int(rl.sum())
+
+
+def ragged_reference(orc, ops, a, b, r_limbs):
+    """Per-op oracle results in the packed result layout."""
+    r = np.zeros(r_limbs, dtype=np.uint64)
+    for kind, na, nb, oa, ob, orr in ops:
+        x, y = a[oa:oa + na], b[ob:ob + nb]
+        if kind == 1:
+            r[orr:orr + na + nb] = orc.mul_unbal(x, y)
+        else:
+            r[orr:orr + max(na, nb) + 1] = orc.add_unbal(x, y)
+    return r
+
+
+def size_class(n):
+    """Power-of-two size class (>= 4 limbs) of the plain per-size launches."""
+    c = 4
+    while c < n:
+        c *= 2
+    return c
