@@ -150,6 +150,47 @@ midint_status midint_ragged_plan_stats(const midint_ragged_plan* plan, uint64_t*
                                        uint64_t* lds_bytes);
 midint_status midint_ragged_plan_destroy(midint_ragged_plan* plan);
 
+/* --------------------------------------------------------- serving loop */
+/* Continuous batching of single add / mul requests (operands 1..4096 limbs).
+ * Requests queue at submit; midint_server_poll() (non-blocking) retires
+ * finished micro-batches, admits queued requests into new micro-batches
+ * while their device region fits the free part of the HBM pool (requests
+ * that do not fit wait; none is dropped), and launches them on free streams,
+ * so later requests overlap on the device with earlier ones. Results are
+ * exact (full sums and products) and independent of batching. One thread
+ * drives a server. No reference counterpart (the reference has no serving
+ * path). */
+typedef struct {
+    uint64_t hbm_budget_bytes; /* device pool for in-flight batches (0: 1 GiB) */
+    uint64_t max_batch_bytes;  /* device bytes of one micro-batch (0: 64 MiB) */
+    uint32_t streams;          /* micro-batches in flight, 1..8 (0: 4) */
+    uint32_t max_batch_ops;    /* requests per micro-batch (0: 8192) */
+} midint_server_config;
+
+typedef struct {
+    uint64_t submitted, completed, queued;
+    uint64_t inflight_batches, batches_launched, ops_in_batches;
+    uint64_t pool_bytes_in_use, pool_bytes_total;
+    uint64_t deferred_admissions; /* admission attempts stopped by the budget */
+} midint_server_counters;
+
+typedef struct midint_server midint_server;
+
+midint_status midint_server_create(midint_ctx* ctx, const midint_server_config* cfg, midint_server** server);
+/* Copies the operands and queues the request; *id identifies it. */
+midint_status midint_server_submit(midint_server* s, uint32_t kind, const uint64_t* a, uint32_t n_a,
+                                   const uint64_t* b, uint32_t n_b, uint64_t* id);
+/* Drives the scheduler once. Up to `cap` ids of requests completed in this
+ * call go to done_ids; *n_done receives their number. */
+midint_status midint_server_poll(midint_server* s, uint64_t* done_ids, size_t cap, size_t* n_done);
+/* Copies a completed request's result (n_a + n_b limbs for mul,
+ * max(n_a, n_b) + 1 for add) and releases it; *latency_s = completion time
+ * minus submission time. MIDINT_EBUSY while it is still in flight. */
+midint_status midint_server_result(midint_server* s, uint64_t id, uint64_t* r, size_t cap_limbs, size_t* n_limbs,
+                                   double* latency_s);
+midint_status midint_server_stats(const midint_server* s, midint_server_counters* out);
+midint_status midint_server_destroy(midint_server* s);
+
 /* ------------------------------------------------- host-pointer drop-ins */
 /* Mirror the reference host drivers: allocate device buffers, copy inputs,
  * run the operation `runs` times back to back (after one untimed run),

@@ -62,6 +62,13 @@ _SIGS = {
     "midint_ragged_plan_stats": (_int, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
                                         ctypes.POINTER(ctypes.c_uint64)]),
     "midint_ragged_plan_destroy": (_int, [_vp]),
+    "midint_server_create": (_int, [_vp, _vp, ctypes.POINTER(_vp)]),
+    "midint_server_submit": (_int, [_vp, _u32, _vp, _u32, _vp, _u32, ctypes.POINTER(ctypes.c_uint64)]),
+    "midint_server_poll": (_int, [_vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
+    "midint_server_result": (_int, [_vp, ctypes.c_uint64, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
+                                    _dblp]),
+    "midint_server_stats": (_int, [_vp, _vp]),
+    "midint_server_destroy": (_int, [_vp]),
     "midint_gen_limbs": (None, [ctypes.c_uint64, _vp, ctypes.c_size_t]),
     "midint_gen_ref_rand": (None, [_u32, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _vp]),
 }
@@ -243,6 +250,108 @@ class RaggedPlan:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- serving
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget_bytes", ctypes.c_uint64), ("max_batch_bytes", ctypes.c_uint64),
+                ("streams", ctypes.c_uint32), ("max_batch_ops", ctypes.c_uint32)]
+
+
+class ServerCounters(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("submitted", "completed", "queued", "inflight_batches",
+                                                 "batches_launched", "ops_in_batches", "pool_bytes_in_use",
+                                                 "pool_bytes_total", "deferred_admissions")]
+
+
+class Server:
+    """midint_server: continuous batching of single add / mul requests."""
+
+    def __init__(self, ctx, hbm_budget=0, max_batch_bytes=0, streams=0, max_batch_ops=0):
+        cfg = ServerConfig(hbm_budget, max_batch_bytes, streams, max_batch_ops)
+        self._h = ctypes.c_void_p()
+        self._ids = (ctypes.c_uint64 * 65536)()
+        check(lib().midint_server_create(ctx.handle, ctypes.byref(cfg), ctypes.byref(self._h)),
+              "midint_server_create")
+
+    def submit(self, kind, a, b):
+        rid = ctypes.c_uint64()
+        check(lib().midint_server_submit(self._h, kind, _ptr(a), a.size, _ptr(b), b.size, ctypes.byref(rid)),
+              "midint_server_submit")
+        return rid.value
+
+    def poll(self):
+        n = ctypes.c_size_t()
+        check(lib().midint_server_poll(self._h, self._ids, len(self._ids), ctypes.byref(n)), "midint_server_poll")
+        return list(self._ids[: n.value])
+
+    def result(self, rid, n_limbs):
+        import numpy as np
+
+        out = np.empty(n_limbs, dtype=np.uint64)
+        got = ctypes.c_size_t()
+        lat = ctypes.c_double()
+        check(lib().midint_server_result(self._h, rid, _ptr(out), n_limbs, ctypes.byref(got), ctypes.byref(lat)),
+              "midint_server_result")
+        return out[: got.value], lat.value
+
+    def stats(self):
+        c = ServerCounters()
+        check(lib().midint_server_stats(self._h, ctypes.byref(c)), "midint_server_stats")
+        return {n: getattr(c, n) for n, _ in ServerCounters._fields_}
+
+    def close(self):
+        if self._h:
+            lib().midint_server_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s=120.0):
+    """Drives `server` with `requests` [(kind, a, b)] arriving as a Poisson
+    stream of rate `qps`. Latency of a request = completion - scheduled
+    arrival (host monotonic clock; includes time waiting in the queue).
+    Returns dict(p50_ms, p99_ms, mean_ms, ops_per_s, results{index: limbs})."""
+    import time
+
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    n = len(requests)
+    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    out_len = [(len(a) + len(b)) if k == 1 else max(len(a), len(b)) + 1 for k, a, b in requests]
+    id2idx, lat, results = {}, np.zeros(n), {}
+    submit_late = np.zeros(n)
+    i = done = 0
+    t0 = time.perf_counter()
+    last = t0
+    while done < n:
+        now = time.perf_counter() - t0
+        if now > time_limit_s:
+            raise RuntimeError(f"serve_poisson: {done}/{n} done after {time_limit_s} s")
+        while i < n and arrivals[i] <= now:
+            k, a, b = requests[i]
+            rid = server.submit(k, a, b)
+            id2idx[rid] = i
+            submit_late[i] = (time.perf_counter() - t0) - arrivals[i]
+            i += 1
+        for rid in server.poll():
+            j = id2idx.pop(rid)
+            r, l_c = server.result(rid, out_len[j])
+            lat[j] = submit_late[j] + l_c
+            if keep_results:
+                results[j] = r
+            done += 1
+            last = time.perf_counter()
+    span = last - (t0 + arrivals[0])
+    return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+            "mean_ms": float(lat.mean() * 1e3), "ops_per_s": n / span, "offered_qps": qps, "requests": n,
+            "results": results}
 
 
 # ---------------------------------------------------------------- host side

@@ -195,6 +195,12 @@ static midint_status get_tables(midint_ctx* ctx, uint32_t lg, NttTables* out) {
     return MIDINT_OK;
 }
 
+}  // extern "C"
+
+midint_status midint::ctx_ntt_tables(midint_ctx* ctx, uint32_t lg, NttTables* out) { return get_tables(ctx, lg, out); }
+
+extern "C" {
+
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m) {
     if (!ctx || m == 0) return MIDINT_EINVAL;
     if (m > midint_max_limbs_ntt()) return MIDINT_EUNSUPPORTED;

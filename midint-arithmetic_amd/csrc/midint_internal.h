@@ -6,6 +6,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <vector>
+
+#include "midint.h"
+
 namespace midint {
 
 struct AddGeom {
@@ -69,7 +73,18 @@ struct RgItem {
 };
 
 size_t ragged_lds_size(uint32_t threads, uint32_t conv_cap);
+// Packs ops (offsets relative to base_a / base_b / base_r) into work items.
+midint_status ragged_pack(const midint_op* ops, size_t n_ops, uint64_t base_a, uint64_t base_b, uint64_t base_r,
+                          std::vector<RgItem>& items, std::vector<RgOp>& rgops, uint32_t* conv_cap);
 const void* ragged_kernel_ptr(uint32_t threads);
 const void* ragged_reset_ptr();
+
+// Serving: products of operands above 2048 limbs are split into four
+// 2048 x 2048 NTT products [P00 | P11 | P01 | P10] (4096 limbs each); the
+// combine kernel adds P01 + P10 into limbs [2048, 8192) of P00 | P11.
+hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s);
+
+// NTT twiddle tables of a context (cached).
+midint_status ctx_ntt_tables(midint_ctx* ctx, uint32_t lg, NttTables* out);
 
 }  // namespace midint

@@ -36,11 +36,6 @@ constexpr uint32_t kConvLimit = 4 * kRgWindow;  // words: the L/H window area is
 
 uint32_t words_pad(uint32_t limbs) { return (2 * limbs + kMulQ - 1) / kMulQ * kMulQ; }
 
-struct Cand {
-    RgOp op;
-    uint64_t cost;
-};
-
 void free_plan(midint_ragged_plan* p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
@@ -53,15 +48,13 @@ void free_plan(midint_ragged_plan* p) {
 
 }  // namespace
 
-extern "C" {
+namespace midint {
 
-uint32_t midint_max_limbs_ragged(void) { return kRgMaxLimbs; }
-
-midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, size_t n_ops, const uint64_t* d_a,
-                                        const uint64_t* d_b, uint64_t* d_r, const midint_ragged_tuning* tuning,
-                                        midint_ragged_plan** out) {
-    if (!ctx || !out || (n_ops > 0 && (!ops || !d_a || !d_b || !d_r))) return MIDINT_EINVAL;
-    *out = nullptr;
+// Packs ops into work items (see ragged.hip). Returns EINVAL / EUNSUPPORTED
+// for malformed or oversized ops. Offsets in the returned RgOps are the ops'
+// own limb offsets plus the given bases.
+midint_status ragged_pack(const midint_op* ops, size_t n_ops, uint64_t base_a, uint64_t base_b, uint64_t base_r,
+                          std::vector<RgItem>& h_items, std::vector<RgOp>& h_ops, uint32_t* conv_cap_out) {
     constexpr uint32_t Q = kMulQ;
     std::vector<RgOp> muls, adds;
     for (size_t i = 0; i < n_ops; i++) {
@@ -70,11 +63,11 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         if (o.n_a == 0 || o.n_b == 0) return MIDINT_EINVAL;
         if (o.n_a > kRgMaxLimbs || o.n_b > kRgMaxLimbs) return MIDINT_EUNSUPPORTED;
         RgOp r = {};
-        r.off_r = o.off_r;
+        r.off_r = base_r + o.off_r;
         if (o.kind == MIDINT_OP_MUL) {
             r.swap = o.n_a > o.n_b;
-            r.off_s = r.swap ? o.off_b : o.off_a;
-            r.off_l = r.swap ? o.off_a : o.off_b;
+            r.off_s = r.swap ? base_b + o.off_b : base_a + o.off_a;
+            r.off_l = r.swap ? base_a + o.off_a : base_b + o.off_b;
             r.ns = r.swap ? o.n_b : o.n_a;
             r.nl = r.swap ? o.n_a : o.n_b;
             r.ns_w = words_pad(r.ns);
@@ -84,8 +77,8 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
             r.out_len = o.n_a + o.n_b;
             muls.push_back(r);
         } else {
-            r.off_s = o.off_a;
-            r.off_l = o.off_b;
+            r.off_s = base_a + o.off_a;
+            r.off_l = base_b + o.off_b;
             r.ns = o.n_a;
             r.nl = o.n_b;
             r.seg_len = std::max(o.n_a, o.n_b) + 1;
@@ -93,7 +86,6 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
             adds.push_back(r);
         }
     }
-
     // ---- pack multiplications: sorted by the loop length (ns), then nl
     std::sort(muls.begin(), muls.end(), [](const RgOp& x, const RgOp& y) {
         return x.ns_w != y.ns_w ? x.ns_w < y.ns_w : x.nl_w < y.nl_w;
@@ -150,14 +142,35 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     std::vector<size_t> order(items.size());
     for (size_t i = 0; i < order.size(); i++) order[i] = i;
     std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return cost[x] > cost[y]; });
-    std::vector<RgItem> h_items;
-    std::vector<RgOp> h_ops;
+    h_items.clear();
+    h_ops.clear();
     for (size_t k : order) {
         RgItem it = items[k];
         it.first = (uint32_t)h_ops.size();
         h_items.push_back(it);
         h_ops.insert(h_ops.end(), item_ops[k].begin(), item_ops[k].end());
     }
+
+    *conv_cap_out = (conv_cap + 3) / 4 * 4;
+    return MIDINT_OK;
+}
+
+}  // namespace midint
+
+extern "C" {
+
+uint32_t midint_max_limbs_ragged(void) { return kRgMaxLimbs; }
+
+midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, size_t n_ops, const uint64_t* d_a,
+                                        const uint64_t* d_b, uint64_t* d_r, const midint_ragged_tuning* tuning,
+                                        midint_ragged_plan** out) {
+    if (!ctx || !out || (n_ops > 0 && (!ops || !d_a || !d_b || !d_r))) return MIDINT_EINVAL;
+    *out = nullptr;
+    std::vector<RgItem> h_items;
+    std::vector<RgOp> h_ops;
+    uint32_t conv_cap = 0;
+    midint_status st = ragged_pack(ops, n_ops, 0, 0, 0, h_items, h_ops, &conv_cap);
+    if (st != MIDINT_OK) return st;
 
     midint_ragged_plan* p = new (std::nothrow) midint_ragged_plan;
     if (!p) return MIDINT_ENOMEM;
@@ -168,7 +181,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     }
     p->threads = (tuning && tuning->threads == 512) ? 512 : 256;
     p->n_items = (uint32_t)h_items.size();
-    p->conv_cap = (conv_cap + 3) / 4 * 4;
+    p->conv_cap = conv_cap;
     p->lds = ragged_lds_size(p->threads, p->conv_cap);
     p->A = d_a;
     p->B = d_b;

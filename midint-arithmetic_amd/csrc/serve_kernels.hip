@@ -1,0 +1,57 @@
+// Device helpers of the serving loop.
+//
+// bigmul_combine: a product of operands up to 4096 limbs is computed as four
+// 2048 x 2048-limb NTT products laid out per op as [P00 | P11 | P01 | P10]
+// (4096 limbs each, 16384 per op). With X = 2^(64*2048),
+//   a*b = P00 + (P01 + P10) X + P11 X^2,
+// and P00 | P11 already is P00 + P11 X^2 (disjoint limbs), so the combine
+// adds P01 and then P10 into limbs [2048, 8192): two chained carry-resolved
+// additions on register-resident rows (6144 limbs = 16 waves x 3 rows).
+#include "add_core.hpp"
+#include "midint_internal.h"
+
+namespace midint {
+
+__global__ __launch_bounds__(1024) void bigmul_combine_kernel(uint64_t* __restrict__ blocks) {
+    constexpr int R = 3;
+    constexpr uint32_t W = 6144;
+    __shared__ uint32_t lds_gp[16];
+    uint64_t* base = blocks + (size_t)blockIdx.x * 16384;
+    uint64_t* dst = base + 2048;
+    const uint64_t* p01 = base + 8192;
+    const uint64_t* p10 = base + 12288;
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<R> t;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        t.x0[v] = k < W ? dst[k] : 0ull;
+        t.x1[v] = k + 1 < W ? dst[k + 1] : 0ull;
+        t.y0[v] = k < 4096 ? p01[k] : 0ull;
+        t.y1[v] = k + 1 < 4096 ? p01[k + 1] : 0ull;
+        t.s0[v] = (k == 0);
+        t.s1[v] = false;
+    }
+    block_add_rows<R>(t, lds_gp);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        t.y0[v] = k < 4096 ? p10[k] : 0ull;
+        t.y1[v] = k + 1 < 4096 ? p10[k + 1] : 0ull;
+    }
+    block_add_rows<R>(t, lds_gp);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        if (k < W) dst[k] = t.x0[v];
+        if (k + 1 < W) dst[k + 1] = t.x1[v];
+    }
+}
+
+hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    bigmul_combine_kernel<<<n, 1024, 0, s>>>(blocks16k);
+    return hipGetLastError();
+}
+
+}  // namespace midint

@@ -1,0 +1,597 @@
+// Continuous-batching serving loop (midint_server_*).
+//
+// Requests (one add or mul each, operands 1..4096 limbs) are queued on the
+// host at submit time. poll() drives the scheduler without blocking:
+//
+//  1. retire: micro-batches whose completion event has fired hand their
+//     results to their requests (latency = completion - submission) and give
+//     their device region back to the pool;
+//  2. admit: while a stream slot is free and requests are queued, a new
+//     micro-batch takes queued requests in arrival order as long as its
+//     device region fits the free part of the HBM pool (the admission
+//     budget) and the staging limits; a request that does not fit waits
+//     (deferred, never dropped);
+//  3. launch: the batch's operands go to HBM with one async copy from its
+//     pinned staging buffer, its kernels run on the slot's stream, results
+//     come back with one async copy, and an event marks completion.
+//
+// Several micro-batches are in flight on different streams, so work of
+// requests admitted later overlaps on the device with work admitted earlier.
+// Kernels per micro-batch: the ragged persistent kernel for ops whose
+// operands are <= 1024 limbs; the add kernel for larger additions (padded
+// to a 2048- or 4096-limb class, carry-out kept); the NTT kernel for larger
+// products (padded to 2048 limbs, or split into four 2048-limb products plus
+// the combine kernel above 2048 limbs). Every path is exact, so a request's
+// result does not depend on what it was batched with.
+#include <hip/hip_runtime.h>
+#include <string.h>
+#include <time.h>
+
+#include <algorithm>
+#include <deque>
+#include <map>
+#include <new>
+#include <unordered_map>
+#include <vector>
+
+#include "midint.h"
+#include "midint_internal.h"
+#include "mul_core.hpp"
+
+using namespace midint;
+
+namespace {
+
+constexpr uint32_t kServeMaxLimbs = 4096;
+constexpr uint32_t kNttM = 2048;
+
+double now_s() {
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
+
+struct Request {
+    uint32_t kind, na, nb;
+    std::vector<uint64_t> a, b;   // host copies until the request is staged
+    std::vector<uint64_t> result; // filled at completion
+    double t_submit = 0, t_done = 0;
+    bool done = false;
+};
+
+// Where a request's result sits in the batch's output staging.
+struct ResultRef {
+    uint64_t id;
+    size_t out_off;  // limbs from the start of the output area
+    uint32_t len;    // limbs of the result
+    int carry_idx;   // >= 0: result limb len-1 is this carry byte (big adds at the class size)
+    uint32_t carry_class;
+};
+
+enum class Route { Ragged, AddClass2048, AddClass4096, Ntt2048, Ntt4096 };
+
+Route route_of(uint32_t kind, uint32_t na, uint32_t nb) {
+    const uint32_t mx = std::max(na, nb);
+    if (mx <= kRgMaxLimbs) return Route::Ragged;
+    if (kind == MIDINT_OP_ADD) return mx <= 2048 ? Route::AddClass2048 : Route::AddClass4096;
+    return mx <= kNttM ? Route::Ntt2048 : Route::Ntt4096;
+}
+
+// Device bytes a request needs inside a batch region (inputs + outputs,
+// upper bound including its share of the ragged metadata).
+size_t request_bytes(uint32_t kind, uint32_t na, uint32_t nb) {
+    switch (route_of(kind, na, nb)) {
+        case Route::Ragged:
+            return 8ull * (na + nb + (kind == MIDINT_OP_MUL ? na + nb : std::max(na, nb) + 1)) + 2 * sizeof(RgOp);
+        case Route::AddClass2048: return 8ull * 3 * 2048 + 1;
+        case Route::AddClass4096: return 8ull * 3 * 4096 + 1;
+        case Route::Ntt2048: return 8ull * (2 * kNttM + 2 * kNttM);
+        case Route::Ntt4096: return 4 * 8ull * (2 * kNttM + 2 * kNttM);
+    }
+    return 0;
+}
+
+struct Pool {  // first-fit allocator over one device allocation
+    uint8_t* base = nullptr;
+    size_t total = 0, used = 0;
+    std::map<size_t, size_t> free_;  // offset -> size
+    void init(uint8_t* b, size_t n) {
+        base = b;
+        total = n;
+        free_.clear();
+        free_[0] = n;
+        used = 0;
+    }
+    bool alloc(size_t n, size_t* off) {
+        n = align_up(n, 256);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second >= n) {
+                *off = it->first;
+                const size_t rest = it->second - n;
+                const size_t noff = it->first + n;
+                free_.erase(it);
+                if (rest) free_[noff] = rest;
+                used += n;
+                return true;
+            }
+        }
+        return false;
+    }
+    size_t largest() const {
+        size_t m = 0;
+        for (auto& kv : free_) m = std::max(m, kv.second);
+        return m;
+    }
+    void release(size_t off, size_t n) {
+        n = align_up(n, 256);
+        used -= n;
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+struct Slot {
+    hipStream_t stream = nullptr;
+    hipEvent_t ev = nullptr;
+    uint8_t* pinned = nullptr;  // staging: inputs then outputs
+    size_t pinned_bytes = 0;
+    bool busy = false;
+    // in-flight batch
+    size_t region_off = 0, region_bytes = 0, out_off_bytes = 0, out_bytes = 0, carry_off_bytes = 0;
+    std::vector<ResultRef> refs;
+    std::vector<uint64_t> ids;
+};
+
+}  // namespace
+
+struct midint_server {
+    midint_ctx* ctx = nullptr;
+    int device = 0;
+    uint64_t max_batch_bytes = 0;
+    uint32_t max_batch_ops = 0;
+    Pool pool;
+    uint8_t* d_pool = nullptr;
+    std::vector<Slot> slots;
+    std::deque<uint64_t> queue;
+    std::unordered_map<uint64_t, Request> reqs;
+    uint64_t next_id = 1;
+    midint_server_counters st = {};
+    uint32_t cus = 0;
+    int ragged_per_cu = 1;
+    size_t ragged_lds = 0;
+};
+
+namespace {
+
+void destroy_server(midint_server* s) {
+    if (!s) return;
+    for (Slot& sl : s->slots) {
+        if (sl.stream) {
+            (void)hipStreamSynchronize(sl.stream);
+            (void)hipStreamDestroy(sl.stream);
+        }
+        if (sl.ev) (void)hipEventDestroy(sl.ev);
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+    }
+    if (s->d_pool) (void)hipFree(s->d_pool);
+    delete s;
+}
+
+// Builds and launches one micro-batch on slot `si` from the queue head.
+// Returns MIDINT_EBUSY when the head request does not fit the free pool.
+midint_status launch_batch(midint_server* s, size_t si) {
+    Slot& sl = s->slots[si];
+    // ---- choose requests in arrival order
+    std::vector<uint64_t> take;
+    size_t est = 64 * 1024;  // metadata slack
+    const size_t free_max = s->pool.largest();
+    while (!s->queue.empty() && take.size() < s->max_batch_ops) {
+        const Request& r = s->reqs[s->queue.front()];
+        const size_t need = request_bytes(r.kind, r.na, r.nb);
+        if (est + need > s->max_batch_bytes || est + need > free_max) break;
+        est += need;
+        take.push_back(s->queue.front());
+        s->queue.pop_front();
+    }
+    if (take.empty()) {
+        s->st.deferred_admissions++;
+        return MIDINT_EBUSY;
+    }
+
+    // ---- lay out the batch: [inputs | outputs] in limbs
+    std::vector<midint_op> rops;        // ragged ops (offsets filled below)
+    std::vector<uint64_t> rids;
+    std::vector<uint64_t> add2k, add4k, ntt2k, ntt4k;
+    for (uint64_t id : take) {
+        Request& r = s->reqs[id];
+        switch (route_of(r.kind, r.na, r.nb)) {
+            case Route::Ragged: rids.push_back(id); break;
+            case Route::AddClass2048: add2k.push_back(id); break;
+            case Route::AddClass4096: add4k.push_back(id); break;
+            case Route::Ntt2048: ntt2k.push_back(id); break;
+            case Route::Ntt4096: ntt4k.push_back(id); break;
+        }
+    }
+    // inputs
+    size_t in = 0;
+    const size_t rg_a = in;
+    for (uint64_t id : rids) {
+        Request& r = s->reqs[id];
+        midint_op o = {r.kind, r.na, r.nb, 0, in - rg_a, 0, 0};
+        rops.push_back(o);
+        in += r.na;
+    }
+    const size_t rg_b = in;
+    for (size_t k = 0; k < rids.size(); k++) {
+        rops[k].off_b = in - rg_b;
+        in += s->reqs[rids[k]].nb;
+    }
+    const size_t a2k = in;
+    in += (size_t)add2k.size() * 2048 * 2;
+    const size_t a4k = in;
+    in += (size_t)add4k.size() * 4096 * 2;
+    const size_t nttx = in;
+    const size_t n_ntt = ntt2k.size() + 4 * ntt4k.size();
+    in += n_ntt * kNttM * 2;
+    // ragged metadata (as limbs)
+    std::vector<RgItem> items;
+    std::vector<RgOp> rgops;
+    uint32_t conv_cap = 0;
+    // outputs
+    size_t out = 0;
+    const size_t rg_r = out;
+    for (size_t k = 0; k < rids.size(); k++) {
+        Request& r = s->reqs[rids[k]];
+        rops[k].off_r = out - rg_r;
+        out += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
+    }
+    const size_t a2k_r = out;
+    out += add2k.size() * 2048;
+    const size_t a4k_r = out;
+    out += add4k.size() * 4096;
+    const size_t ntt_r = out;
+    out += n_ntt * 2 * kNttM;
+    const size_t carry_bytes = align_up(add2k.size() + add4k.size(), 8);
+    const size_t items_off = in;  // limbs
+    midint_status ps = ragged_pack(rops.data(), rops.size(), 0, 0, 0, items, rgops, &conv_cap);
+    if (ps != MIDINT_OK) return ps;
+    const size_t meta_limbs =
+        align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 8) / 8 + 1 /* counter */;
+    in += meta_limbs;
+    in = align_up(in, 32);
+    const size_t in_bytes = in * 8;
+    const size_t out_bytes = out * 8 + carry_bytes;
+    const size_t region = in_bytes + out_bytes;
+    if (region > sl.pinned_bytes) {
+        // rare: staging too small for this batch; grow it
+        if (sl.pinned) (void)hipHostFree(sl.pinned);
+        sl.pinned = nullptr;
+        sl.pinned_bytes = 0;
+        if (hipHostMalloc((void**)&sl.pinned, region, 0) != hipSuccess) {
+            for (auto it = take.rbegin(); it != take.rend(); ++it) s->queue.push_front(*it);
+            return MIDINT_ENOMEM;
+        }
+        sl.pinned_bytes = region;
+    }
+    size_t off = 0;
+    if (!s->pool.alloc(region, &off)) {
+        for (auto it = take.rbegin(); it != take.rend(); ++it) s->queue.push_front(*it);
+        s->st.deferred_admissions++;
+        return MIDINT_EBUSY;
+    }
+
+    // ---- fill the staging inputs
+    uint64_t* h = reinterpret_cast<uint64_t*>(sl.pinned);
+    memset(h + a2k, 0, (in - a2k) * 8);
+    for (size_t k = 0; k < rids.size(); k++) {
+        Request& r = s->reqs[rids[k]];
+        memcpy(h + rg_a + rops[k].off_a, r.a.data(), 8ull * r.na);
+        memcpy(h + rg_b + rops[k].off_b, r.b.data(), 8ull * r.nb);
+    }
+    for (size_t k = 0; k < add2k.size(); k++) {
+        Request& r = s->reqs[add2k[k]];
+        memcpy(h + a2k + k * 2048, r.a.data(), 8ull * r.na);
+        memcpy(h + a2k + (add2k.size() + k) * 2048, r.b.data(), 8ull * r.nb);
+    }
+    for (size_t k = 0; k < add4k.size(); k++) {
+        Request& r = s->reqs[add4k[k]];
+        memcpy(h + a4k + k * 4096, r.a.data(), 8ull * r.na);
+        memcpy(h + a4k + (add4k.size() + k) * 4096, r.b.data(), 8ull * r.nb);
+    }
+    {
+        uint64_t* X = h + nttx;
+        uint64_t* Y = h + nttx + n_ntt * kNttM;
+        size_t inst = 0;
+        for (uint64_t id : ntt2k) {
+            Request& r = s->reqs[id];
+            memcpy(X + inst * kNttM, r.a.data(), 8ull * r.na);
+            memcpy(Y + inst * kNttM, r.b.data(), 8ull * r.nb);
+            inst++;
+        }
+        for (uint64_t id : ntt4k) {
+            Request& r = s->reqs[id];
+            // halves: a = a1 X + a0, b = b1 X + b0; instances a0b0, a1b1, a0b1, a1b0
+            const uint32_t a0 = std::min(r.na, kNttM), a1 = r.na > kNttM ? r.na - kNttM : 0;
+            const uint32_t b0 = std::min(r.nb, kNttM), b1 = r.nb > kNttM ? r.nb - kNttM : 0;
+            const uint64_t* pa0 = r.a.data();
+            const uint64_t* pa1 = r.a.data() + a0;
+            const uint64_t* pb0 = r.b.data();
+            const uint64_t* pb1 = r.b.data() + b0;
+            const uint64_t* xs[4] = {pa0, pa1, pa0, pa1};
+            const uint32_t xl[4] = {a0, a1, a0, a1};
+            const uint64_t* ys[4] = {pb0, pb1, pb1, pb0};
+            const uint32_t yl[4] = {b0, b1, b1, b0};
+            for (int q = 0; q < 4; q++) {
+                if (xl[q]) memcpy(X + inst * kNttM, xs[q], 8ull * xl[q]);
+                if (yl[q]) memcpy(Y + inst * kNttM, ys[q], 8ull * yl[q]);
+                inst++;
+            }
+        }
+    }
+    uint8_t* meta = reinterpret_cast<uint8_t*>(h + items_off);
+    memcpy(meta, items.data(), items.size() * sizeof(RgItem));
+    memcpy(meta + items.size() * sizeof(RgItem), rgops.data(), rgops.size() * sizeof(RgOp));
+
+    // ---- device pointers
+    uint8_t* dreg = s->pool.base + off;
+    uint64_t* d_in = reinterpret_cast<uint64_t*>(dreg);
+    uint64_t* d_out = reinterpret_cast<uint64_t*>(dreg + in_bytes);
+    uint8_t* d_carry = dreg + in_bytes + out * 8;
+    RgItem* d_items = reinterpret_cast<RgItem*>(d_in + items_off);
+    RgOp* d_rgops = reinterpret_cast<RgOp*>(reinterpret_cast<uint8_t*>(d_items) + items.size() * sizeof(RgItem));
+    uint32_t* d_counter = reinterpret_cast<uint32_t*>(d_in + items_off + meta_limbs - 1);
+
+    hipStream_t st = sl.stream;
+    hipError_t e = hipMemcpyAsync(d_in, h, in_bytes, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !items.empty()) {
+        e = hipMemsetAsync(d_counter, 0, sizeof(uint32_t), st);
+        const uint32_t n_items = (uint32_t)items.size();
+        const uint64_t* dA = d_in + rg_a;
+        const uint64_t* dB = d_in + rg_b;
+        uint64_t* dR = d_out + rg_r;
+        uint32_t cap = conv_cap;
+        void* args[] = {(void*)&d_items, (void*)&d_rgops, (void*)&n_items, (void*)&d_counter,
+                        (void*)&dA,      (void*)&dB,      (void*)&dR,      (void*)&cap};
+        const size_t lds = ragged_lds_size(256, cap);
+        const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, s->cus * s->ragged_per_cu));
+        if (e == hipSuccess)
+            e = hipLaunchKernel(ragged_kernel_ptr(256), dim3(grid), dim3(256), args, lds, st);
+    }
+    if (e == hipSuccess && !add2k.empty())
+        e = launch_add((uint32_t)add2k.size(), 2048, d_in + a2k, d_in + a2k + add2k.size() * 2048, d_out + a2k_r,
+                       d_carry, false, st);
+    if (e == hipSuccess && !add4k.empty())
+        e = launch_add((uint32_t)add4k.size(), 4096, d_in + a4k, d_in + a4k + add4k.size() * 4096, d_out + a4k_r,
+                       d_carry + add2k.size(), false, st);
+    if (e == hipSuccess && n_ntt > 0) {
+        NttTables t;
+        midint_status ts = ctx_ntt_tables(s->ctx, ntt_lg_for_limbs(kNttM), &t);
+        if (ts != MIDINT_OK) e = hipErrorUnknown;
+        if (e == hipSuccess)
+            e = launch_mul_ntt((uint32_t)n_ntt, kNttM, d_in + nttx, d_in + nttx + n_ntt * kNttM, d_out + ntt_r, t,
+                               st);
+        if (e == hipSuccess && !ntt4k.empty())
+            e = launch_bigmul_combine((uint32_t)ntt4k.size(), d_out + ntt_r + ntt2k.size() * 2 * kNttM, st);
+    }
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(sl.pinned + in_bytes, d_out, out_bytes, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipEventRecord(sl.ev, st);
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(st);
+        s->pool.release(off, region);
+        for (auto it = take.rbegin(); it != take.rend(); ++it) s->queue.push_front(*it);
+        return MIDINT_EHIP;
+    }
+
+    // ---- bookkeeping for retirement
+    sl.busy = true;
+    sl.region_off = off;
+    sl.region_bytes = region;
+    sl.out_off_bytes = in_bytes;
+    sl.out_bytes = out_bytes;
+    sl.carry_off_bytes = in_bytes + out * 8;
+    sl.refs.clear();
+    sl.ids = take;
+    for (size_t k = 0; k < rids.size(); k++) {
+        Request& r = s->reqs[rids[k]];
+        const uint32_t len = r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
+        sl.refs.push_back({rids[k], rg_r + rops[k].off_r, len, -1, 0});
+    }
+    for (size_t k = 0; k < add2k.size(); k++) {
+        Request& r = s->reqs[add2k[k]];
+        const uint32_t len = std::max(r.na, r.nb) + 1;
+        sl.refs.push_back({add2k[k], a2k_r + k * 2048, len, len - 1 == 2048 ? (int)k : -1, 2048});
+    }
+    for (size_t k = 0; k < add4k.size(); k++) {
+        Request& r = s->reqs[add4k[k]];
+        const uint32_t len = std::max(r.na, r.nb) + 1;
+        sl.refs.push_back({add4k[k], a4k_r + k * 4096, len, len - 1 == 4096 ? (int)(add2k.size() + k) : -1, 4096});
+    }
+    size_t inst = 0;
+    for (uint64_t id : ntt2k) {
+        Request& r = s->reqs[id];
+        sl.refs.push_back({id, ntt_r + inst * 2 * kNttM, r.na + r.nb, -1, 0});
+        inst++;
+    }
+    for (uint64_t id : ntt4k) {
+        Request& r = s->reqs[id];
+        sl.refs.push_back({id, ntt_r + inst * 2 * kNttM, r.na + r.nb, -1, 0});
+        inst += 4;
+    }
+    for (uint64_t id : take) {
+        Request& r = s->reqs[id];
+        r.a.clear();
+        r.a.shrink_to_fit();
+        r.b.clear();
+        r.b.shrink_to_fit();
+    }
+    s->st.batches_launched++;
+    s->st.inflight_batches++;
+    s->st.ops_in_batches += take.size();
+    return MIDINT_OK;
+}
+
+void retire(midint_server* s, Slot& sl, double t, std::vector<uint64_t>& done) {
+    const uint64_t* outp = reinterpret_cast<const uint64_t*>(sl.pinned + sl.out_off_bytes);
+    const uint8_t* carries = sl.pinned + sl.carry_off_bytes;
+    for (const ResultRef& rr : sl.refs) {
+        Request& r = s->reqs[rr.id];
+        if (rr.carry_idx >= 0) {
+            // full-width sum at its class size: the top limb is the kernel's carry-out
+            r.result.assign(outp + rr.out_off, outp + rr.out_off + rr.len - 1);
+            r.result.push_back(carries[rr.carry_idx]);
+        } else {
+            r.result.assign(outp + rr.out_off, outp + rr.out_off + rr.len);
+        }
+        r.done = true;
+        r.t_done = t;
+        done.push_back(rr.id);
+    }
+    s->pool.release(sl.region_off, sl.region_bytes);
+    sl.busy = false;
+    s->st.inflight_batches--;
+    s->st.completed += sl.refs.size();
+}
+
+}  // namespace
+
+extern "C" {
+
+midint_status midint_server_create(midint_ctx* ctx, const midint_server_config* cfg, midint_server** out) {
+    if (!ctx || !out) return MIDINT_EINVAL;
+    *out = nullptr;
+    midint_server* s = new (std::nothrow) midint_server;
+    if (!s) return MIDINT_ENOMEM;
+    s->ctx = ctx;
+    const uint64_t budget = (cfg && cfg->hbm_budget_bytes) ? cfg->hbm_budget_bytes : (1ull << 30);
+    s->max_batch_bytes = (cfg && cfg->max_batch_bytes) ? cfg->max_batch_bytes : (64ull << 20);
+    s->max_batch_ops = (cfg && cfg->max_batch_ops) ? cfg->max_batch_ops : 8192;
+    uint32_t nstreams = (cfg && cfg->streams) ? cfg->streams : 4;
+    if (nstreams > 8) nstreams = 8;
+    if (hipGetDevice(&s->device) != hipSuccess || hipMalloc((void**)&s->d_pool, budget) != hipSuccess) {
+        (void)hipGetLastError();
+        destroy_server(s);
+        return MIDINT_ENOMEM;
+    }
+    s->pool.init(s->d_pool, budget);
+    s->slots.resize(nstreams);
+    for (Slot& sl : s->slots) {
+        if (hipStreamCreateWithFlags(&sl.stream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&sl.ev, hipEventDisableTiming) != hipSuccess ||
+            hipHostMalloc((void**)&sl.pinned, s->max_batch_bytes + (4u << 20), 0) != hipSuccess) {
+            destroy_server(s);
+            return MIDINT_EHIP;
+        }
+        sl.pinned_bytes = s->max_batch_bytes + (4u << 20);
+    }
+    int cus = 0;
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device);
+    s->cus = (uint32_t)std::max(1, cus);
+    s->ragged_lds = ragged_lds_size(256, 4 * kRgWindow);
+    int per = 1;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, ragged_kernel_ptr(256), 256, s->ragged_lds) ==
+        hipSuccess)
+        s->ragged_per_cu = std::max(1, per);
+    NttTables t;
+    if (ctx_ntt_tables(ctx, ntt_lg_for_limbs(kNttM), &t) != MIDINT_OK) {
+        destroy_server(s);
+        return MIDINT_EHIP;
+    }
+    s->st.pool_bytes_total = budget;
+    *out = s;
+    return MIDINT_OK;
+}
+
+midint_status midint_server_submit(midint_server* s, uint32_t kind, const uint64_t* a, uint32_t n_a,
+                                   const uint64_t* b, uint32_t n_b, uint64_t* id) {
+    if (!s || !a || !b || !id || n_a == 0 || n_b == 0) return MIDINT_EINVAL;
+    if (kind != MIDINT_OP_ADD && kind != MIDINT_OP_MUL) return MIDINT_EINVAL;
+    if (n_a > kServeMaxLimbs || n_b > kServeMaxLimbs) return MIDINT_EUNSUPPORTED;
+    if (request_bytes(kind, n_a, n_b) + 64 * 1024 > std::min<uint64_t>(s->max_batch_bytes, s->pool.total))
+        return MIDINT_EUNSUPPORTED;
+    const uint64_t k = s->next_id++;
+    Request& r = s->reqs[k];
+    r.kind = kind;
+    r.na = n_a;
+    r.nb = n_b;
+    r.a.assign(a, a + n_a);
+    r.b.assign(b, b + n_b);
+    r.t_submit = now_s();
+    s->queue.push_back(k);
+    s->st.submitted++;
+    *id = k;
+    return MIDINT_OK;
+}
+
+midint_status midint_server_poll(midint_server* s, uint64_t* done_ids, size_t cap, size_t* n_done) {
+    if (!s) return MIDINT_EINVAL;
+    std::vector<uint64_t> done;
+    const double t = now_s();
+    for (Slot& sl : s->slots) {
+        if (!sl.busy) continue;
+        hipError_t q = hipEventQuery(sl.ev);
+        if (q == hipSuccess) {
+            retire(s, sl, t, done);
+        } else if (q != hipErrorNotReady) {
+            return MIDINT_EHIP;
+        }
+    }
+    for (size_t i = 0; i < s->slots.size() && !s->queue.empty(); i++) {
+        if (s->slots[i].busy) continue;
+        midint_status ls = launch_batch(s, i);
+        if (ls == MIDINT_EBUSY) break;
+        if (ls != MIDINT_OK) return ls;
+    }
+    s->st.queued = s->queue.size();
+    s->st.pool_bytes_in_use = s->pool.used;
+    const size_t k = std::min(cap, done.size());
+    if (done_ids)
+        for (size_t i = 0; i < k; i++) done_ids[i] = done[i];
+    if (n_done) *n_done = done_ids ? k : done.size();
+    // ids beyond `cap` are still reported done by midint_server_result
+    return MIDINT_OK;
+}
+
+midint_status midint_server_result(midint_server* s, uint64_t id, uint64_t* r, size_t cap_limbs, size_t* n_limbs,
+                                   double* latency_s) {
+    if (!s) return MIDINT_EINVAL;
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return MIDINT_ENOTFOUND;
+    Request& q = it->second;
+    if (!q.done) return MIDINT_EBUSY;
+    if (n_limbs) *n_limbs = q.result.size();
+    if (latency_s) *latency_s = q.t_done - q.t_submit;
+    if (r) {
+        if (cap_limbs < q.result.size()) return MIDINT_EINVAL;
+        memcpy(r, q.result.data(), q.result.size() * 8);
+    }
+    s->reqs.erase(it);
+    return MIDINT_OK;
+}
+
+midint_status midint_server_stats(const midint_server* s, midint_server_counters* out) {
+    if (!s || !out) return MIDINT_EINVAL;
+    *out = s->st;
+    return MIDINT_OK;
+}
+
+midint_status midint_server_destroy(midint_server* s) {
+    if (!s) return MIDINT_EINVAL;
+    destroy_server(s);
+    return MIDINT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+"""Serving loop: results of a Poisson stream under continuous batching equal
+the oracle's, requests give the same result alone and in the mix, and the
+HBM budget defers (never drops) requests."""
+import numpy as np
+import pytest
+
+from workloads import make_requests, request_reference
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx(mi):
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    c = mi.Context(0)
+    yield c
+    c.close()
+
+
+def test_stream_matches_oracle(mi, ctx, orc):
+    reqs = make_requests(mi.gen_limbs, 400, seed=5)
+    srv = mi.Server(ctx, hbm_budget=256 << 20, streams=4)
+    out = mi.serve_poisson(srv, reqs, qps=4000, seed=1)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["queued"] == 0
+    assert st["batches_launched"] > 1
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out["results"][j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
+
+
+def test_alone_equals_mixed(mi, ctx, orc):
+    reqs = make_requests(mi.gen_limbs, 120, seed=9)
+    srv = mi.Server(ctx, streams=3)
+    mixed = mi.serve_poisson(srv, reqs, qps=20000, seed=2)["results"]
+    srv.close()
+    for j in (0, 1, 17, 63, 119):
+        k, a, b = reqs[j]
+        solo = mi.Server(ctx, streams=1)
+        alone = mi.serve_poisson(solo, [reqs[j]], qps=1000, seed=3)["results"][0]
+        solo.close()
+        assert np.array_equal(alone, mixed[j])
+
+
+def test_edge_sizes_every_route(mi, ctx, orc):
+    """Both operands at the route boundaries: ragged (<= 1024 limbs), add
+    classes 2048 / 4096 (including exactly 2048 and 4096, where the carry is
+    the kernel's carry-out), NTT at <= 2048 and the four-way split above."""
+    MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+    rng = np.random.default_rng(4)
+    shapes = [(1, 1), (1024, 1024), (1025, 3), (2048, 2048), (2049, 17), (4096, 4096), (4096, 1), (3000, 2500),
+              (1, 4096), (2047, 2049)]
+    reqs = []
+    for kind in (0, 1):
+        for na, nb in shapes:
+            for fill in ("ones", "rand"):
+                a = np.full(na, MASK, dtype=np.uint64) if fill == "ones" else rng.integers(0, 2**64, na, dtype=np.uint64)
+                b = np.full(nb, MASK, dtype=np.uint64) if fill == "ones" else rng.integers(0, 2**64, nb, dtype=np.uint64)
+                reqs.append((kind, a, b))
+    srv = mi.Server(ctx, streams=2)
+    out = mi.serve_poisson(srv, reqs, qps=50000, seed=4)["results"]
+    srv.close()
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out[j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
+
+
+def test_budget_defers_not_drops(mi, ctx, orc):
+    """A pool that holds about one big request at a time still completes all."""
+    reqs = make_requests(mi.gen_limbs, 60, seed=13, lo_bits=16, hi_bits=18)
+    srv = mi.Server(ctx, hbm_budget=2 << 20, max_batch_bytes=1 << 20, streams=4)
+    out = mi.serve_poisson(srv, reqs, qps=100000, seed=5)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    assert st["deferred_admissions"] > 0
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out["results"][j], request_reference(orc, k, a, b))

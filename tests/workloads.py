@@ -84,3 +84,27 @@ def size_class(n):
     while c < n:
         c *= 2
     return c
+
+
+# ---------------------------------------------------------------- serving stream
+def make_requests(gen, n, seed=SEED + 3, mul_frac=0.7, lo_bits=9, hi_bits=18):
+    """Serving requests: kind mul with probability mul_frac, operand sizes
+    max(8, round(2^U(lo_bits, hi_bits) / 64)) limbs (512..262144 bits for the
+    defaults), independent per operand; limbs from gen(seed, count)."""
+    rng = np.random.default_rng(seed)
+    kinds = (rng.random(n) < mul_frac).astype(int)
+    sizes = np.maximum(8, np.rint(2.0 ** rng.uniform(lo_bits, hi_bits, size=(n, 2)) / 64)).astype(np.int64)
+    limbs = gen(seed, int(sizes.sum()))
+    out, pos = [], 0
+    for i in range(n):
+        na, nb = int(sizes[i, 0]), int(sizes[i, 1])
+        a = np.ascontiguousarray(limbs[pos:pos + na])
+        pos += na
+        b = np.ascontiguousarray(limbs[pos:pos + nb])
+        pos += nb
+        out.append((int(kinds[i]), a, b))
+    return out
+
+
+def request_reference(orc, kind, a, b):
+    return orc.mul_unbal(a, b) if kind == 1 else orc.add_unbal(a, b)
