@@ -191,6 +191,23 @@ def main():
         f"speedup {ms_plain / ms_rag:.2f}x")
     del plan, plain, dra, drb, drr
 
+    # ---------------------------------------------------------- serving loop
+    from workloads import make_requests
+
+    reqs = make_requests(mi.gen_limbs, 3000)
+    serving = []
+    for qps in (1000, 4000, 16000):
+        srv = mi.Server(ctx, hbm_budget=4 << 30, streams=4)
+        res = mi.serve_poisson(srv, reqs, qps=qps, seed=qps, keep_results=False)
+        st = srv.stats()
+        srv.close()
+        res.pop("results")
+        res["batches"] = st["batches_launched"]
+        res["mean_batch_ops"] = st["ops_in_batches"] / max(1, st["batches_launched"])
+        serving.append(res)
+        log(f"serving qps {qps}: p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
+            f"{res['ops_per_s']:.0f} ops/s, {res['batches']} batches")
+
     if args.calibrate:
         src = torch.ones(32 * MIB, dtype=torch.int64, device="cuda:0")
         dst = torch.empty_like(src)
@@ -245,6 +262,8 @@ def main():
                              "note": "one hipGraph launch (counter reset + persistent kernel) vs one fixed-size "
                                      "launch per (op, power-of-two size class) on zero-padded operands"},
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
+            "serving": {"requests": len(reqs), "mix": "70% mul / 30% add, operands 8..4096 limbs log-uniform",
+                        "points": serving},
         },
     }
     ctx.close()
