@@ -208,6 +208,20 @@ midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, 
 midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
                                       const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
 
+/* ------------------------------------------------------------- tuning */
+/* Process-wide launch parameters used by the kernels' launchers, set by the
+ * autotuner (midint-arithmetic_amd/autotune.py) or by hand: "mul.q" (4|8),
+ * "mul.prefetch" (0|1), "add.rows", "add.threads", "add.nt" (0|1),
+ * "ntt.threads", "ragged.threads". "key@C" (C a power of two) sets the value
+ * for operands of (C/2, C] limbs. Unknown keys: MIDINT_ENOTFOUND. Results
+ * never depend on these values, only speed. No reference counterpart (the reference fixes
+ * Q and block sizes at compile time, cuda/main.cu:64-78, 243-261). */
+midint_status midint_tune_set(const char* key, int64_t value);
+midint_status midint_tune_get(const char* key, int64_t* value);
+/* Applies a tuning file: one "key value" pair per line, '#' comments
+ * (written by the autotuner, e.g. midint-arithmetic_amd/tune/mi355x.txt). */
+midint_status midint_tune_load(const char* path);
+
 /* ------------------------------------------------------ microbenchmark */
 /* Issue throughput of the VALU operations the kernels are built from,
  * measured on `device`: 32x32->64 multiply-adds per second (the price of
@@ -215,6 +229,10 @@ midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const 
  * basic-op latency kernels testBasicOps / multiplyKer
  * (cuda/ker-helpers.cu.h:189-317). Synchronous. */
 midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_per_s);
+/* Partial products per second of the classical kernel's accumulate pattern
+ * (v_mad_u64_u32 with carry-out + v_addc_co_u32 per product) on register
+ * operands: the ceiling of the multiplication loop without LDS traffic. */
+midint_status midint_probe_mac(int device, double* pp_per_s);
 
 /* ------------------------------------------------------ data generation */
 /* xoshiro256** stream (state seeded by four splitmix64 draws of `seed`)

@@ -56,6 +56,7 @@ _SIGS = {
     "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_probe_valu": (_int, [_int, _dblp, _dblp]),
+    "midint_probe_mac": (_int, [_int, _dblp]),
     "midint_max_limbs_ragged": (_u32, []),
     "midint_ragged_plan_create": (_int, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "midint_ragged_launch": (_int, [_vp, _vp]),
@@ -69,6 +70,9 @@ _SIGS = {
                                     _dblp]),
     "midint_server_stats": (_int, [_vp, _vp]),
     "midint_server_destroy": (_int, [_vp]),
+    "midint_tune_set": (_int, [ctypes.c_char_p, ctypes.c_int64]),
+    "midint_tune_get": (_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int64)]),
+    "midint_tune_load": (_int, [ctypes.c_char_p]),
     "midint_gen_limbs": (None, [ctypes.c_uint64, _vp, ctypes.c_size_t]),
     "midint_gen_ref_rand": (None, [_u32, ctypes.c_size_t, ctypes.c_size_t, ctypes.c_size_t, _vp]),
 }
@@ -145,10 +149,12 @@ def _stream(stream):
 class Context:
     """Owns a midint_ctx (device binding, NTT twiddle cache, scratch)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, tuned: bool = True):
         self._h = ctypes.c_void_p()
         check(lib().midint_init(ctypes.byref(self._h), device), "midint_init")
         self.device = device
+        if tuned:
+            tune_load()
 
     def close(self):
         if self._h:
@@ -371,12 +377,40 @@ def gen_ref_rand(seed: int, n: int, m_u32: int, nz: int):
     return out
 
 
+def tune_set(key: str, value: int) -> None:
+    check(lib().midint_tune_set(key.encode(), int(value)), f"midint_tune_set({key})")
+
+
+TUNE_FILE = os.path.join(HERE, "tune", "mi355x.txt")
+
+
+def tune_load(path: str = TUNE_FILE) -> bool:
+    """Applies a tuning file if it exists; returns whether it was applied."""
+    if not os.path.exists(path):
+        return False
+    check(lib().midint_tune_load(path.encode()), "midint_tune_load")
+    return True
+
+
+def tune_get(key: str) -> int:
+    v = ctypes.c_int64()
+    check(lib().midint_tune_get(key.encode(), ctypes.byref(v)), f"midint_tune_get({key})")
+    return v.value
+
+
 def probe_valu(device=0):
     """(v_mad_u64_u32 per second, 32-bit adds per second) measured on device."""
     mad = ctypes.c_double(0.0)
     add = ctypes.c_double(0.0)
     check(lib().midint_probe_valu(device, ctypes.byref(mad), ctypes.byref(add)), "midint_probe_valu")
     return mad.value, add.value
+
+
+def probe_mac(device=0):
+    """Partial products per second of the mad + addc accumulate pattern."""
+    v = ctypes.c_double(0.0)
+    check(lib().midint_probe_mac(device, ctypes.byref(v)), "midint_probe_mac")
+    return v.value
 
 
 def gpu_add_host(a, b, m, runs=0, device=0):

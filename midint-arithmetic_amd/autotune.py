@@ -1,0 +1,138 @@
+"""Autotuner of the midint launch parameters (run on the target GPU).
+
+For each power-of-two operand-size class it times every candidate launch
+configuration of a kernel on a synthetic batch of that class and keeps the
+fastest. A candidate is only timed after its output is bit-identical to the
+output of the library's default configuration on the same inputs, so tuning
+can never trade correctness for speed. The result is written as
+``key@class value`` lines (see midint_tune_set) to tune/mi355x.txt, which
+Context() applies when it is created (and C callers with midint_tune_load).
+
+    python midint-arithmetic_amd/autotune.py [--out PATH] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import itertools
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _pkg():
+    name = "midint_arithmetic_amd"
+    if name in sys.modules:
+        return sys.modules[name]
+    spec = importlib.util.spec_from_file_location(name, os.path.join(HERE, "__init__.py"),
+                                                  submodule_search_locations=[HERE])
+    mod = importlib.util.module_from_spec(spec)
+    sys.modules[name] = mod
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def _timed(torch, fn, steps, warm=3):
+    for _ in range(warm):
+        fn()
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def _set(mi, params, cls):
+    for k, v in params.items():
+        mi.tune_set(f"{k}@{cls}", v)
+
+
+def tune_op(mi, torch, ctx, op, cls, grid, defaults, limbs_total, rng, log):
+    """Returns (best params, table of (params, us))."""
+    m = cls
+    n = max(1, limbs_total // m)
+    a = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+    b = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+    a[: m] = np.uint64(2**64 - 1)
+    b[: m] = np.uint64(2**64 - 1)
+    da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
+    db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
+    out_l = 2 * n * m if op == "mul" else n * m
+    dr = torch.empty(out_l, dtype=torch.int64, device="cuda:0")
+
+    def run():
+        if op == "mul":
+            ctx.mul_full(da, db, dr, m)
+        else:
+            ctx.add(da, db, dr, m)
+
+    _set(mi, defaults, cls)
+    run()
+    torch.cuda.synchronize()
+    ref = dr.clone()
+    rows = []
+    best, best_us = dict(defaults), None
+    for values in itertools.product(*grid.values()):
+        params = dict(zip(grid.keys(), values))
+        _set(mi, params, cls)
+        dr.zero_()
+        run()
+        torch.cuda.synchronize()
+        if not torch.equal(dr, ref):
+            log(f"  {op}@{cls} {params}: output differs from the default configuration -- rejected")
+            continue
+        us = _timed(torch, run, 20) * 1e3
+        rows.append((params, us))
+        if best_us is None or us < best_us:
+            best, best_us = params, us
+    _set(mi, best, cls)
+    log(f"  {op}@{cls}: best {best} {best_us:.2f} us over {len(rows)} candidates")
+    return best, rows
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", default=os.path.join(HERE, "tune", "mi355x.txt"))
+    ap.add_argument("--quick", action="store_true", help="fewer classes and candidates")
+    args = ap.parse_args(argv)
+
+    import torch
+
+    mi = _pkg()
+    ctx = mi.Context(0, tuned=False)
+    rng = np.random.default_rng(7)
+    log = lambda s: print(s, file=sys.stderr, flush=True)  # noqa: E731
+    lines = ["# midint launch parameters tuned on " + torch.cuda.get_device_name(0),
+             "# written by midint-arithmetic_amd/autotune.py " + time.strftime("%Y-%m-%d %H:%M:%S")]
+
+    add_classes = [16, 64, 256] if args.quick else [1, 2, 4, 8, 16, 32, 64, 128, 256]
+    add_grid = {"add.rows": [1, 2, 4], "add.threads": [64, 128, 256, 512], "add.nt": [0, 1]}
+    for cls in add_classes:
+        best, _ = tune_op(mi, torch, ctx, "add", cls, add_grid, {"add.rows": 2, "add.threads": 256, "add.nt": 1},
+                          1 << 22, rng, log)
+        lines += [f"{k}@{cls} {v}" for k, v in best.items()]
+
+    mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
+    mul_grid = {"mul.q": [4, 8], "mul.prefetch": [0, 1]}
+    for cls in mul_classes:
+        best, _ = tune_op(mi, torch, ctx, "mul", cls, mul_grid, {"mul.q": 4, "mul.prefetch": 0},
+                          max(1 << 16, 2048 * cls), rng, log)
+        lines += [f"{k}@{cls} {v}" for k, v in best.items()]
+
+    os.makedirs(os.path.dirname(args.out), exist_ok=True)
+    with open(args.out, "w") as f:
+        f.write("\n".join(lines) + "\n")
+    log(f"wrote {args.out}")
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -15,11 +15,18 @@
 
 namespace midint {
 
-template <bool VEC>
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
+
+// NT: streaming (non-temporal) 16-byte accesses for data touched once.
+template <bool VEC, bool NT = false>
 __device__ __forceinline__ void load_pair(const uint64_t* __restrict__ p, uint32_t k, uint32_t L,
                                           uint64_t& v0, uint64_t& v1) {
     if (VEC && k + 1 < L) {
-        ulonglong2 t = *reinterpret_cast<const ulonglong2*>(p + k);
+        u64x2 t;
+        if (NT)
+            t = __builtin_nontemporal_load(reinterpret_cast<const u64x2*>(p + k));
+        else
+            t = *reinterpret_cast<const u64x2*>(p + k);
         v0 = t.x;
         v1 = t.y;
     } else {
@@ -28,14 +35,17 @@ __device__ __forceinline__ void load_pair(const uint64_t* __restrict__ p, uint32
     }
 }
 
-template <bool VEC>
+template <bool VEC, bool NT = false>
 __device__ __forceinline__ void store_pair(uint64_t* __restrict__ p, uint32_t k, uint32_t L, uint64_t v0,
                                            uint64_t v1) {
     if (VEC && k + 1 < L) {
-        ulonglong2 t;
+        u64x2 t;
         t.x = v0;
         t.y = v1;
-        *reinterpret_cast<ulonglong2*>(p + k) = t;
+        if (NT)
+            __builtin_nontemporal_store(t, reinterpret_cast<u64x2*>(p + k));
+        else
+            *reinterpret_cast<u64x2*>(p + k) = t;
     } else {
         if (k < L) p[k] = v0;
         if (k + 1 < L) p[k + 1] = v1;
@@ -43,7 +53,7 @@ __device__ __forceinline__ void store_pair(uint64_t* __restrict__ p, uint32_t k,
 }
 
 // Loads both operands of the block window into rows and marks segment starts.
-template <int R, bool VEC>
+template <int R, bool VEC, bool NT = false>
 __device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restrict__ a,
                                             const uint64_t* __restrict__ b, uint32_t L, uint32_t m,
                                             uint32_t step_q, uint32_t step_r, uint32_t* off0,
@@ -54,8 +64,8 @@ __device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restri
 #pragma unroll
     for (int v = 0; v < R; v++) {
         const uint32_t k = k_first + v * kRowLimbs;
-        load_pair<VEC>(a, k, L, t.x0[v], t.x1[v]);
-        load_pair<VEC>(b, k, L, t.y0[v], t.y1[v]);
+        load_pair<VEC, NT>(a, k, L, t.x0[v], t.x1[v]);
+        load_pair<VEC, NT>(b, k, L, t.y0[v], t.y1[v]);
         t.s0[v] = (sp.off == 0) && (k < L);
         t.s1[v] = (sp.off + 1 == m) && (k + 1 < L);
         off0[v] = sp.off;
@@ -64,7 +74,7 @@ __device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restri
     }
 }
 
-template <int R, bool VEC>
+template <int R, bool VEC, bool NT>
 __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
                                                    uint32_t step_r, const uint64_t* __restrict__ a,
                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
@@ -77,7 +87,7 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
 
     Rows<R> t;
     uint32_t off0[R], inst0[R];
-    load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+    load_window<R, VEC, NT>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
 
     uint64_t ax0[R], ax1[R];
     if (carry) {
@@ -92,7 +102,7 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
 
     const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
 #pragma unroll
-    for (int v = 0; v < R; v++) store_pair<VEC>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
+    for (int v = 0; v < R; v++) store_pair<VEC, NT>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
 
     if (carry) {
 #pragma unroll
@@ -185,8 +195,13 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
 AddGeom add_geometry(uint32_t m) {
     AddGeom g;
     if (m <= 256) {
-        g.rows = 2;
-        g.threads = 256;
+        g.rows = (uint32_t)tune_get_class("add.rows", m, 2);
+        g.threads = (uint32_t)tune_get_class("add.threads", m, 256);
+        if ((g.rows != 1 && g.rows != 2 && g.rows != 4 && g.rows != 8) || g.threads < 64 || g.threads > 1024 ||
+            g.threads % 64 != 0 || (g.threads / kWave) * g.rows * kRowLimbs < m) {
+            g.rows = 2;
+            g.threads = 256;
+        }
     } else {
         g.rows = 0;
         const uint32_t rows_opts[] = {2, 4, 8, 16};
@@ -213,6 +228,7 @@ AddGeom add_geometry(uint32_t m) {
 template <int R>
 static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n, uint32_t m, const uint64_t* a,
                                const uint64_t* b, uint64_t* r, uint8_t* carry, hipStream_t s) {
+    const bool nt = tune_get_class("add.nt", m, 1) != 0;
     const uint32_t blocks = (uint32_t)(((uint64_t)n + g.ipb - 1) / g.ipb);
     const uint32_t sq = kRowLimbs / m, sr = kRowLimbs % m;
     if (six) {
@@ -221,10 +237,12 @@ static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n,
         else
             add6_kernel<R, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r);
     } else {
-        if (vec)
-            add_kernel<R, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+        if (vec && nt)
+            add_kernel<R, true, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+        else if (vec)
+            add_kernel<R, true, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
         else
-            add_kernel<R, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+            add_kernel<R, false, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
     }
     return hipGetLastError();
 }
@@ -236,6 +254,7 @@ hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t*
     const bool aligned = ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 15u) == 0;
     const bool vec = aligned && ((uint64_t)g.ipb * m) % 2 == 0;
     switch (g.rows) {
+        case 1: return launch_add_r<1>(g, vec, six, n, m, a, b, r, carry, s);
         case 2: return launch_add_r<2>(g, vec, six, n, m, a, b, r, carry, s);
         case 4: return launch_add_r<4>(g, vec, six, n, m, a, b, r, carry, s);
         case 8: return launch_add_r<8>(g, vec, six, n, m, a, b, r, carry, s);

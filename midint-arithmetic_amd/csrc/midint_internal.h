@@ -84,6 +84,11 @@ const void* ragged_reset_ptr();
 // combine kernel adds P01 + P10 into limbs [2048, 8192) of P00 | P11.
 hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s);
 
+// Autotuned launch parameter (see tuning.cpp), or dflt for unknown keys.
+int64_t tune_get(const char* key, int64_t dflt);
+// Value for operand size `limbs` ("key@C" for its power-of-two class C), else tune_get.
+int64_t tune_get_class(const char* key, uint32_t limbs, int64_t dflt);
+
 // NTT twiddle tables of a context (cached).
 midint_status ctx_ntt_tables(midint_ctx* ctx, uint32_t lg, NttTables* out);
 

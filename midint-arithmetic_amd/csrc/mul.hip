@@ -32,6 +32,7 @@
 namespace midint {
 
 struct MulLayout {
+    uint32_t Q;       // columns per lane group
     uint32_t n_pad;   // padded operand length in u32 words (multiple of 2Q)
     uint32_t T;       // lanes per instance
     uint32_t P;       // product limbs computed per instance
@@ -40,17 +41,11 @@ struct MulLayout {
     uint32_t threads; // block size
 };
 
-__host__ __device__ inline uint32_t mul_npad(uint32_t m) {
-    uint32_t n = 2 * m;
-    return (n + 2 * kMulQ - 1) / (2 * kMulQ) * (2 * kMulQ);
-}
-
-template <bool FULL>
+template <bool FULL, int Q, bool PF>
 __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
                                                    uint32_t P, uint32_t words, uint32_t ipb, uint32_t out_len,
-                                                   const uint64_t* __restrict__ a, const uint64_t* __restrict__ b,
-                                                   uint64_t* __restrict__ r) {
-    constexpr uint32_t Q = kMulQ;
+                                                   int32_t m_shift, const uint64_t* __restrict__ a,
+                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ r) {
     extern __shared__ __align__(16) uint64_t lds64[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
     __shared__ uint32_t lds_gp[16];
@@ -59,16 +54,31 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
 
-    // ---- stage operands into LDS: A[n_pad], B[(Q-1) + j] with zero pads
-    const uint32_t zero_words = ipb * words;
-    for (uint32_t f = tid; f < zero_words; f += blockDim.x) lds[f] = 0u;
-    __syncthreads();
+    // ---- stage operands into LDS: A[n_pad], B[(Q-1) + j] with zero pads.
+    // Only the pad words are cleared: A[2m, n_pad), B[0, Q-1) and
+    // B[Q-1+2m, n_pad+2Q) of every slot (the snapshot area needs no init).
+    {
+        const uint32_t padA = n_pad - 2 * m;
+        const uint32_t padB = (Q - 1) + (n_pad + Q + 1 - 2 * m);
+        const uint32_t per = padA + padB;
+        for (uint32_t f = tid; f < ipb * per; f += blockDim.x) {
+            const uint32_t s = f / per;
+            const uint32_t x = f - s * per;
+            uint32_t* A = lds + s * words;
+            if (x < padA)
+                A[2 * m + x] = 0u;
+            else if (x < padA + (Q - 1))
+                A[n_pad + (x - padA)] = 0u;
+            else
+                A[n_pad + (Q - 1) + 2 * m + (x - padA - (Q - 1))] = 0u;
+        }
+    }
     {
         const uint32_t total = ninst * m;
         const uint64_t* ga = a + inst_base * m;
         const uint64_t* gb = b + inst_base * m;
         for (uint32_t f = tid; f < total; f += blockDim.x) {
-            const uint32_t s = f / m;
+            const uint32_t s = m_shift >= 0 ? (f >> m_shift) : f / m;
             const uint32_t i = f - s * m;
             const uint64_t va = ga[f], vb = gb[f];
             uint32_t* A = lds + s * words;
@@ -78,6 +88,15 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
             Bw[2 * i] = (uint32_t)vb;
             Bw[2 * i + 1] = (uint32_t)(vb >> 32);
         }
+        // slots past the last instance of the grid hold zero operands
+        for (uint32_t f = ninst * m + tid; f < ipb * m; f += blockDim.x) {
+            const uint32_t s = m_shift >= 0 ? (f >> m_shift) : f / m;
+            const uint32_t i = f - s * m;
+            uint32_t* A = lds + s * words;
+            uint32_t* Bw = A + n_pad + (Q - 1);
+            A[2 * i] = A[2 * i + 1] = 0u;
+            Bw[2 * i] = Bw[2 * i + 1] = 0u;
+        }
     }
     __syncthreads();
 
@@ -85,7 +104,6 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     const uint32_t slot = tid / T;
     const uint32_t t = tid - slot * T;
     const bool active = slot < ipb;
-
     uint32_t k1 = Q * t, k2, i2, NB;
     if (FULL) {
         k2 = n_pad + Q * t;
@@ -96,21 +114,21 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
         i2 = 0;
         NB = 2 * T + 1;
     }
-    ColGroups g;
+    ColGroups<Q> g;
     if (active) {
         const uint32_t* A = lds + slot * words;
         uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
-        conv_lane(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
+        conv_lane<Q, PF>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
     }
     __syncthreads();  // everyone is done with A/B/snap; reuse LDS for L/H
 
     uint64_t* Lb = lds64;
     uint64_t* Hb = lds64 + ipb * P;
-    if (active) lane_store(g, true, k1, k2, Lb + slot * P, Hb + slot * P, P);
+    if (active) lane_store<Q>(g, true, k1, k2, Lb + slot * P, Hb + slot * P, P);
     __syncthreads();
 
-    // ---- normalisation: L + H over ipb segments of P limbs (2 rows per wave)
-    constexpr int R = 2;
+    // ---- normalisation: L + H over ipb segments of P limbs (Q/2 rows per wave)
+    constexpr int R = Q / 2;
     const uint32_t Lw = ipb * P;
     const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
     Rows<R> rows;
@@ -146,10 +164,11 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     }
 }
 
-static bool mul_layout(uint32_t m, bool full, MulLayout* L) {
-    constexpr uint32_t Q = kMulQ;
+static bool mul_layout(uint32_t m, bool full, uint32_t Q, MulLayout* L) {
     if (m == 0 || m > kMulMaxLimbs) return false;
-    L->n_pad = mul_npad(m);
+    L->Q = Q;
+    const uint32_t n = 2 * m;
+    L->n_pad = (n + 2 * Q - 1) / (2 * Q) * (2 * Q);
     L->T = full ? L->n_pad / Q : L->n_pad / (2 * Q);
     L->P = full ? L->n_pad : L->n_pad / 2;
     L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
@@ -161,8 +180,8 @@ static bool mul_layout(uint32_t m, bool full, MulLayout* L) {
         L->ipb = 1;
     }
     if (L->threads > 1024) return false;
-    // normalisation: R=2 rows of 128 limbs per wave must cover ipb*P limbs
-    if ((uint64_t)(L->threads / 64) * 2 * 128 < (uint64_t)L->ipb * L->P) return false;
+    // normalisation: Q/2 rows of 128 limbs per wave must cover ipb*P limbs
+    if ((uint64_t)(L->threads / 64) * (Q / 2) * 128 < (uint64_t)L->ipb * L->P) return false;
     return true;
 }
 
@@ -172,26 +191,42 @@ static size_t mul_lds_bytes(const MulLayout& L) {
     return conv > norm ? conv : norm;
 }
 
-hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
-                      hipStream_t s) {
-    MulLayout L;
-    if (!mul_layout(m, full, &L)) return hipErrorInvalidValue;
+template <bool FULL, int Q, bool PF>
+static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
+                               uint64_t* r, hipStream_t s) {
     const size_t lds = mul_lds_bytes(L);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
-    const uint32_t out_len = full ? 2 * m : m;
-    const void* fn = full ? (const void*)mul_kernel<true> : (const void*)mul_kernel<false>;
+    const void* fn = (const void*)mul_kernel<FULL, Q, PF>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    if (full) {
-        mul_kernel<true><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len, a, b, r);
-    } else {
-        mul_kernel<false><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len, a, b,
-                                                         r);
-    }
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
+    const uint32_t out_len = FULL ? 2 * m : m;
+    const int32_t m_shift = (m & (m - 1)) == 0 ? __builtin_ctz(m) : -1;
+    mul_kernel<FULL, Q, PF><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len,
+                                                           m_shift, a, b, r);
     return hipGetLastError();
+}
+
+template <bool FULL>
+static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                                        hipStream_t s) {
+    uint32_t Q = tune_get_class("mul.q", m, 8) == 8 ? 8 : 4;
+    const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
+    MulLayout L;
+    if (!mul_layout(m, FULL, Q, &L) || mul_lds_bytes(L) > 160 * 1024) {
+        Q = 4;  // fall back to the smaller footprint
+        if (!mul_layout(m, FULL, Q, &L)) return hipErrorInvalidValue;
+    }
+    if (Q == 8) return pf ? launch_mul_t<FULL, 8, true>(L, n, m, a, b, r, s)
+                          : launch_mul_t<FULL, 8, false>(L, n, m, a, b, r, s);
+    return pf ? launch_mul_t<FULL, 4, true>(L, n, m, a, b, r, s) : launch_mul_t<FULL, 4, false>(L, n, m, a, b, r, s);
+}
+
+hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                      hipStream_t s) {
+    return full ? launch_mul_full_or_lo<true>(n, m, a, b, r, s) : launch_mul_full_or_lo<false>(n, m, a, b, r, s);
 }
 
 // Reference polyKerQ (cuda/ker-classic-mul.cu.h:533-559) as a chain of

@@ -21,20 +21,22 @@
 // number of blocks (the reference's per-thread column pairing,
 // cuda/ker-classic-mul.cu.h:406-423, balances work per thread but leaves
 // lanes of a warp waiting on each other's longer loops).
+//
+// With PF (prefetch) the LDS reads of block beta+1 are issued before the
+// products of block beta, so their latency overlaps the VALU work.
 #pragma once
 
 #include "midint_dev.hpp"
 
 namespace midint {
 
-constexpr uint32_t kMulQ = 4;
-
-// acc[q] += a * b_q (32x32->64) and ovf[q] += carry-out, for q = 0..3.
-// One v_mad_u64_u32 per product writes the 64-bit sum and its carry-out lane
+// c_q += a * b_q (32x32->64) and o_q += carry-out, q = 0..3. One
+// v_mad_u64_u32 per product writes the 64-bit sum and its carry-out lane
 // mask; one v_addc_co_u32 folds the mask into the overflow count. Each mask
 // is read three VALU instructions after it is written.
-__device__ __forceinline__ void mac4(uint32_t a, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3,
-                                     uint64_t (&acc)[4], uint32_t (&ovf)[4]) {
+__device__ __forceinline__ void mac4(uint32_t a, uint32_t b0, uint32_t b1, uint32_t b2, uint32_t b3, uint64_t& c0,
+                                     uint64_t& c1, uint64_t& c2, uint64_t& c3, uint32_t& o0, uint32_t& o1,
+                                     uint32_t& o2, uint32_t& o3) {
     uint64_t k0, k1, k2, k3;
     asm volatile(
         "v_mad_u64_u32 %[c0], %[k0], %[a], %[b0], %[c0]\n\t"
@@ -45,50 +47,164 @@ __device__ __forceinline__ void mac4(uint32_t a, uint32_t b0, uint32_t b1, uint3
         "v_addc_co_u32_e64 %[o1], %[k1], %[o1], 0, %[k1]\n\t"
         "v_addc_co_u32_e64 %[o2], %[k2], %[o2], 0, %[k2]\n\t"
         "v_addc_co_u32_e64 %[o3], %[k3], %[o3], 0, %[k3]"
-        : [c0] "+v"(acc[0]), [c1] "+v"(acc[1]), [c2] "+v"(acc[2]), [c3] "+v"(acc[3]), [o0] "+v"(ovf[0]),
-          [o1] "+v"(ovf[1]), [o2] "+v"(ovf[2]), [o3] "+v"(ovf[3]), [k0] "=&s"(k0), [k1] "=&s"(k1),
-          [k2] "=&s"(k2), [k3] "=&s"(k3)
+        : [c0] "+v"(c0), [c1] "+v"(c1), [c2] "+v"(c2), [c3] "+v"(c3), [o0] "+v"(o0), [o1] "+v"(o1),
+          [o2] "+v"(o2), [o3] "+v"(o3), [k0] "=&s"(k0), [k1] "=&s"(k1), [k2] "=&s"(k2), [k3] "=&s"(k3)
         : [a] "v"(a), [b0] "v"(b0), [b1] "v"(b1), [b2] "v"(b2), [b3] "v"(b3));
 }
 
+// Eight columns against one a-word: 8 x v_mad_u64_u32 (carry masks in eight
+// SGPR pairs) followed by 8 x v_addc_co_u32, so every mask is read eight
+// instructions after it is written (fewer stalls at low occupancy than the
+// 4+4 order of mac4).
+__device__ __forceinline__ void mac8(uint32_t a, const uint32_t* b, uint64_t* c, uint32_t* o) {
+    uint64_t k0, k1, k2, k3, k4, k5, k6, k7;
+    asm volatile(
+        "v_mad_u64_u32 %[c0], %[k0], %[a], %[b0], %[c0]\n\t"
+        "v_mad_u64_u32 %[c1], %[k1], %[a], %[b1], %[c1]\n\t"
+        "v_mad_u64_u32 %[c2], %[k2], %[a], %[b2], %[c2]\n\t"
+        "v_mad_u64_u32 %[c3], %[k3], %[a], %[b3], %[c3]\n\t"
+        "v_mad_u64_u32 %[c4], %[k4], %[a], %[b4], %[c4]\n\t"
+        "v_mad_u64_u32 %[c5], %[k5], %[a], %[b5], %[c5]\n\t"
+        "v_mad_u64_u32 %[c6], %[k6], %[a], %[b6], %[c6]\n\t"
+        "v_mad_u64_u32 %[c7], %[k7], %[a], %[b7], %[c7]"
+        : [c0] "+v"(c[0]), [c1] "+v"(c[1]), [c2] "+v"(c[2]), [c3] "+v"(c[3]), [c4] "+v"(c[4]), [c5] "+v"(c[5]),
+          [c6] "+v"(c[6]), [c7] "+v"(c[7]), [k0] "=&s"(k0), [k1] "=&s"(k1), [k2] "=&s"(k2), [k3] "=&s"(k3),
+          [k4] "=&s"(k4), [k5] "=&s"(k5), [k6] "=&s"(k6), [k7] "=&s"(k7)
+        : [a] "v"(a), [b0] "v"(b[0]), [b1] "v"(b[1]), [b2] "v"(b[2]), [b3] "v"(b[3]), [b4] "v"(b[4]),
+          [b5] "v"(b[5]), [b6] "v"(b[6]), [b7] "v"(b[7]));
+    asm volatile(
+        "v_addc_co_u32_e64 %[o0], %[k0], %[o0], 0, %[k0]\n\t"
+        "v_addc_co_u32_e64 %[o1], %[k1], %[o1], 0, %[k1]\n\t"
+        "v_addc_co_u32_e64 %[o2], %[k2], %[o2], 0, %[k2]\n\t"
+        "v_addc_co_u32_e64 %[o3], %[k3], %[o3], 0, %[k3]\n\t"
+        "v_addc_co_u32_e64 %[o4], %[k4], %[o4], 0, %[k4]\n\t"
+        "v_addc_co_u32_e64 %[o5], %[k5], %[o5], 0, %[k5]\n\t"
+        "v_addc_co_u32_e64 %[o6], %[k6], %[o6], 0, %[k6]\n\t"
+        "v_addc_co_u32_e64 %[o7], %[k7], %[o7], 0, %[k7]"
+        : [o0] "+v"(o[0]), [o1] "+v"(o[1]), [o2] "+v"(o[2]), [o3] "+v"(o[3]), [o4] "+v"(o[4]), [o5] "+v"(o[5]),
+          [o6] "+v"(o[6]), [o7] "+v"(o[7]), [k0] "+s"(k0), [k1] "+s"(k1), [k2] "+s"(k2), [k3] "+s"(k3),
+          [k4] "+s"(k4), [k5] "+s"(k5), [k6] "+s"(k6), [k7] "+s"(k7));
+}
+
+template <int Q>
 struct ColGroups {
-    uint64_t acc[kMulQ];  // after the loop: group 1 + group 2 (or the single group)
-    uint32_t ovf[kMulQ];
-    uint64_t sacc[kMulQ];  // parked group 1 (pair lanes)
-    uint32_t sovf[kMulQ];
+    uint64_t acc[Q];  // after the loop: group 1 + group 2 (or the single group)
+    uint32_t ovf[Q];
+    uint64_t sacc[Q];  // parked group 1 (pair lanes)
+    uint32_t sovf[Q];
 };
 
+template <int Q>
+struct Operands {
+    uint32_t a[Q];
+    uint32_t b[2 * Q];
+};
+
+template <int Q>
+__device__ __forceinline__ void load_block(const uint32_t* pa, const uint32_t* pb, Operands<Q>& o) {
+#pragma unroll
+    for (int i = 0; i < Q; i += 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(pa + i);
+        o.a[i] = v.x;
+        o.a[i + 1] = v.y;
+        o.a[i + 2] = v.z;
+        o.a[i + 3] = v.w;
+    }
+#pragma unroll
+    for (int i = 0; i < 2 * Q; i += 4) {
+        const uint4 v = *reinterpret_cast<const uint4*>(pb + i);
+        o.b[i] = v.x;
+        o.b[i + 1] = v.y;
+        o.b[i + 2] = v.z;
+        o.b[i + 3] = v.w;
+    }
+}
+
+// Column q, a-word u uses window word q - u + Q - 1.
+template <int Q>
+__device__ __forceinline__ void mac_block(const Operands<Q>& o, ColGroups<Q>& g) {
+    if (Q == 8) {
+#pragma unroll
+        for (int u = 0; u < Q; u++) mac8(o.a[u], &o.b[Q - 1 - u], g.acc, g.ovf);
+        return;
+    }
+#pragma unroll
+    for (int u = 0; u < Q; u++) {
+#pragma unroll
+        for (int q0 = 0; q0 < Q; q0 += 4) {
+            const int x = q0 - u + Q - 1;
+            mac4(o.a[u], o.b[x], o.b[x + 1], o.b[x + 2], o.b[x + 3], g.acc[q0], g.acc[q0 + 1], g.acc[q0 + 2],
+                 g.acc[q0 + 3], g.ovf[q0], g.ovf[q0 + 1], g.ovf[q0 + 2], g.ovf[q0 + 3]);
+        }
+    }
+}
+
+template <int Q>
+__device__ __forceinline__ void park(uint32_t* snap, const ColGroups<Q>& g) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        snap[3 * q] = (uint32_t)g.acc[q];
+        snap[3 * q + 1] = (uint32_t)(g.acc[q] >> 32);
+        snap[3 * q + 2] = g.ovf[q];
+    }
+}
+
+template <int Q, bool PF>
 __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, uint32_t k1, uint32_t k2,
-                                          uint32_t i2, uint32_t sw, uint32_t NB, uint32_t* snap, ColGroups& g) {
-    constexpr uint32_t Q = kMulQ;
+                                          uint32_t i2, uint32_t sw, uint32_t NB, uint32_t* snap, ColGroups<Q>& g) {
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         g.acc[q] = 0;
         g.ovf[q] = 0;
     }
-    const uint32_t* pa = S;
-    const uint32_t* pb = W + k1;
-    for (uint32_t beta = 0; beta < NB; beta++) {
-        if (beta == sw) {
-#pragma unroll
-            for (int q = 0; q < Q; q++) {
-                snap[3 * q] = (uint32_t)g.acc[q];
-                snap[3 * q + 1] = (uint32_t)(g.acc[q] >> 32);
-                snap[3 * q + 2] = g.ovf[q];
+    if (PF) {
+        // Two operand buffers in ping-pong, loop unrolled by two. The load
+        // of block beta+1 is issued before the products of block beta. The
+        // prefetch past the last block reads one block beyond the lane's
+        // range, which stays inside the LDS staging area (pa <= S + ns + 2Q,
+        // pb >= W - Q >= S); its values are never used.
+        const uint32_t* pa = S;
+        const uint32_t* pb = W + k1;
+        Operands<Q> x0, x1;
+        load_block<Q>(pa, pb, x0);
+        for (uint32_t beta = 0; beta < NB; beta += 2) {
+            if (beta + 1 == sw) {
+                pa = S + i2;
+                pb = W + (k2 - i2);
+            } else {
+                pa += Q;
+                pb -= Q;
             }
-            pa = S + i2;
-            pb = W + (k2 - i2);
+            load_block<Q>(pa, pb, x1);
+            if (beta == sw) park<Q>(snap, g);
+            mac_block<Q>(x0, g);
+            if (beta + 1 >= NB) break;
+            if (beta + 2 == sw) {
+                pa = S + i2;
+                pb = W + (k2 - i2);
+            } else {
+                pa += Q;
+                pb -= Q;
+            }
+            load_block<Q>(pa, pb, x0);
+            if (beta + 1 == sw) park<Q>(snap, g);
+            mac_block<Q>(x1, g);
         }
-        const uint4 av = *reinterpret_cast<const uint4*>(pa);
-        const uint4 bw0 = *reinterpret_cast<const uint4*>(pb);
-        const uint4 bw1 = *reinterpret_cast<const uint4*>(pb + 4);
-        // column q, a-word u uses window word q - u + Q - 1
-        mac4(av.x, bw0.w, bw1.x, bw1.y, bw1.z, g.acc, g.ovf);
-        mac4(av.y, bw0.z, bw0.w, bw1.x, bw1.y, g.acc, g.ovf);
-        mac4(av.z, bw0.y, bw0.z, bw0.w, bw1.x, g.acc, g.ovf);
-        mac4(av.w, bw0.x, bw0.y, bw0.z, bw0.w, g.acc, g.ovf);
-        pa += Q;
-        pb -= Q;
+    } else {
+        const uint32_t* pa = S;
+        const uint32_t* pb = W + k1;
+        for (uint32_t beta = 0; beta < NB; beta++) {
+            if (beta == sw) {
+                park<Q>(snap, g);
+                pa = S + i2;
+                pb = W + (k2 - i2);
+            }
+            Operands<Q> cur;
+            load_block<Q>(pa, pb, cur);
+            mac_block<Q>(cur, g);
+            pa += Q;
+            pb -= Q;
+        }
     }
     if (sw < NB) {
 #pragma unroll
@@ -101,11 +217,12 @@ __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, 
 
 // Splits the value sum_q (va[q] + vo[q] 2^64) 2^(32q) of a column group at
 // word column kcol into Q/2 low limbs written to L[kcol/2 ...] and a high
-// part of two limbs written to H[kcol/2 + Q/2 ...] (dropped at or beyond P
-// limbs). H[0..Q/2) of the segment is cleared by the group at column 0.
-__device__ __forceinline__ void group_store(const uint64_t (&va)[kMulQ], const uint32_t (&vo)[kMulQ],
-                                            uint32_t kcol, uint64_t* L, uint64_t* H, uint32_t P) {
-    constexpr uint32_t Q = kMulQ;
+// part of at most two limbs written to the H slot H[kcol/2 + Q/2 ...] of
+// Q/2 limbs (rest of the slot zeroed; limbs at or beyond P dropped).
+// H[0..Q/2) of the segment is cleared by the group at column 0.
+template <int Q>
+__device__ __forceinline__ void group_store(const uint64_t (&va)[Q], const uint32_t (&vo)[Q], uint32_t kcol,
+                                            uint64_t* L, uint64_t* H, uint32_t P) {
     uint64_t lo = 0, hi = 0;
     uint32_t w[Q];
 #pragma unroll
@@ -122,6 +239,10 @@ __device__ __forceinline__ void group_store(const uint64_t (&va)[kMulQ], const u
     for (int i = 0; i < Q / 2; i++) L[limb + i] = (uint64_t)w[2 * i] | ((uint64_t)w[2 * i + 1] << 32);
     if (limb + Q / 2 < P) H[limb + Q / 2] = lo;
     if (limb + Q / 2 + 1 < P) H[limb + Q / 2 + 1] = hi;
+    // the rest of this group's H slot (Q > 4: the slot is Q/2 limbs wide)
+#pragma unroll
+    for (int i = 2; i < Q / 2; i++)
+        if (limb + Q / 2 + i < P) H[limb + Q / 2 + i] = 0;
     if (kcol == 0) {
 #pragma unroll
         for (int i = 0; i < Q / 2; i++) H[i] = 0;
@@ -129,11 +250,11 @@ __device__ __forceinline__ void group_store(const uint64_t (&va)[kMulQ], const u
 }
 
 // Stores the one or two column groups of a lane after conv_lane.
-__device__ __forceinline__ void lane_store(const ColGroups& g, bool two, uint32_t k1, uint32_t k2, uint64_t* L,
+template <int Q>
+__device__ __forceinline__ void lane_store(const ColGroups<Q>& g, bool two, uint32_t k1, uint32_t k2, uint64_t* L,
                                            uint64_t* H, uint32_t P) {
-    constexpr uint32_t Q = kMulQ;
     if (two) {
-        group_store(g.sacc, g.sovf, k1, L, H, P);
+        group_store<Q>(g.sacc, g.sovf, k1, L, H, P);
         uint64_t va[Q];
         uint32_t vo[Q];
 #pragma unroll
@@ -141,10 +262,13 @@ __device__ __forceinline__ void lane_store(const ColGroups& g, bool two, uint32_
             va[q] = g.acc[q] - g.sacc[q];
             vo[q] = g.ovf[q] - g.sovf[q] - (g.acc[q] < g.sacc[q] ? 1u : 0u);
         }
-        group_store(va, vo, k2, L, H, P);
+        group_store<Q>(va, vo, k2, L, H, P);
     } else {
-        group_store(g.acc, g.ovf, k1, L, H, P);
+        group_store<Q>(g.acc, g.ovf, k1, L, H, P);
     }
 }
+
+// The ragged kernel uses Q = 4.
+constexpr uint32_t kMulQ = 4;
 
 }  // namespace midint

@@ -8,6 +8,7 @@
 // multiplication roofline.
 #include "midint.h"
 #include "midint_internal.h"
+#include "mul_core.hpp"
 
 namespace midint {
 
@@ -46,9 +47,63 @@ __global__ __launch_bounds__(256) void probe_add32(uint32_t iters, uint32_t seed
     if (s == 0x12345678u) out[0] = s;
 }
 
+// The classical kernel's accumulate pattern (mac4: 4 x v_mad_u64_u32 with
+// carry-out + 4 x v_addc_co_u32) on register operands: partial products per
+// second without LDS traffic or loop bookkeeping.
+__global__ __launch_bounds__(256) void probe_mac(uint32_t iters, uint32_t seed, uint64_t* out) {
+    uint64_t acc[8];
+    uint32_t ovf[8];
+    uint32_t a = seed ^ threadIdx.x, b0 = seed * 3u + blockIdx.x, b1 = b0 ^ 0x55u, b2 = b0 + 7u, b3 = b0 * 5u;
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+        acc[c] = c;
+        ovf[c] = 0;
+    }
+    for (uint32_t i = 0; i < iters; i++) {
+        mac4(a, b0, b1, b2, b3, acc[0], acc[1], acc[2], acc[3], ovf[0], ovf[1], ovf[2], ovf[3]);
+        mac4(a, b1, b2, b3, b0, acc[4], acc[5], acc[6], acc[7], ovf[4], ovf[5], ovf[6], ovf[7]);
+    }
+    uint64_t s = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) s ^= acc[c] + ovf[c];
+    if (s == 0x123456789ull) out[0] = s;
+}
+
 }  // namespace midint
 
 using namespace midint;
+
+extern "C" midint_status midint_probe_mac(int device, double* pp_per_s) {
+    if (!pp_per_s) return MIDINT_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MIDINT_EHIP;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return MIDINT_EHIP;
+    uint64_t* d_out = nullptr;
+    if (hipMalloc((void**)&d_out, 64) != hipSuccess) return MIDINT_ENOMEM;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const uint32_t blocks = (uint32_t)cus * 32, threads = 256, iters = 2048;
+    double best = 0;
+    for (int rep = 0; rep < 3; rep++) {
+        (void)hipEventRecord(e0, nullptr);
+        probe_mac<<<blocks, threads>>>(iters, 777u + rep, d_out);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        const double pp = (double)blocks * threads * iters * 8;
+        if (pp / (ms * 1e-3) > best) best = pp / (ms * 1e-3);
+    }
+    hipError_t e = hipGetLastError();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return MIDINT_EHIP;
+    *pp_per_s = best;
+    return MIDINT_OK;
+}
 
 extern "C" midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_per_s) {
     if (!mad64_per_s || !add32_per_s) return MIDINT_EINVAL;

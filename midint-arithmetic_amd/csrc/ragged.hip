@@ -108,7 +108,7 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
             __syncthreads();
 
             // ---- column-group convolution, up to kRounds virtual lanes per thread
-            ColGroups g[C::kRounds];
+            ColGroups<Q> g[C::kRounds];
             uint32_t gk1[C::kRounds], gk2[C::kRounds], gop[C::kRounds];
             bool gtwo[C::kRounds], gon[C::kRounds];
 #pragma unroll
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
                         two = false;
                     }
                     const uint32_t* S = lds + o.lds_base;
-                    conv_lane(S, S + o.ns_w + Q, k1, k2, i2, sw, NB, snap_all + 3 * Q * tid, g[rr]);
+                    conv_lane<Q, false>(S, S + o.ns_w + Q, k1, k2, i2, sw, NB, snap_all + 3 * Q * tid, g[rr]);
                     gk1[rr] = k1;
                     gk2[rr] = k2;
                     gop[rr] = k;
@@ -147,7 +147,7 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
             for (int rr = 0; rr < C::kRounds; rr++) {
                 if (gon[rr]) {
                     const RgOp& o = iops[gop[rr]];
-                    lane_store(g[rr], gtwo[rr], gk1[rr], gk2[rr], Lb + o.seg_base, Hb + o.seg_base, o.seg_len);
+                    lane_store<Q>(g[rr], gtwo[rr], gk1[rr], gk2[rr], Lb + o.seg_base, Hb + o.seg_base, o.seg_len);
                 }
             }
             __syncthreads();

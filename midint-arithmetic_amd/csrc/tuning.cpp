@@ -1,0 +1,108 @@
+// Process-wide launch parameters chosen by the autotuner (or defaults):
+// integer values under string keys, read by the launchers.
+//
+//   mul.q         columns per lane group of the classical kernel (4 or 8)
+//   mul.prefetch  1: double-buffer the operand LDS reads
+//   add.rows      rows of 128 limbs per wave for small-m additions (1..8)
+//   add.threads   threads per block for small-m additions (64..1024)
+//   add.nt        1: non-temporal (streaming) 16-byte loads and stores
+//   ntt.threads   threads per block of the NTT kernel (0: L/16)
+//   ragged.threads  256 or 512
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <mutex>
+#include <string>
+#include <unordered_map>
+
+#include "midint.h"
+#include "midint_internal.h"
+
+namespace midint {
+
+namespace {
+std::mutex g_mu;
+std::unordered_map<std::string, int64_t>& table() {
+    static std::unordered_map<std::string, int64_t> t = {
+        {"mul.q", 8}, {"mul.prefetch", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
+        {"ntt.threads", 0}, {"ragged.threads", 256},
+    };
+    return t;
+}
+}  // namespace
+
+int64_t tune_get_class(const char* key, uint32_t limbs, int64_t dflt) {
+    uint32_t c = 1;
+    while (c < limbs) c <<= 1;
+    std::string k = std::string(key) + "@" + std::to_string(c);
+    {
+        std::lock_guard<std::mutex> lk(g_mu);
+        auto it = table().find(k);
+        if (it != table().end()) return it->second;
+    }
+    return tune_get(key, dflt);
+}
+
+int64_t tune_get(const char* key, int64_t dflt) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    auto it = table().find(key);
+    return it == table().end() ? dflt : it->second;
+}
+
+}  // namespace midint
+
+extern "C" {
+
+// "name" sets the default of a parameter; "name@C" (C a power of two) sets
+// it for operand sizes in (C/2, C] limbs.
+midint_status midint_tune_set(const char* key, int64_t value) {
+    if (!key) return MIDINT_EINVAL;
+    std::lock_guard<std::mutex> lk(midint::g_mu);
+    auto& t = midint::table();
+    std::string k(key);
+    const size_t at = k.find('@');
+    const std::string base = at == std::string::npos ? k : k.substr(0, at);
+    if (t.find(base) == t.end()) return MIDINT_ENOTFOUND;
+    if (at != std::string::npos) {
+        const std::string cls = k.substr(at + 1);
+        if (cls.empty() || cls.find_first_not_of("0123456789") != std::string::npos) return MIDINT_EINVAL;
+        const unsigned long c = strtoul(cls.c_str(), nullptr, 10);
+        if (c == 0 || (c & (c - 1)) != 0) return MIDINT_EINVAL;
+    }
+    t[k] = value;
+    return MIDINT_OK;
+}
+
+midint_status midint_tune_load(const char* path) {
+    if (!path) return MIDINT_EINVAL;
+    FILE* f = fopen(path, "r");
+    if (!f) return MIDINT_ENOTFOUND;
+    char line[256];
+    midint_status st = MIDINT_OK;
+    while (fgets(line, sizeof line, f)) {
+        char key[128];
+        long long v = 0;
+        if (line[0] == '#' || line[0] == '\n') continue;
+        if (sscanf(line, "%127s %lld", key, &v) != 2) {
+            st = MIDINT_EINVAL;
+            continue;
+        }
+        midint_status s = midint_tune_set(key, v);
+        if (s != MIDINT_OK) st = s;
+    }
+    fclose(f);
+    return st;
+}
+
+midint_status midint_tune_get(const char* key, int64_t* value) {
+    if (!key || !value) return MIDINT_EINVAL;
+    std::lock_guard<std::mutex> lk(midint::g_mu);
+    auto& t = midint::table();
+    auto it = t.find(key);
+    if (it == t.end()) return MIDINT_ENOTFOUND;
+    *value = it->second;
+    return MIDINT_OK;
+}
+
+}  // extern "C"

@@ -65,3 +65,22 @@ def test_harness_binary_built(mi):
     import os
 
     assert os.access(os.path.join(mi.HERE, "bin", "midint-harness"), os.X_OK)
+
+
+def test_tuning_table(mi, tmp_path):
+    L = mi.lib()
+    old = mi.tune_get("add.rows")
+    mi.tune_set("add.rows", 4)
+    assert mi.tune_get("add.rows") == 4
+    mi.tune_set("add.rows@64", 1)            # per size class
+    assert mi.tune_get("add.rows@64") == 1
+    assert L.midint_tune_set(b"add.rows@63", 1) == 1   # class must be a power of two
+    assert L.midint_tune_set(b"no.such.key", 1) == 6
+    f = tmp_path / "t.txt"
+    f.write_text("# comment\nmul.q@128 4\nadd.nt 0\n")
+    assert mi.tune_load(str(f))
+    assert mi.tune_get("mul.q@128") == 4 and mi.tune_get("add.nt") == 0
+    f.write_text("bogus.key 3\n")
+    assert L.midint_tune_load(str(f).encode()) == 6
+    mi.tune_set("add.rows", old)
+    mi.tune_set("add.nt", 1)
