@@ -55,47 +55,83 @@ def _set(mi, params, cls):
         mi.tune_set(f"{k}@{cls}", v)
 
 
-def tune_op(mi, torch, ctx, op, cls, grid, defaults, limbs_total, rng, log):
-    """Returns (best params, table of (params, us))."""
+def tune_op(mi, torch, ctx, op, cls, grid, defaults, n, rng, log, sets=1):
+    """Times every configuration of `grid` for operand size `cls` on n
+    instances (cycling through `sets` buffer sets, so that additions stream
+    from HBM rather than the Infinity Cache); returns the fastest params."""
     m = cls
-    n = max(1, limbs_total // m)
-    a = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
-    b = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
-    a[: m] = np.uint64(2**64 - 1)
-    b[: m] = np.uint64(2**64 - 1)
-    da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
-    db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
-    out_l = 2 * n * m if op == "mul" else n * m
-    dr = torch.empty(out_l, dtype=torch.int64, device="cuda:0")
+    bufs = []
+    for _ in range(sets):
+        a = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+        b = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+        a[: m] = np.uint64(2**64 - 1)
+        b[: m] = np.uint64(2**64 - 1)
+        out_l = 2 * n * m if op == "mul" else n * m
+        bufs.append((torch.from_numpy(a.view(np.int64)).to("cuda:0"), torch.from_numpy(b.view(np.int64)).to("cuda:0"),
+                     torch.empty(out_l, dtype=torch.int64, device="cuda:0")))
+    it = [0]
 
     def run():
+        da, db, dr = bufs[it[0] % sets]
+        it[0] += 1
         if op == "mul":
             ctx.mul_full(da, db, dr, m)
         else:
             ctx.add(da, db, dr, m)
 
     _set(mi, defaults, cls)
+    it[0] = 0
     run()
     torch.cuda.synchronize()
-    ref = dr.clone()
+    ref = bufs[0][2].clone()
     rows = []
     best, best_us = dict(defaults), None
     for values in itertools.product(*grid.values()):
         params = dict(zip(grid.keys(), values))
         _set(mi, params, cls)
-        dr.zero_()
+        bufs[0][2].zero_()
+        it[0] = 0
         run()
         torch.cuda.synchronize()
-        if not torch.equal(dr, ref):
+        if not torch.equal(bufs[0][2], ref):
             log(f"  {op}@{cls} {params}: output differs from the default configuration -- rejected")
             continue
-        us = _timed(torch, run, 20) * 1e3
+        us = _timed(torch, run, 40 if op == "add" else 10) * 1e3
         rows.append((params, us))
         if best_us is None or us < best_us:
             best, best_us = params, us
     _set(mi, best, cls)
-    log(f"  {op}@{cls}: best {best} {best_us:.2f} us over {len(rows)} candidates")
+    log(f"  {op}@{cls}: best {best} {best_us:.2f} us over {len(rows)} candidates (n = {n})")
     return best, rows
+
+
+def tune_ragged(mi, torch, ctx, log):
+    """Block size of the persistent ragged kernel on the 50000-op batch."""
+    sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
+    from workloads import make_ragged
+
+    ops, a, b, rl = make_ragged(mi.gen_limbs)
+    da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
+    db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
+    ref = None
+    best, best_ms = 256, None
+    for th in (256, 512):
+        dr = torch.zeros(rl, dtype=torch.int64, device="cuda:0")
+        plan = mi.RaggedPlan(ctx, ops, da, db, dr, threads=th)
+        plan.launch()
+        torch.cuda.synchronize()
+        if ref is None:
+            ref = dr.clone()
+        elif not torch.equal(dr, ref):
+            log(f"  ragged threads={th}: output differs -- rejected")
+            plan.close()
+            continue
+        ms = _timed(torch, plan.launch, 10)
+        plan.close()
+        log(f"  ragged threads={th}: {ms:.3f} ms")
+        if best_ms is None or ms < best_ms:
+            best, best_ms = th, ms
+    return best
 
 
 def main(argv=None):
@@ -116,16 +152,20 @@ def main(argv=None):
     add_classes = [16, 64, 256] if args.quick else [1, 2, 4, 8, 16, 32, 64, 128, 256]
     add_grid = {"add.rows": [1, 2, 4], "add.threads": [64, 128, 256, 512], "add.nt": [0, 1]}
     for cls in add_classes:
+        # 32 MiB per operand, 4 rotating sets (384 MiB > the 256 MiB Infinity Cache)
         best, _ = tune_op(mi, torch, ctx, "add", cls, add_grid, {"add.rows": 2, "add.threads": 256, "add.nt": 1},
-                          1 << 22, rng, log)
+                          (1 << 22) // cls, rng, log, sets=4)
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
 
     mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
     mul_grid = {"mul.q": [4, 8], "mul.prefetch": [0, 1]}
     for cls in mul_classes:
-        best, _ = tune_op(mi, torch, ctx, "mul", cls, mul_grid, {"mul.q": 4, "mul.prefetch": 0},
-                          max(1 << 16, 2048 * cls), rng, log)
+        # about 2^30 partial products per launch, never fewer than 512 instances
+        n = min(1 << 20, max(512, (1 << 30) // (2 * cls) ** 2))
+        best, _ = tune_op(mi, torch, ctx, "mul", cls, mul_grid, {"mul.q": 4, "mul.prefetch": 0}, n, rng, log)
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
+
+    lines.append(f"ragged.threads {tune_ragged(mi, torch, ctx, log)}")
 
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:

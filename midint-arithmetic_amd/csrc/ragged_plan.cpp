@@ -179,7 +179,8 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         free_plan(p);
         return MIDINT_EHIP;
     }
-    p->threads = (tuning && tuning->threads == 512) ? 512 : 256;
+    const int64_t th = (tuning && tuning->threads) ? tuning->threads : tune_get("ragged.threads", 256);
+    p->threads = th == 512 ? 512 : 256;
     p->n_items = (uint32_t)h_items.size();
     p->conv_cap = conv_cap;
     p->lds = ragged_lds_size(p->threads, p->conv_cap);
