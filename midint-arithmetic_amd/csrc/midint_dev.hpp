@@ -147,31 +147,90 @@ __device__ __forceinline__ uint32_t block_wave_carry_in(uint32_t g, uint32_t p, 
 constexpr uint64_t kGlP = 0xFFFFFFFF00000001ull;
 constexpr uint64_t kGlEps = 0xFFFFFFFFull;  // 2^64 mod p
 
+// The add/sub/reduce steps are written with the 32-bit carry chain
+// (v_add_co / v_addc / v_sub_co / v_subb) so that every carry and borrow
+// comes from the flag the add itself produces; the compiler otherwise
+// rebuilds them with 64-bit compares. All inputs must be canonical.
+//
+// Each step is ONE asm statement and its carry masks die inside it: the
+// compiler cannot see the SGPR writes of an asm statement, so a mask handed
+// to later code could be spilled (v_writelane) or read without the wait
+// states that VALU-written SGPRs need.
+
+__device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
+__device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
+__device__ __forceinline__ uint64_t join32(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
+
+// a + b mod p: s = a + b (carry c1), t = s + eps (carry c2); t when c1 | c2
+// (c1 and c2 are never both set).
 __device__ __forceinline__ uint64_t gl_add(uint64_t a, uint64_t b) {
-    uint64_t s = a + b;
-    bool c = s < a;
-    return (c || s >= kGlP) ? s - kGlP : s;
+    uint32_t s0, s1, t0, t1;
+    uint64_t k1, k2;
+    asm("v_add_co_u32_e64 %[s0], %[k1], %[a0], %[b0]\n\t"
+        "v_addc_co_u32_e64 %[s1], %[k1], %[a1], %[b1], %[k1]\n\t"
+        "v_add_co_u32_e64 %[t0], %[k2], %[s0], -1\n\t"
+        "v_addc_co_u32_e64 %[t1], %[k2], %[s1], 0, %[k2]\n\t"
+        "v_cndmask_b32_e64 %[s0], %[s0], %[t0], %[k2]\n\t"
+        "v_cndmask_b32_e64 %[s1], %[s1], %[t1], %[k2]\n\t"
+        "v_cndmask_b32_e64 %[s0], %[s0], %[t0], %[k1]\n\t"
+        "v_cndmask_b32_e64 %[s1], %[s1], %[t1], %[k1]"
+        : [s0] "=&v"(s0), [s1] "=&v"(s1), [t0] "=&v"(t0), [t1] "=&v"(t1), [k1] "=&s"(k1), [k2] "=&s"(k2)
+        : [a0] "v"(lo32(a)), [a1] "v"(hi32(a)), [b0] "v"(lo32(b)), [b1] "v"(hi32(b)));
+    return join32(s0, s1);
 }
 
+// a - b mod p: d = a - b, and on borrow d - eps (= d + p mod 2^64), which
+// cannot borrow again for canonical inputs.
 __device__ __forceinline__ uint64_t gl_sub(uint64_t a, uint64_t b) {
-    uint64_t d = a - b;
-    return (a < b) ? d + kGlP : d;
+    uint32_t d0, d1, e;
+    uint64_t k;
+    asm("v_sub_co_u32_e64 %[d0], %[k], %[a0], %[b0]\n\t"
+        "v_subb_co_u32_e64 %[d1], %[k], %[a1], %[b1], %[k]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]\n\t"
+        "v_sub_co_u32_e64 %[d0], %[k], %[d0], %[e]\n\t"
+        "v_subb_co_u32_e64 %[d1], %[k], %[d1], 0, %[k]"
+        : [d0] "=&v"(d0), [d1] "=&v"(d1), [e] "=&v"(e), [k] "=&s"(k)
+        : [a0] "v"(lo32(a)), [a1] "v"(hi32(a)), [b0] "v"(lo32(b)), [b1] "v"(hi32(b)));
+    return join32(d0, d1);
 }
 
-// x = hi * 2^64 + lo reduced with 2^64 = 2^32 - 1 and 2^96 = -1 (mod p).
-__device__ __forceinline__ uint64_t gl_reduce128(uint64_t lo, uint64_t hi) {
-    uint64_t hh = hi >> 32;
-    uint64_t hl = hi & 0xFFFFFFFFull;
-    uint64_t t0 = lo - hh;
-    if (lo < hh) t0 -= kGlEps;
-    uint64_t t1 = (hl << 32) - hl;
-    uint64_t t2 = t0 + t1;
-    if (t2 < t1) t2 += kGlEps;
-    return t2 >= kGlP ? t2 - kGlP : t2;
+// x = hi * 2^64 + lo (lo = l1:l0, hi = hh:hl) reduced with 2^64 = 2^32 - 1
+// and 2^96 = -1 (mod p):  t0 = lo - hh (borrow: -eps), t2 = t0 + hl*eps
+// (carry: +eps; it cannot carry twice), then one conditional subtraction of
+// p, done as t2 + eps with the carry as the condition.
+__device__ __forceinline__ uint64_t gl_reduce(uint32_t l0, uint32_t l1, uint32_t hl, uint32_t hh) {
+    uint32_t x0, x1, y0, y1, e;
+    uint64_t k;
+    asm("v_sub_co_u32_e64 %[x0], %[k], %[l0], %[hh]\n\t"
+        "v_subb_co_u32_e64 %[x1], %[k], %[l1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]\n\t"
+        "v_sub_co_u32_e64 %[x0], %[k], %[x0], %[e]\n\t"
+        "v_subb_co_u32_e64 %[x1], %[k], %[x1], 0, %[k]\n\t"
+        "v_sub_co_u32_e64 %[y0], %[k], 0, %[hl]\n\t"
+        "v_subb_co_u32_e64 %[y1], %[k], %[hl], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[x0], %[k], %[x0], %[y0]\n\t"
+        "v_addc_co_u32_e64 %[x1], %[k], %[x1], %[y1], %[k]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]\n\t"
+        "v_add_co_u32_e64 %[x0], %[k], %[x0], %[e]\n\t"
+        "v_addc_co_u32_e64 %[x1], %[k], %[x1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[y0], %[k], %[x0], -1\n\t"
+        "v_addc_co_u32_e64 %[y1], %[k], %[x1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[x0], %[x0], %[y0], %[k]\n\t"
+        "v_cndmask_b32_e64 %[x1], %[x1], %[y1], %[k]"
+        : [x0] "=&v"(x0), [x1] "=&v"(x1), [y0] "=&v"(y0), [y1] "=&v"(y1), [e] "=&v"(e), [k] "=&s"(k)
+        : [l0] "v"(l0), [l1] "v"(l1), [hl] "v"(hl), [hh] "v"(hh));
+    return join32(x0, x1);
 }
 
+// 64x64 -> 128 from four v_mad_u64_u32 (no partial sum can overflow 64
+// bits), then the reduction.
 __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
-    return gl_reduce128(a * b, __umul64hi(a, b));
+    const uint32_t al = lo32(a), ah = hi32(a), bl = lo32(b), bh = hi32(b);
+    const uint64_t p00 = (uint64_t)al * bl;
+    const uint64_t t = (uint64_t)ah * bl + hi32(p00);
+    const uint64_t u = (uint64_t)al * bh + lo32(t);
+    const uint64_t h = (uint64_t)ah * bh + hi32(t) + hi32(u);
+    return gl_reduce(lo32(p00), lo32(u), lo32(h), hi32(h));
 }
 
 }  // namespace midint

@@ -18,11 +18,16 @@
 //    into a low limb and a high part one limb up, and the two vectors are
 //    added with the wave-ballot row adder.
 //
-// One instance per block, the whole transform in LDS (8L bytes, 128 KiB at
-// L = 16384). Twiddles w^i and w^-i (i < L/2) come from a per-length table
-// built once on the host.
+// One instance per block. Twiddles w^i and w^-i (i < L/2) come from a
+// per-length table built once on the host. Two kernels:
+//  * ntt_kernel (L = 2^6 .. 2^9): the whole transform in LDS (8L bytes),
+//    one radix-2 stage per barrier;
+//  * ntt_fast_kernel (L = 2^10 .. 2^14): register passes of 4-5 stages
+//    (ntt_fast.hpp), one LDS round trip (padded layout) between passes,
+//    a's transform kept in registers for the pointwise product.
 #include "add_core.hpp"
 #include "midint_internal.h"
+#include "ntt_fast.hpp"
 
 namespace midint {
 
@@ -183,6 +188,163 @@ __global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t lg, cons
 }
 
 // ---------------------------------------------------------------------------
+// Register-pass kernel for L = 2^LG, LG in [10, 14]: T = L / V threads with
+// V values each (NttPlan). a's transform stays in registers; b's last
+// forward pass, the pointwise product (with L^-1) and the first inverse pass
+// run on the same registers, so each transform costs NP - 1 LDS round trips.
+
+// Coefficient i of operand `g`: 16-bit pieces of the m-limb integer for
+// products (MODE 0), raw values for the convolution mode (MODE 1).
+template <int MODE>
+__device__ __forceinline__ uint64_t ntt_coef(const uint64_t* __restrict__ g, uint32_t i, uint32_t m) {
+    if (MODE == 0) return i < 4 * m ? (g[i >> 2] >> (16 * (i & 3))) & 0xFFFFull : 0ull;
+    return g[i];
+}
+
+// Forward transform of one operand read straight from global memory (in the
+// first pass's grouping); returns with the last pass's values in v.
+template <int LG, int V, int MODE>
+__device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
+                                        const uint64_t* __restrict__ twf) {
+    using Pl = NttPlan<LG, V>;
+    const uint32_t tid = threadIdx.x;
+    constexpr int R0 = Pl::radix(0);
+#pragma unroll
+    for (int j = 0; j < (V >> R0); j++)
+#pragma unroll
+        for (int k = 0; k < (1 << R0); k++)
+            v[(j << R0) + k] = ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
+    NttFwd<LG, V, 0>::run(x, v, tid, twf);
+}
+
+template <int LG, int V, int MODE>
+__global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, const uint64_t* __restrict__ a,
+                                                                  const uint64_t* __restrict__ b,
+                                                                  uint64_t* __restrict__ r,
+                                                                  const uint64_t* __restrict__ twf,
+                                                                  const uint64_t* __restrict__ twi,
+                                                                  uint64_t inv_len) {
+    using Pl = NttPlan<LG, V>;
+    constexpr uint32_t L = 1u << LG;
+    constexpr uint32_t T = Pl::T;
+    extern __shared__ __align__(16) uint64_t x[];
+    __shared__ uint32_t lds_gp[16];
+    const uint32_t tid = threadIdx.x;
+    const uint64_t* ga = a + (size_t)blockIdx.x * (MODE == 0 ? m : L);
+    const uint64_t* gb = b + (size_t)blockIdx.x * (MODE == 0 ? m : L);
+
+    uint64_t ah[V], v[V];
+    ntt_fwd<LG, V, MODE>(ga, m, x, ah, twf);
+#pragma unroll
+    for (int k = 0; k < V; k++) ah[k] = gl_mul(ah[k], inv_len);
+    __syncthreads();  // every thread is done reading LDS for a's last pass
+    ntt_fwd<LG, V, MODE>(gb, m, x, v, twf);
+#pragma unroll
+    for (int k = 0; k < V; k++) v[k] = gl_mul(v[k], ah[k]);
+    __syncthreads();
+    NttInv<LG, V, Pl::NP - 1>::run(x, v, tid, twi);
+
+    if (MODE == 1) {
+        uint64_t* gr = r + (size_t)blockIdx.x * L;
+        for (uint32_t k = tid; k < L; k += T) gr[k] = x[ntt_pad(k)];
+        return;
+    }
+
+    // ---- exact recombination of the 16-bit-spaced coefficients: limb j is
+    // c[4j] + c[4j+1] 2^16 + c[4j+2] 2^32 + c[4j+3] 2^48 (< 2^112), split into
+    // L (low 64 bits, at limb j) and H (high bits, at limb j + 1), then one
+    // carry-propagating row addition L + H.
+    constexpr uint32_t LPT = V / 4;  // limbs per thread: 2m <= L / 4 = LPT * T
+    const uint32_t outl = 2 * m;
+    uint64_t lo_v[LPT], hi_v[LPT];
+#pragma unroll
+    for (uint32_t e = 0; e < LPT; e++) {
+        const uint32_t j = tid + T * e;
+        uint64_t lo = 0, hi = 0;
+        if (j < outl) {
+            const uint64_t c0 = x[ntt_pad(4 * j)], c1 = x[ntt_pad(4 * j + 1)], c2 = x[ntt_pad(4 * j + 2)],
+                           c3 = x[ntt_pad(4 * j + 3)];
+            unsigned __int128 acc = (unsigned __int128)c0 + ((unsigned __int128)c1 << 16) +
+                                    ((unsigned __int128)c2 << 32) + ((unsigned __int128)c3 << 48);
+            lo = (uint64_t)acc;
+            hi = (uint64_t)(acc >> 64);
+        }
+        lo_v[e] = lo;
+        hi_v[e] = hi;
+    }
+    __syncthreads();
+    uint64_t* Lb = x;
+    uint64_t* Hb = x + outl;
+    if (tid == 0) Hb[0] = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < LPT; e++) {
+        const uint32_t j = tid + T * e;
+        if (j < outl) {
+            Lb[j] = lo_v[e];
+            if (j + 1 < outl) Hb[j + 1] = hi_v[e];
+        }
+    }
+    __syncthreads();
+    constexpr int R = V / 8;  // rows per wave: (T / 64) * R * 128 = L / 4 limbs
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<R> rows;
+#pragma unroll
+    for (int vv = 0; vv < R; vv++) {
+        const uint32_t k = k_first + vv * kRowLimbs;
+        rows.x0[vv] = k < outl ? Lb[k] : 0ull;
+        rows.x1[vv] = k + 1 < outl ? Lb[k + 1] : 0ull;
+        rows.y0[vv] = k < outl ? Hb[k] : 0ull;
+        rows.y1[vv] = k + 1 < outl ? Hb[k + 1] : 0ull;
+        rows.s0[vv] = (k == 0);
+        rows.s1[vv] = false;
+    }
+    block_add_rows<R>(rows, lds_gp);
+    uint64_t* gr = r + (size_t)blockIdx.x * outl;
+#pragma unroll
+    for (int vv = 0; vv < R; vv++) {
+        const uint32_t k = k_first + vv * kRowLimbs;
+        if (k < outl) gr[k] = rows.x0[vv];
+        if (k + 1 < outl) gr[k + 1] = rows.x1[vv];
+    }
+}
+
+template <int LG, int V, int MODE>
+static hipError_t launch_fast(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                              const NttTables& t, hipStream_t s) {
+    constexpr uint32_t L = 1u << LG;
+    static_assert(L / V >= 64 && L / V <= 1024, "one to sixteen waves per block");
+    const size_t lds = (size_t)(L + L / 32) * 8;
+    const void* fn = (const void*)ntt_fast_kernel<LG, V, MODE>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    ntt_fast_kernel<LG, V, MODE><<<n, L / V, lds, s>>>(m, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    return hipGetLastError();
+}
+
+// Values per thread: tunable per transform length ("ntt.vals@L", 16 or 32;
+// 0 for the default): 32 at 2^14 points, where 16 values would need 1024
+// threads and leave too few registers for both transforms.
+static uint32_t ntt_vals(uint32_t lg) {
+    const uint32_t def = lg >= 14 ? 32 : 16;
+    const int64_t v = tune_get_class("ntt.vals", 1u << lg, def);
+    return (v == 16 || (v == 32 && lg >= 11)) ? (uint32_t)v : def;
+}
+
+template <int MODE>
+static hipError_t launch_fast_lg(uint32_t lg, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
+                                 uint64_t* r, const NttTables& t, hipStream_t s) {
+    const bool v32 = ntt_vals(lg) == 32;
+    switch (lg) {
+        case 10: return launch_fast<10, 16, MODE>(n, m, a, b, r, t, s);
+        case 11: return v32 ? launch_fast<11, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<11, 16, MODE>(n, m, a, b, r, t, s);
+        case 12: return v32 ? launch_fast<12, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<12, 16, MODE>(n, m, a, b, r, t, s);
+        case 13: return v32 ? launch_fast<13, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<13, 16, MODE>(n, m, a, b, r, t, s);
+        case 14: return v32 ? launch_fast<14, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<14, 16, MODE>(n, m, a, b, r, t, s);
+    }
+    return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // Host side.
 
 static inline uint64_t h_mulmod(uint64_t a, uint64_t b) {
@@ -234,6 +396,7 @@ hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint6
     const size_t lds = ((size_t)1 << t.lg) * 8;
     if ((1u << t.lg) / T > kNttMaxPerThread) return hipErrorInvalidValue;
     if (2ull * m > (uint64_t)kNttMaxLimbsPerThread * T) return hipErrorInvalidValue;
+    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<0>(t.lg, n, m, a, b, r, t, s);
     hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
@@ -246,6 +409,7 @@ hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uin
     if (t.lg > kNttMaxLg || t.lg < 6) return hipErrorInvalidValue;
     const uint32_t T = ntt_threads(t.lg);
     const size_t lds = ((size_t)1 << t.lg) * 8;
+    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<1>(t.lg, n, 0, a, b, r, t, s);
     hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;

@@ -1,0 +1,144 @@
+// Register-pass Goldilocks NTT for transform lengths L = 2^10 .. 2^14 with
+// T = L / V threads, V values per thread (used by ntt.hip).
+//
+// Stage s of the forward decimation-in-frequency transform pairs x[i] and
+// x[i + h], h = L >> (s+1), (i & h) == 0:  x[i] = u + v,
+// x[i + h] = (u - v) w^e with e = (i mod h) << s. The inverse is the
+// decimation-in-time transform with w^-e, stages in reverse order
+// (bit-reversed in, natural out), so no permutation pass exists.
+//
+// A pass applies R consecutive stages (h from H down to h_min = H >> (R-1))
+// to groups of 2^R elements that those stages only mix among themselves:
+// group (g_hi, g_lo), g_lo < h_min, holds i = g_hi * 2H + g_lo + k * h_min,
+// k < 2^R. Each thread owns V / 2^R groups, g = tid + T * j. Between passes
+// the values go through LDS with one padding slot every 32 values (index i
+// lives at i + (i >> 5)), which keeps the strided group accesses of every
+// pass free of (or at most 2-way) bank conflicts.
+//
+// The transform is split into passes of RM = log2(V) stages, the last one
+// taking what is left: NttPlan<LG, V>.
+#pragma once
+
+#include "midint_dev.hpp"
+
+namespace midint {
+
+__device__ __forceinline__ uint32_t ntt_pad(uint32_t i) { return i + (i >> 5); }
+
+template <int LG, int V>
+struct NttPlan {
+    static constexpr int RM = V == 32 ? 5 : 4;  // stages per full pass
+    static constexpr int NP = (LG + RM - 1) / RM;
+    static constexpr int radix(int p) { return (p + 1) * RM <= LG ? RM : LG - p * RM; }
+    static constexpr uint32_t T = (1u << LG) / V;
+};
+
+// One pass of R stages on the thread's registers v[V] (V / 2^R groups of
+// 2^R values, group j at v[j << R ...]). s0: first stage index of the pass.
+template <int R, bool INV, int V>
+__device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t lg, uint32_t s0, uint32_t tid,
+                                              uint32_t T, const uint64_t* __restrict__ tw) {
+    constexpr int G = V >> R;  // groups per thread
+    constexpr int K = 1 << R;  // values per group
+    const uint32_t lgH = lg - 1 - s0;      // H = 2^lgH
+    const uint32_t lgmin = lgH - (R - 1);  // h_min
+#pragma unroll
+    for (int j = 0; j < G; j++) {
+        const uint32_t g = tid + T * j;
+        const uint32_t g_lo = g & ((1u << lgmin) - 1);
+#pragma unroll
+        for (int st = 0; st < R; st++) {
+            // forward: jj = R-1 .. 0 (h = H .. h_min); inverse: jj = 0 .. R-1
+            const int jj = INV ? st : (R - 1 - st);
+            const uint32_t s = s0 + (R - 1 - jj);  // stage index of h = h_min << jj
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (k & (1 << jj)) continue;
+                const uint32_t e = (g_lo + ((uint32_t)(k & ((1 << jj) - 1)) << lgmin)) << s;
+                const uint64_t w = tw[e];
+                uint64_t& a = v[j * K + k];
+                uint64_t& b = v[j * K + k + (1 << jj)];
+                if (!INV) {
+                    const uint64_t u = a, x = b;
+                    a = gl_add(u, x);
+                    b = gl_mul(gl_sub(u, x), w);
+                } else {
+                    const uint64_t u = a, x = gl_mul(b, w);
+                    a = gl_add(u, x);
+                    b = gl_sub(u, x);
+                }
+            }
+        }
+    }
+}
+
+// Element index of value k of the thread's group j in the pass starting at
+// stage s0 with R stages.
+template <int R>
+__device__ __forceinline__ uint32_t ntt_group_index(uint32_t lg, uint32_t s0, uint32_t tid, uint32_t T, int j,
+                                                    int k) {
+    const uint32_t lgH = lg - 1 - s0;
+    const uint32_t lgmin = lgH - (R - 1);
+    const uint32_t g = tid + T * j;
+    const uint32_t g_lo = g & ((1u << lgmin) - 1);
+    const uint32_t g_hi = g >> lgmin;
+    return (g_hi << (lgH + 1)) + g_lo + ((uint32_t)k << lgmin);
+}
+
+template <int R, int V>
+__device__ __forceinline__ void ntt_load_pass(const uint64_t* x, uint64_t (&v)[V], uint32_t lg, uint32_t s0,
+                                              uint32_t tid, uint32_t T) {
+#pragma unroll
+    for (int j = 0; j < (V >> R); j++)
+#pragma unroll
+        for (int k = 0; k < (1 << R); k++) v[(j << R) + k] = x[ntt_pad(ntt_group_index<R>(lg, s0, tid, T, j, k))];
+}
+
+template <int R, int V>
+__device__ __forceinline__ void ntt_store_pass(uint64_t* x, const uint64_t (&v)[V], uint32_t lg, uint32_t s0,
+                                               uint32_t tid, uint32_t T) {
+#pragma unroll
+    for (int j = 0; j < (V >> R); j++)
+#pragma unroll
+        for (int k = 0; k < (1 << R); k++) x[ntt_pad(ntt_group_index<R>(lg, s0, tid, T, j, k))] = v[(j << R) + k];
+}
+
+// Forward passes p .. NP-1 starting from values of pass p's grouping in v;
+// returns with the last pass's values in v.
+template <int LG, int V, int P>
+struct NttFwd {
+    __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
+                                               const uint64_t* __restrict__ twf) {
+        using Pl = NttPlan<LG, V>;
+        constexpr int R = Pl::radix(P);
+        ntt_pass_regs<R, false, V>(v, LG, P * Pl::RM, tid, Pl::T, twf);
+        if constexpr (P + 1 < Pl::NP) {
+            constexpr int R1 = Pl::radix(P + 1);
+            ntt_store_pass<R, V>(x, v, LG, P * Pl::RM, tid, Pl::T);
+            __syncthreads();
+            ntt_load_pass<R1, V>(x, v, LG, (P + 1) * Pl::RM, tid, Pl::T);
+            NttFwd<LG, V, P + 1>::run(x, v, tid, twf);
+        }
+    }
+};
+
+// Inverse passes p .. 0 starting from values of pass p's grouping in v; ends
+// with the natural-order result stored in LDS (padded) and a barrier.
+template <int LG, int V, int P>
+struct NttInv {
+    __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
+                                               const uint64_t* __restrict__ twi) {
+        using Pl = NttPlan<LG, V>;
+        constexpr int R = Pl::radix(P);
+        ntt_pass_regs<R, true, V>(v, LG, P * Pl::RM, tid, Pl::T, twi);
+        ntt_store_pass<R, V>(x, v, LG, P * Pl::RM, tid, Pl::T);
+        __syncthreads();
+        if constexpr (P > 0) {
+            constexpr int R1 = Pl::radix(P - 1);
+            ntt_load_pass<R1, V>(x, v, LG, (P - 1) * Pl::RM, tid, Pl::T);
+            NttInv<LG, V, P - 1>::run(x, v, tid, twi);
+        }
+    }
+};
+
+}  // namespace midint
