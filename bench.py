@@ -82,75 +82,9 @@ def cpu_baseline_mul(a, b, m, threads, budget_s=1.0):
     return out
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--calibrate", action="store_true",
-                    help="also run a 256 MiB int64 negation (2 x 256 MiB of HBM traffic per launch) x5, "
-                         "the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(f"bench.py runs on one GPU only: every instance is independent, so N GPUs would each run "
-              f"this same kernel on 1/N of the batch with no exchange (see DESIGN.md); --gpus {args.gpus} "
-              f"is not measured here.", file=sys.stderr)
-        sys.exit(2)
-
-    import torch
-
-    from __graft_entry__ import load_package
-    from workloads import make_inputs
-
-    mi = load_package()
-    mi.lib()
-    ctx = mi.Context(0)
-
-    # ---------------------------------------------------------- headline: mul
-    n, m = 16384, 128
-    a, b = make_inputs("mul_16384x128", mi.gen_limbs)
-    da, db = to_dev(torch, a), to_dev(torch, b)
-    dr = torch.empty(2 * n * m, dtype=torch.int64, device="cuda:0")
-    log(f"mul_full {n} x {m} limbs: warmup {args.warmup}, steps {args.steps}")
-    ms = timed(torch, lambda: ctx.mul_full(da, db, dr, m), args.steps, args.warmup)
-    muls_per_s = n / (ms * 1e-3)
-    pp_per_launch = n * (2 * m) ** 2  # 32x32->64 partial products of the full product
-    mad_rate, add_rate = mi.probe_valu(0)
-    achieved_pp = pp_per_launch / (ms * 1e-3)
-    log(f"mul_full: {ms * 1e3:.1f} us/batch, {muls_per_s / 1e6:.2f} M muls/s; "
-        f"probe v_mad_u64_u32 {mad_rate / 1e12:.2f} T/s")
-
-    # classical low-half product (reference semantics) at 256 limbs = 128x128 padded, like the reference harness
-    # ---------------------------------------------------------- NTT
-    nn, mn = 1024, 2048
-    an, bn = make_inputs("ntt_1024x2048", mi.gen_limbs)
-    dan, dbn = to_dev(torch, an), to_dev(torch, bn)
-    drn = torch.empty(2 * nn * mn, dtype=torch.int64, device="cuda:0")
-    ctx.ntt_prepare(mn)
-    ms_ntt = timed(torch, lambda: ctx.mul_ntt(dan, dbn, drn, mn), max(5, args.steps // 20), 2)
-    log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch")
-    del dan, dbn, drn
-
-    # ---------------------------------------------------------- add (HBM)
-    na, ma = 65536, 64
-    aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
-    sets = []
-    for _ in range(4):  # 4 x 96 MiB > the 256 MiB Infinity Cache
-        sets.append((to_dev(torch, aa), to_dev(torch, ba), torch.empty(na * ma, dtype=torch.int64, device="cuda:0")))
-    it = [0]
-
-    def add_step():
-        x, y, z = sets[it[0] & 3]
-        it[0] += 1
-        ctx.add(x, y, z, ma)
-
-    ms_add = timed(torch, add_step, 2 * args.steps, max(4, args.warmup))
-    add_bytes = 3 * na * ma * 8
-    add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
-    log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
-
-    # ---------------------------------------------------------- ragged batch
+def bench_ragged(torch, mi, ctx, args):
+    """Ragged batch (50000 ops): one persistent hipGraph launch vs one
+    fixed-size launch per (op, power-of-two size class)."""
     from workloads import make_ragged, size_class
 
     ops, ra, rb, rlimbs = make_ragged(mi.gen_limbs)
@@ -190,8 +124,15 @@ def main():
     log(f"ragged {len(ops)} ops: persistent {ms_rag:.3f} ms ({pstats}), plain per-size {ms_plain:.3f} ms, "
         f"speedup {ms_plain / ms_rag:.2f}x")
     del plan, plain, dra, drb, drr
+    return {"ops": len(ops), "muls": n_mul, "persistent_ms": ms_rag, "plain_per_size_ms": ms_plain,
+            "speedup": ms_plain / ms_rag, "muls_per_s_persistent": n_mul / (ms_rag * 1e-3),
+            "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plan": pstats,
+            "note": "one hipGraph launch (counter reset + persistent kernel) vs one fixed-size "
+                    "launch per (op, power-of-two size class) on zero-padded operands"}
 
-    # ---------------------------------------------------------- serving loop
+
+def bench_serving(mi, ctx):
+    """Serving loop under Poisson arrivals at three offered rates."""
     from workloads import make_requests
 
     reqs = make_requests(mi.gen_limbs, 3000)
@@ -207,6 +148,86 @@ def main():
         serving.append(res)
         log(f"serving qps {qps}: p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
             f"{res['ops_per_s']:.0f} ops/s, {res['batches']} batches")
+
+    return {"requests": len(reqs), "mix": "70% mul / 30% add, operands 8..4096 limbs log-uniform",
+            "points": serving}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", choices=("all", "core"), default="all",
+                    help="core: only the three fixed-size workloads (mul, NTT, add), so that a profiler run "
+                         "sees each kernel from one workload")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run a 256 MiB int64 negation (2 x 256 MiB of HBM traffic per launch) x5, "
+                         "the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only: every instance is independent, so N GPUs would each run "
+              f"this same kernel on 1/N of the batch with no exchange (see DESIGN.md); --gpus {args.gpus} "
+              f"is not measured here.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    from __graft_entry__ import load_package
+    from workloads import make_inputs
+
+    mi = load_package()
+    mi.lib()
+    ctx = mi.Context(0)
+
+    # ---------------------------------------------------------- headline: mul
+    n, m = 16384, 128
+    a, b = make_inputs("mul_16384x128", mi.gen_limbs)
+    da, db = to_dev(torch, a), to_dev(torch, b)
+    dr = torch.empty(2 * n * m, dtype=torch.int64, device="cuda:0")
+    log(f"mul_full {n} x {m} limbs: warmup {args.warmup}, steps {args.steps}")
+    ms = timed(torch, lambda: ctx.mul_full(da, db, dr, m), args.steps, args.warmup)
+    muls_per_s = n / (ms * 1e-3)
+    pp_per_launch = n * (2 * m) ** 2  # 32x32->64 partial products of the full product
+    mad_rate, add_rate = mi.probe_valu(0)
+    achieved_pp = pp_per_launch / (ms * 1e-3)
+    log(f"mul_full: {ms * 1e3:.1f} us/batch, {muls_per_s / 1e6:.2f} M muls/s; "
+        f"probe v_mad_u64_u32 {mad_rate / 1e12:.2f} T/s")
+
+    # ---------------------------------------------------------- NTT
+    nn, mn = 1024, 2048
+    an, bn = make_inputs("ntt_1024x2048", mi.gen_limbs)
+    dan, dbn = to_dev(torch, an), to_dev(torch, bn)
+    drn = torch.empty(2 * nn * mn, dtype=torch.int64, device="cuda:0")
+    ctx.ntt_prepare(mn)
+    ms_ntt = timed(torch, lambda: ctx.mul_ntt(dan, dbn, drn, mn), max(5, args.steps // 20), 2)
+    ntt_pp = nn * 360448 * 4  # radix-2-equivalent modmuls per product (3 transforms + pointwise), 4 pp each
+    log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch")
+    del dan, dbn, drn
+
+    # ---------------------------------------------------------- add (HBM)
+    na, ma = 65536, 64
+    aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
+    sets = []
+    for _ in range(4):  # 4 x 96 MiB > the 256 MiB Infinity Cache
+        sets.append((to_dev(torch, aa), to_dev(torch, ba), torch.empty(na * ma, dtype=torch.int64, device="cuda:0")))
+    it = [0]
+
+    def add_step():
+        x, y, z = sets[it[0] & 3]
+        it[0] += 1
+        ctx.add(x, y, z, ma)
+
+    ms_add = timed(torch, add_step, 2 * args.steps, max(4, args.warmup))
+    add_bytes = 3 * na * ma * 8
+    add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
+    log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
+
+    ragged, serving = None, None
+    if args.only == "all":
+        ragged = bench_ragged(torch, mi, ctx, args)
+        serving = bench_serving(mi, ctx)
 
     if args.calibrate:
         src = torch.ones(32 * MIB, dtype=torch.int64, device="cuda:0")
@@ -227,7 +248,6 @@ def main():
                          f"on {threads} threads; oracle C restatement",
                "gmp_mpn_mul_muls_per_s": base.get("gmp")}
         log(f"cpu baseline: {cpu}")
-
     out = {
         "metric": "big-int muls/s at 8192-bit",
         "value": muls_per_s,
@@ -255,15 +275,16 @@ def main():
                                           "unit": "GB/s", "frac": add_gbs / HBM_PEAK_GBS,
                                           "traffic": measured_traffic("add_65536x64")},
                              "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)"},
-            "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3)},
-            "ragged_50000": {"ops": len(ops), "muls": n_mul, "persistent_ms": ms_rag, "plain_per_size_ms": ms_plain,
-                             "speedup": ms_plain / ms_rag, "muls_per_s_persistent": n_mul / (ms_rag * 1e-3),
-                             "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plan": pstats,
-                             "note": "one hipGraph launch (counter reset + persistent kernel) vs one fixed-size "
-                                     "launch per (op, power-of-two size class) on zero-padded operands"},
+            "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3),
+                              "roofline": {"bound": "valu", "achieved": ntt_pp / (ms_ntt * 1e-3) / 1e12,
+                                           "peak": mad_rate / 1e12,
+                                           "unit": "T partial-products/s (360448 radix-2-equivalent modmuls x 4 "
+                                                   "per product)",
+                                           "frac": ntt_pp / (ms_ntt * 1e-3) / mad_rate,
+                                           "traffic": measured_traffic("mul_ntt_1024x2048")}},
+            "ragged_50000": ragged,
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
-            "serving": {"requests": len(reqs), "mix": "70% mul / 30% add, operands 8..4096 limbs log-uniform",
-                        "points": serving},
+            "serving": serving,
         },
     }
     ctx.close()

@@ -209,12 +209,17 @@ __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t
     using Pl = NttPlan<LG, V>;
     const uint32_t tid = threadIdx.x;
     constexpr int R0 = Pl::radix(0);
+    // products: the 4m <= L/2 coefficients sit in the lower half, which is
+    // where k < 2^(R0-1) of the first pass's groups point
+    constexpr bool ZHI = MODE == 0;
 #pragma unroll
     for (int j = 0; j < (V >> R0); j++)
 #pragma unroll
         for (int k = 0; k < (1 << R0); k++)
-            v[(j << R0) + k] = ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
-    NttFwd<LG, V, 0>::run(x, v, tid, twf);
+            v[(j << R0) + k] = (ZHI && k >= (1 << (R0 - 1)))
+                                   ? 0ull
+                                   : ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
+    NttFwd<LG, V, 0, ZHI>::run(x, v, tid, twf);
 }
 
 template <int LG, int V, int MODE>

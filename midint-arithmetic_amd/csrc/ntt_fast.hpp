@@ -33,15 +33,19 @@ struct NttPlan {
     static constexpr uint32_t T = (1u << LG) / V;
 };
 
-// One pass of R stages on the thread's registers v[V] (V / 2^R groups of
-// 2^R values, group j at v[j << R ...]). s0: first stage index of the pass.
-template <int R, bool INV, int V>
-__device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t lg, uint32_t s0, uint32_t tid,
-                                              uint32_t T, const uint64_t* __restrict__ tw) {
+// One pass of R stages starting at stage S0 on the thread's registers v[V]
+// (V / 2^R groups of 2^R values, group j at v[j << R ...]). In the last
+// pass h_min = 1, so g_lo = 0 and every twiddle index is a compile-time
+// constant; the butterflies whose index is 0 (twiddle 1) skip the multiply.
+// ZHI (forward, first pass): the upper half of the input is zero, so the
+// first stage is a = u, b = u w.
+template <int R, bool INV, int V, int LG, int S0, bool ZHI = false>
+__device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, const uint64_t* __restrict__ tw) {
     constexpr int G = V >> R;  // groups per thread
     constexpr int K = 1 << R;  // values per group
-    const uint32_t lgH = lg - 1 - s0;      // H = 2^lgH
-    const uint32_t lgmin = lgH - (R - 1);  // h_min
+    constexpr uint32_t T = (1u << LG) / V;
+    constexpr int lgH = LG - 1 - S0;      // H = 2^lgH
+    constexpr int lgmin = lgH - (R - 1);  // h_min
 #pragma unroll
     for (int j = 0; j < G; j++) {
         const uint32_t g = tid + T * j;
@@ -50,20 +54,22 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t lg, uin
         for (int st = 0; st < R; st++) {
             // forward: jj = R-1 .. 0 (h = H .. h_min); inverse: jj = 0 .. R-1
             const int jj = INV ? st : (R - 1 - st);
-            const uint32_t s = s0 + (R - 1 - jj);  // stage index of h = h_min << jj
+            const int s = S0 + (R - 1 - jj);  // stage index of h = h_min << jj
 #pragma unroll
             for (int k = 0; k < K; k++) {
                 if (k & (1 << jj)) continue;
-                const uint32_t e = (g_lo + ((uint32_t)(k & ((1 << jj) - 1)) << lgmin)) << s;
-                const uint64_t w = tw[e];
+                const uint32_t klo = (uint32_t)(k & ((1 << jj) - 1));
+                const bool unit = lgmin == 0 && klo == 0;  // w^0
                 uint64_t& a = v[j * K + k];
                 uint64_t& b = v[j * K + k + (1 << jj)];
                 if (!INV) {
                     const uint64_t u = a, x = b;
-                    a = gl_add(u, x);
-                    b = gl_mul(gl_sub(u, x), w);
+                    const bool zero_x = ZHI && S0 == 0 && s == 0;
+                    a = zero_x ? u : gl_add(u, x);
+                    const uint64_t d = zero_x ? u : gl_sub(u, x);
+                    b = unit ? d : gl_mul(d, tw[(g_lo + (klo << lgmin)) << s]);
                 } else {
-                    const uint64_t u = a, x = gl_mul(b, w);
+                    const uint64_t u = a, x = unit ? b : gl_mul(b, tw[(g_lo + (klo << lgmin)) << s]);
                     a = gl_add(u, x);
                     b = gl_sub(u, x);
                 }
@@ -73,7 +79,7 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t lg, uin
 }
 
 // Element index of value k of the thread's group j in the pass starting at
-// stage s0 with R stages.
+// stage s0 with R stages: B_j + k * h_min with B_j = g_hi * 2H + g_lo.
 template <int R>
 __device__ __forceinline__ uint32_t ntt_group_index(uint32_t lg, uint32_t s0, uint32_t tid, uint32_t T, int j,
                                                     int k) {
@@ -85,38 +91,60 @@ __device__ __forceinline__ uint32_t ntt_group_index(uint32_t lg, uint32_t s0, ui
     return (g_hi << (lgH + 1)) + g_lo + ((uint32_t)k << lgmin);
 }
 
-template <int R, int V>
-__device__ __forceinline__ void ntt_load_pass(const uint64_t* x, uint64_t (&v)[V], uint32_t lg, uint32_t s0,
-                                              uint32_t tid, uint32_t T) {
+// Padded LDS slot of that element. The padding is linear inside a group:
+// pad(B + C) = pad(B) + C + (C >> 5) for C = k * h_min, because the low five
+// bits of B and C never carry (g_lo < h_min; when h_min < 32 the group lies
+// in an aligned 2H-block or 2H >= 32 and B is 32-aligned apart from g_lo).
+// So each group needs one base register and the per-k part is a
+// compile-time offset.
+template <int R, int V, int LG, int S0>
+__device__ __forceinline__ void ntt_load_pass(const uint64_t* x, uint64_t (&v)[V], uint32_t tid) {
+    constexpr uint32_t T = (1u << LG) / V;
+    constexpr int lgH = LG - 1 - S0;
+    constexpr int lgmin = lgH - (R - 1);
 #pragma unroll
-    for (int j = 0; j < (V >> R); j++)
+    for (int j = 0; j < (V >> R); j++) {
+        const uint32_t g = tid + T * j;
+        const uint64_t* xb = x + ntt_pad(((g >> lgmin) << (lgH + 1)) + (g & ((1u << lgmin) - 1)));
 #pragma unroll
-        for (int k = 0; k < (1 << R); k++) v[(j << R) + k] = x[ntt_pad(ntt_group_index<R>(lg, s0, tid, T, j, k))];
+        for (int k = 0; k < (1 << R); k++) {
+            const uint32_t c = (uint32_t)k << lgmin;
+            v[(j << R) + k] = xb[c + (c >> 5)];
+        }
+    }
 }
 
-template <int R, int V>
-__device__ __forceinline__ void ntt_store_pass(uint64_t* x, const uint64_t (&v)[V], uint32_t lg, uint32_t s0,
-                                               uint32_t tid, uint32_t T) {
+template <int R, int V, int LG, int S0>
+__device__ __forceinline__ void ntt_store_pass(uint64_t* x, const uint64_t (&v)[V], uint32_t tid) {
+    constexpr uint32_t T = (1u << LG) / V;
+    constexpr int lgH = LG - 1 - S0;
+    constexpr int lgmin = lgH - (R - 1);
 #pragma unroll
-    for (int j = 0; j < (V >> R); j++)
+    for (int j = 0; j < (V >> R); j++) {
+        const uint32_t g = tid + T * j;
+        uint64_t* xb = x + ntt_pad(((g >> lgmin) << (lgH + 1)) + (g & ((1u << lgmin) - 1)));
 #pragma unroll
-        for (int k = 0; k < (1 << R); k++) x[ntt_pad(ntt_group_index<R>(lg, s0, tid, T, j, k))] = v[(j << R) + k];
+        for (int k = 0; k < (1 << R); k++) {
+            const uint32_t c = (uint32_t)k << lgmin;
+            xb[c + (c >> 5)] = v[(j << R) + k];
+        }
+    }
 }
 
 // Forward passes p .. NP-1 starting from values of pass p's grouping in v;
 // returns with the last pass's values in v.
-template <int LG, int V, int P>
+template <int LG, int V, int P, bool ZHI = false>
 struct NttFwd {
     __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
                                                const uint64_t* __restrict__ twf) {
         using Pl = NttPlan<LG, V>;
         constexpr int R = Pl::radix(P);
-        ntt_pass_regs<R, false, V>(v, LG, P * Pl::RM, tid, Pl::T, twf);
+        ntt_pass_regs<R, false, V, LG, P * Pl::RM, ZHI && P == 0>(v, tid, twf);
         if constexpr (P + 1 < Pl::NP) {
             constexpr int R1 = Pl::radix(P + 1);
-            ntt_store_pass<R, V>(x, v, LG, P * Pl::RM, tid, Pl::T);
+            ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
             __syncthreads();
-            ntt_load_pass<R1, V>(x, v, LG, (P + 1) * Pl::RM, tid, Pl::T);
+            ntt_load_pass<R1, V, LG, (P + 1) * Pl::RM>(x, v, tid);
             NttFwd<LG, V, P + 1>::run(x, v, tid, twf);
         }
     }
@@ -130,12 +158,12 @@ struct NttInv {
                                                const uint64_t* __restrict__ twi) {
         using Pl = NttPlan<LG, V>;
         constexpr int R = Pl::radix(P);
-        ntt_pass_regs<R, true, V>(v, LG, P * Pl::RM, tid, Pl::T, twi);
-        ntt_store_pass<R, V>(x, v, LG, P * Pl::RM, tid, Pl::T);
+        ntt_pass_regs<R, true, V, LG, P * Pl::RM>(v, tid, twi);
+        ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
         __syncthreads();
         if constexpr (P > 0) {
             constexpr int R1 = Pl::radix(P - 1);
-            ntt_load_pass<R1, V>(x, v, LG, (P - 1) * Pl::RM, tid, Pl::T);
+            ntt_load_pass<R1, V, LG, (P - 1) * Pl::RM>(x, v, tid);
             NttInv<LG, V, P - 1>::run(x, v, tid, twi);
         }
     }
