@@ -110,6 +110,41 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
  * on first use). */
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);
 
+/* ------------------------------------------------- prime-field variants */
+/* The reference's FFT primes (FftPrime32 / FftPrime64,
+ * cuda/ker-fft-mul.cu.h:44-72; selected in cuda/main.cu:765-766) and
+ * Goldilocks. */
+typedef enum {
+    MIDINT_PRIME_GOLDILOCKS = 0, /* 2^64 - 2^32 + 1, generator 7        */
+    MIDINT_PRIME_P32 = 1,        /* 3 * 2^30 + 1 = 3221225473, g = 13  */
+    MIDINT_PRIME_P64 = 2         /* 29 * 2^57 + 1 = 4179340454199820289, g = 21 */
+} midint_prime;
+
+enum {
+    MIDINT_NTT_INVERSE = 1,       /* transform with omega^-1, then scale by L^-1 */
+    MIDINT_NTT_REFERENCE_ROOT = 2 /* omega = g^(2^(n - lg)) as the reference's
+                                     getOmega (cuda/ker-fft-mul.cu.h:191-199);
+                                     p32 / p64 only. Otherwise omega =
+                                     g^((p-1)/L), a primitive L-th root. */
+};
+
+/* Batched length-2^lg transform (1 <= lg <= 14) of n instances of 2^lg
+ * values each (u64, every value < p), natural order in and out: the
+ * reference's fft / ifft (cuda/ker-fft-mul.cu.h:304-368), i.e. a bit-reversal
+ * permutation then radix-2 decimation-in-time stages. With the reference's
+ * root the p32 / p64 transforms are bit-compatible with the reference's (that
+ * root has order k * L for p - 1 = k * 2^n, so the map is not a DFT and the
+ * inverse does not undo it); with the primitive root they are true NTTs. */
+midint_status midint_ntt_transform(midint_ctx* ctx, uint32_t prime, uint32_t flags, uint32_t n, uint32_t lg,
+                                   const uint64_t* d_x, uint64_t* d_y, void* stream);
+
+/* Exact full product (2m limbs) through an NTT over the chosen prime, the
+ * reference's gpuMulFFT<FftPrime32 / FftPrime64, m> (cuda/main.cu:415-579)
+ * made exact: coefficients of 8 bits (p32, m <= 1024) or 16 bits (p64,
+ * Goldilocks, m <= 2048) keep every product coefficient below p. */
+midint_status midint_mul_ntt_prime(midint_ctx* ctx, uint32_t prime, uint32_t n, uint32_t m, const uint64_t* d_a,
+                                   const uint64_t* d_b, uint64_t* d_r, void* stream);
+
 /* ------------------------------------------------------- ragged batches */
 /* One op of a ragged batch. Operands live in two packed device buffers
  * (d_a, d_b) at limb offsets off_a / off_b; the result is written to d_r at

@@ -51,6 +51,8 @@ _SIGS = {
     "midint_poly": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_conv": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_ntt_transform": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp]),
+    "midint_mul_ntt_prime": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_prepare": (_int, [_vp, _u32]),
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
@@ -146,6 +148,11 @@ def _stream(stream):
     return ctypes.c_void_p(stream.cuda_stream)
 
 
+PRIME_GOLDILOCKS, PRIME_P32, PRIME_P64 = 0, 1, 2
+PRIMES = {PRIME_GOLDILOCKS: 0xFFFFFFFF00000001, PRIME_P32: 3221225473, PRIME_P64: 4179340454199820289}
+NTT_INVERSE, NTT_REFERENCE_ROOT = 1, 2
+
+
 class Context:
     """Owns a midint_ctx (device binding, NTT twiddle cache, scratch)."""
 
@@ -199,6 +206,18 @@ class Context:
         n = a.numel() >> lg
         check(lib().midint_ntt_conv(self._h, n, lg, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
               "midint_ntt_conv")
+
+    def ntt_transform(self, x, y, lg, prime=0, inverse=False, reference_root=False, stream=None):
+        """Batched 2^lg-point transform over PRIME_* (see midint_ntt_transform)."""
+        n = x.numel() >> lg
+        flags = (NTT_INVERSE if inverse else 0) | (NTT_REFERENCE_ROOT if reference_root else 0)
+        check(lib().midint_ntt_transform(self._h, prime, flags, n, lg, _ptr(x), _ptr(y), _stream(stream)),
+              "midint_ntt_transform")
+
+    def mul_ntt_prime(self, a, b, r, m, prime, stream=None):
+        n = a.numel() // m
+        check(lib().midint_mul_ntt_prime(self._h, prime, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_mul_ntt_prime")
 
     def ntt_prepare(self, m):
         check(lib().midint_ntt_prepare(self._h, m), "midint_ntt_prepare")

@@ -7,6 +7,7 @@
 #include <string.h>
 
 #include <map>
+#include <vector>
 #include <mutex>
 
 #include "midint.h"
@@ -18,6 +19,7 @@ struct midint_ctx {
     int device;
     std::mutex mu;
     std::map<uint32_t, NttTables> ntt;  // keyed by log2 length
+    std::map<uint32_t, PrimeTables> prime_ntt;  // keyed by (prime, lg, root kind)
     uint64_t* scratch = nullptr;
     size_t scratch_limbs = 0;
 };
@@ -78,6 +80,10 @@ midint_status midint_destroy(midint_ctx* ctx) {
     (void)hipSetDevice(ctx->device);
     (void)hipDeviceSynchronize();
     for (auto& kv : ctx->ntt) {
+        (void)hipFree(kv.second.d_fwd);
+        (void)hipFree(kv.second.d_inv);
+    }
+    for (auto& kv : ctx->prime_ntt) {
         (void)hipFree(kv.second.d_fwd);
         (void)hipFree(kv.second.d_inv);
     }
@@ -230,6 +236,74 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
     NttTables t;
     if ((st = get_tables(ctx, lg, &t)) != MIDINT_OK) return st;
     return hip_status(launch_ntt_conv(n, d_a, d_b, d_r, t, (hipStream_t)stream));
+}
+
+// ------------------------------------------------- prime-field variants
+static midint_status get_prime_tables(midint_ctx* ctx, uint32_t prime, uint32_t lg, bool ref_root,
+                                      PrimeTables* out) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const uint32_t key = (prime << 8) | (lg << 1) | (ref_root ? 1u : 0u);
+    auto it = ctx->prime_ntt.find(key);
+    if (it != ctx->prime_ntt.end()) {
+        *out = it->second;
+        return MIDINT_OK;
+    }
+    const size_t half = (size_t)1 << (lg - 1);
+    std::vector<uint64_t> hf(half), hi(half);
+    PrimeTables t;
+    if (ntt_prime_tables(prime, lg, ref_root, hf.data(), hi.data(), &t.scale_inv, &t.scale_prod) != 0)
+        return MIDINT_EUNSUPPORTED;
+    t.d_fwd = t.d_inv = nullptr;
+    hipError_t e = hipMalloc((void**)&t.d_fwd, half * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&t.d_inv, half * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpy(t.d_fwd, hf.data(), half * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.d_inv, hi.data(), half * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        if (t.d_fwd) (void)hipFree(t.d_fwd);
+        if (t.d_inv) (void)hipFree(t.d_inv);
+        return hip_status(e);
+    }
+    ctx->prime_ntt[key] = t;
+    *out = t;
+    return MIDINT_OK;
+}
+
+static bool valid_prime(uint32_t prime) {
+    return prime == MIDINT_PRIME_GOLDILOCKS || prime == MIDINT_PRIME_P32 || prime == MIDINT_PRIME_P64;
+}
+
+midint_status midint_ntt_transform(midint_ctx* ctx, uint32_t prime, uint32_t flags, uint32_t n, uint32_t lg,
+                                   const uint64_t* d_x, uint64_t* d_y, void* stream) {
+    if (!ctx || !valid_prime(prime) || (flags & ~3u)) return MIDINT_EINVAL;
+    const bool ref_root = (flags & MIDINT_NTT_REFERENCE_ROOT) != 0;
+    if (ref_root && prime == MIDINT_PRIME_GOLDILOCKS) return MIDINT_EINVAL;
+    if (lg < 1 || lg > kNttMaxLg) return MIDINT_EUNSUPPORTED;
+    if (n == 0) return MIDINT_OK;
+    if (!d_x || !d_y) return MIDINT_EINVAL;
+    midint_status st = bind_device(ctx);
+    if (st != MIDINT_OK) return st;
+    PrimeTables t;
+    if ((st = get_prime_tables(ctx, prime, lg, ref_root, &t)) != MIDINT_OK) return st;
+    const int mode = (flags & MIDINT_NTT_INVERSE) ? 2 : 1;
+    return hip_status(launch_ntt_prime(prime, mode, n, 0, lg, d_x, nullptr, d_y, t.d_fwd, t.d_inv, t.scale_inv,
+                                       (hipStream_t)stream));
+}
+
+midint_status midint_mul_ntt_prime(midint_ctx* ctx, uint32_t prime, uint32_t n, uint32_t m, const uint64_t* d_a,
+                                   const uint64_t* d_b, uint64_t* d_r, void* stream) {
+    if (ctx && prime == MIDINT_PRIME_GOLDILOCKS) return midint_mul_ntt(ctx, n, m, d_a, d_b, d_r, stream);
+    if (!ctx || !valid_prime(prime)) return MIDINT_EINVAL;
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    const uint32_t ncoef = (64 / ntt_prime_digit_bits(prime)) * m;  // coefficients per operand
+    uint32_t lg = 6;
+    while ((1u << lg) < 2 * ncoef) lg++;
+    if (lg > kNttMaxLg) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    PrimeTables t;
+    if ((st = get_prime_tables(ctx, prime, lg, false, &t)) != MIDINT_OK) return st;
+    return hip_status(launch_ntt_prime(prime, 0, n, m, lg, d_a, d_b, d_r, t.d_fwd, t.d_inv, t.scale_prod,
+                                       (hipStream_t)stream));
 }
 
 // ----------------------------------------------------------- host drop-ins

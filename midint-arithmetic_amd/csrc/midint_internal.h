@@ -49,6 +49,22 @@ hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint6
 hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
                            hipStream_t s);
 
+// NTT over the reference's primes (p32, p64) and Goldilocks: tables in the
+// field's multiplier form (Montgomery for p32/p64), and the launcher.
+// mode 0: exact product (m limbs in, 2m out); 1: forward transform of L
+// values; 2: inverse transform.
+int ntt_prime_tables(uint32_t prime, uint32_t lg, bool ref_root, uint64_t* fwd, uint64_t* inv,
+                     uint64_t* scale_inv, uint64_t* scale_prod);
+uint32_t ntt_prime_digit_bits(uint32_t prime);
+hipError_t launch_ntt_prime(uint32_t prime, int mode, uint32_t n, uint32_t m, uint32_t lg, const uint64_t* a,
+                            const uint64_t* b, uint64_t* r, const uint64_t* twf, const uint64_t* twi,
+                            uint64_t scale, hipStream_t s);
+struct PrimeTables {
+    uint64_t* d_fwd;
+    uint64_t* d_inv;
+    uint64_t scale_inv, scale_prod;
+};
+
 // Ragged batches.
 constexpr uint32_t kRgWindow = 2048;   // output limbs per work item
 constexpr uint32_t kRgMaxLanes = 512;  // virtual lanes per multiplication item
