@@ -110,6 +110,27 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
  * on first use). */
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);
 
+/* ------------------------------------------------------------ u32 limbs */
+/* The same operations on integers of m32 u32 words (the reference's U32bits
+ * base, cuda/ker-helpers.cu.h:21-28, run by runAdditions / runNaiveMuls<
+ * U32bits> in cuda/main.cu:768-774): instance i at words [i*m32, (i+1)*m32),
+ * word 0 least significant; r has m32 words (2*m32 for mul_full), carry n
+ * bytes (optional). Pointers need 4-byte alignment. Even m32 with 8-byte
+ * aligned pointers is the u64 layout and runs in place; otherwise each
+ * instance is zero-extended to whole u64 limbs in context scratch
+ * (2..7 * n * ceil(m32/2) limbs), so calls on one context that take that
+ * path must not overlap on different streams. */
+midint_status midint_add_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                             uint32_t* d_r, uint8_t* d_carry, void* stream);
+midint_status midint_add6_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                              uint32_t* d_r, void* stream);
+midint_status midint_mul_lo_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                const uint32_t* d_b, uint32_t* d_r, void* stream);
+midint_status midint_mul_full_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                  const uint32_t* d_b, uint32_t* d_r, void* stream);
+midint_status midint_poly_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                              uint32_t* d_r, void* stream);
+
 /* ------------------------------------------------- prime-field variants */
 /* The reference's FFT primes (FftPrime32 / FftPrime64,
  * cuda/ker-fft-mul.cu.h:44-72; selected in cuda/main.cu:765-766) and

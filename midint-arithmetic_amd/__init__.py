@@ -52,6 +52,11 @@ _SIGS = {
     "midint_mul_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_conv": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_transform": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp]),
+    "midint_add_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
+    "midint_add6_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_lo_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_full_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_poly_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_ntt_prime": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_prepare": (_int, [_vp, _u32]),
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
@@ -206,6 +211,26 @@ class Context:
         n = a.numel() >> lg
         check(lib().midint_ntt_conv(self._h, n, lg, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
               "midint_ntt_conv")
+
+    # u32-limb layout: a, b, r hold n * m32 u32 words (int32 tensors or any
+    # buffer with data_ptr()); counts are in words
+    def add_u32(self, a, b, r, m32, n, carry=None, stream=None):
+        check(lib().midint_add_u32(self._h, n, m32, _ptr(a), _ptr(b), _ptr(r), _ptr(carry), _stream(stream)),
+              "midint_add_u32")
+
+    def add6_u32(self, a, b, r, m32, n, stream=None):
+        check(lib().midint_add6_u32(self._h, n, m32, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_add6_u32")
+
+    def mul_lo_u32(self, a, b, r, m32, n, stream=None):
+        check(lib().midint_mul_lo_u32(self._h, n, m32, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_mul_lo_u32")
+
+    def mul_full_u32(self, a, b, r, m32, n, stream=None):
+        check(lib().midint_mul_full_u32(self._h, n, m32, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
+              "midint_mul_full_u32")
+
+    def poly_u32(self, a, b, r, m32, n, stream=None):
+        check(lib().midint_poly_u32(self._h, n, m32, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_poly_u32")
 
     def ntt_transform(self, x, y, lg, prime=0, inverse=False, reference_root=False, stream=None):
         """Batched 2^lg-point transform over PRIME_* (see midint_ntt_transform)."""

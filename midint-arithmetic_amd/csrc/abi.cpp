@@ -238,6 +238,84 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
     return hip_status(launch_ntt_conv(n, d_a, d_b, d_r, t, (hipStream_t)stream));
 }
 
+// ------------------------------------------------------------ u32 limbs
+enum class U32Op { Add, Add6, MulLo, MulFull, Poly };
+
+static midint_status u32_op(midint_ctx* ctx, U32Op op, uint32_t n, uint32_t m32, const uint32_t* a,
+                            const uint32_t* b, uint32_t* r, uint8_t* carry, void* stream) {
+    if (!ctx || m32 == 0) return MIDINT_EINVAL;
+    if (n == 0) return MIDINT_OK;
+    if (!a || !b || !r || ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 3u)) return MIDINT_EINVAL;
+    const uint32_t m = (m32 + 1) / 2;
+    const bool is_add = op == U32Op::Add || op == U32Op::Add6;
+    if (m > (is_add ? midint_max_limbs_add() : midint_max_limbs_mul())) return MIDINT_EUNSUPPORTED;
+    const uint64_t* a64 = (const uint64_t*)a;
+    const uint64_t* b64 = (const uint64_t*)b;
+    uint64_t* r64 = (uint64_t*)r;
+    if (!(m32 & 1u) && aligned8(a) && aligned8(b) && aligned8(r)) {  // same bytes as the u64 layout
+        switch (op) {
+            case U32Op::Add: return midint_add(ctx, n, m, a64, b64, r64, carry, stream);
+            case U32Op::Add6: return midint_add6(ctx, n, m, a64, b64, r64, stream);
+            case U32Op::MulLo: return midint_mul_lo(ctx, n, m, a64, b64, r64, stream);
+            case U32Op::MulFull: return midint_mul_full(ctx, n, m, a64, b64, r64, stream);
+            case U32Op::Poly: return midint_poly(ctx, n, m, a64, b64, r64, stream);
+        }
+    }
+    midint_status st = bind_device(ctx);
+    if (st != MIDINT_OK) return st;
+    // zero-extend each instance to mp limbs, run the u64 kernel, keep the low
+    // words: every operation here commutes with truncation mod 2^(32 m32). An
+    // addition keeps one spare word above m32 for its carry.
+    const uint32_t mp = op == U32Op::Add ? m32 / 2 + 1 : m;
+    if (op == U32Op::Add && mp > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
+    const uint32_t out_limbs = op == U32Op::MulFull ? 2 * mp : mp;
+    const uint32_t out_w32 = op == U32Op::MulFull ? 2 * m32 : m32;
+    const size_t nm = (size_t)n * mp;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == U32Op::Poly ? 4 * nm : 0))) != MIDINT_OK)
+        return st;
+    uint64_t* sa = ctx->scratch;
+    uint64_t* sb = sa + nm;
+    uint64_t* sr = sb + nm;
+    uint64_t* extra = sr + (size_t)n * out_limbs;
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = launch_pack_u32(n, m32, mp, a, sa, s);
+    if (e == hipSuccess) e = launch_pack_u32(n, m32, mp, b, sb, s);
+    if (e == hipSuccess) {
+        switch (op) {
+            case U32Op::Add: e = launch_add(n, mp, sa, sb, sr, nullptr, false, s); break;
+            case U32Op::Add6: e = launch_add(n, mp, sa, sb, sr, nullptr, true, s); break;
+            case U32Op::MulLo: e = launch_mul(n, mp, sa, sb, sr, false, s); break;
+            case U32Op::MulFull: e = launch_mul(n, mp, sa, sb, sr, true, s); break;
+            case U32Op::Poly: e = launch_poly(n, mp, sa, sb, sr, extra, s); break;
+        }
+    }
+    if (e == hipSuccess)
+        e = launch_unpack_u32(n, out_w32, out_limbs, sr, r, op == U32Op::Add ? carry : nullptr, s);
+    return hip_status(e);
+}
+
+midint_status midint_add_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                             uint32_t* d_r, uint8_t* d_carry, void* stream) {
+    return u32_op(ctx, U32Op::Add, n, m32, d_a, d_b, d_r, d_carry, stream);
+}
+midint_status midint_add6_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                              uint32_t* d_r, void* stream) {
+    return u32_op(ctx, U32Op::Add6, n, m32, d_a, d_b, d_r, nullptr, stream);
+}
+midint_status midint_mul_lo_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                const uint32_t* d_b, uint32_t* d_r, void* stream) {
+    return u32_op(ctx, U32Op::MulLo, n, m32, d_a, d_b, d_r, nullptr, stream);
+}
+midint_status midint_mul_full_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                  const uint32_t* d_b, uint32_t* d_r, void* stream) {
+    return u32_op(ctx, U32Op::MulFull, n, m32, d_a, d_b, d_r, nullptr, stream);
+}
+midint_status midint_poly_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
+                              uint32_t* d_r, void* stream) {
+    return u32_op(ctx, U32Op::Poly, n, m32, d_a, d_b, d_r, nullptr, stream);
+}
+
 // ------------------------------------------------- prime-field variants
 static midint_status get_prime_tables(midint_ctx* ctx, uint32_t prime, uint32_t lg, bool ref_root,
                                       PrimeTables* out) {

@@ -49,6 +49,14 @@ hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint6
 hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
                            hipStream_t s);
 
+// u32-limb layouts: zero-extend m32 words per instance to m limbs (2m >=
+// m32), and copy the low w32 words of each ms-limb instance back (carry:
+// bit 0 of word w32, optional; needs 2 ms > w32).
+hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* src, uint64_t* dst,
+                           hipStream_t s);
+hipError_t launch_unpack_u32(uint64_t n, uint32_t w32, uint32_t ms, const uint64_t* src, uint32_t* dst,
+                             uint8_t* carry, hipStream_t s);
+
 // NTT over the reference's primes (p32, p64) and Goldilocks: tables in the
 // field's multiplier form (Montgomery for p32/p64), and the launcher.
 // mode 0: exact product (m limbs in, 2m out); 1: forward transform of L
