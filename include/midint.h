@@ -106,6 +106,12 @@ midint_status midint_mul_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint
 midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const uint64_t* d_a,
                               const uint64_t* d_b, uint64_t* d_r, void* stream);
 
+/* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance with NTT products, the
+ * composite of midint_poly. Replaces polyFttKer (cuda/ker-fft-mul.cu.h:
+ * 456-509). Uses 4*n*m limbs of context scratch; m <= midint_max_limbs_ntt(). */
+midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                              uint64_t* d_r, void* stream);
+
 /* Precomputes the NTT twiddle tables for operands of m limbs (otherwise done
  * on first use). */
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);

@@ -52,6 +52,7 @@ _SIGS = {
     "midint_mul_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_conv": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_transform": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp]),
+    "midint_poly_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_add_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
     "midint_add6_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_lo_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
@@ -206,6 +207,10 @@ class Context:
         n = a.numel() // m
         check(lib().midint_mul_ntt(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
               "midint_mul_ntt")
+
+    def poly_ntt(self, a, b, r, m, stream=None):
+        n = a.numel() // m
+        check(lib().midint_poly_ntt(self._h, n, m, _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_poly_ntt")
 
     def ntt_conv(self, a, b, r, lg, stream=None):
         n = a.numel() >> lg

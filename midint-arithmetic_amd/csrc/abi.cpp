@@ -238,6 +238,19 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
     return hip_status(launch_ntt_conv(n, d_a, d_b, d_r, t, (hipStream_t)stream));
 }
 
+midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
+                              uint64_t* d_r, void* stream) {
+    midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
+    if (st != MIDINT_OK || n == 0) return st;
+    if (m > midint_max_limbs_ntt()) return MIDINT_EUNSUPPORTED;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    NttTables t;
+    if ((st = get_tables(ctx, ntt_lg_for_limbs(m), &t)) != MIDINT_OK) return st;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
+    return hip_status(launch_poly_ntt(n, m, d_a, d_b, d_r, ctx->scratch, t, (hipStream_t)stream));
+}
+
 // ------------------------------------------------------------ u32 limbs
 enum class U32Op { Add, Add6, MulLo, MulFull, Poly };
 

@@ -76,7 +76,7 @@ constexpr uint32_t kNttMaxLimbsPerThread = 4;  // 2m / blockDim, blockDim >= m/2
 // MODE 0: integer product (r has 2m limbs per instance).
 // MODE 1: raw cyclic convolution of L values (reference bmulFFTvalid).
 template <int MODE>
-__global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t lg, const uint64_t* __restrict__ a,
+__global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t rl, uint32_t lg, const uint64_t* __restrict__ a,
                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
                                                    const uint64_t* __restrict__ twf,
                                                    const uint64_t* __restrict__ twi, uint64_t inv_len) {
@@ -178,12 +178,12 @@ __global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t lg, cons
         rows.s1[v] = false;
     }
     block_add_rows<R>(rows, lds_gp);
-    uint64_t* gr = r + (size_t)blockIdx.x * outl;
+    uint64_t* gr = r + (size_t)blockIdx.x * rl;  // rl = 2m (full) or m (low half)
 #pragma unroll
     for (int v = 0; v < R; v++) {
         const uint32_t k = k_first + v * kRowLimbs;
-        if (k < outl) gr[k] = rows.x0[v];
-        if (k + 1 < outl) gr[k + 1] = rows.x1[v];
+        if (k < rl) gr[k] = rows.x0[v];
+        if (k + 1 < rl) gr[k + 1] = rows.x1[v];
     }
 }
 
@@ -223,7 +223,8 @@ __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t
 }
 
 template <int LG, int V, int MODE>
-__global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, const uint64_t* __restrict__ a,
+__global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, uint32_t rl,
+                                                                  const uint64_t* __restrict__ a,
                                                                   const uint64_t* __restrict__ b,
                                                                   uint64_t* __restrict__ r,
                                                                   const uint64_t* __restrict__ twf,
@@ -304,17 +305,17 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, co
         rows.s1[vv] = false;
     }
     block_add_rows<R>(rows, lds_gp);
-    uint64_t* gr = r + (size_t)blockIdx.x * outl;
+    uint64_t* gr = r + (size_t)blockIdx.x * rl;  // rl = 2m (full) or m (low half)
 #pragma unroll
     for (int vv = 0; vv < R; vv++) {
         const uint32_t k = k_first + vv * kRowLimbs;
-        if (k < outl) gr[k] = rows.x0[vv];
-        if (k + 1 < outl) gr[k + 1] = rows.x1[vv];
+        if (k < rl) gr[k] = rows.x0[vv];
+        if (k + 1 < rl) gr[k + 1] = rows.x1[vv];
     }
 }
 
 template <int LG, int V, int MODE>
-static hipError_t launch_fast(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+static hipError_t launch_fast(uint32_t n, uint32_t m, uint32_t rl, const uint64_t* a, const uint64_t* b, uint64_t* r,
                               const NttTables& t, hipStream_t s) {
     constexpr uint32_t L = 1u << LG;
     static_assert(L / V >= 64 && L / V <= 1024, "one to sixteen waves per block");
@@ -322,7 +323,7 @@ static hipError_t launch_fast(uint32_t n, uint32_t m, const uint64_t* a, const u
     const void* fn = (const void*)ntt_fast_kernel<LG, V, MODE>;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return e;
-    ntt_fast_kernel<LG, V, MODE><<<n, L / V, lds, s>>>(m, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    ntt_fast_kernel<LG, V, MODE><<<n, L / V, lds, s>>>(m, rl, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
     return hipGetLastError();
 }
 
@@ -336,15 +337,15 @@ static uint32_t ntt_vals(uint32_t lg) {
 }
 
 template <int MODE>
-static hipError_t launch_fast_lg(uint32_t lg, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
-                                 uint64_t* r, const NttTables& t, hipStream_t s) {
+static hipError_t launch_fast_lg(uint32_t lg, uint32_t n, uint32_t m, uint32_t rl, const uint64_t* a,
+                                 const uint64_t* b, uint64_t* r, const NttTables& t, hipStream_t s) {
     const bool v32 = ntt_vals(lg) == 32;
     switch (lg) {
-        case 10: return launch_fast<10, 16, MODE>(n, m, a, b, r, t, s);
-        case 11: return v32 ? launch_fast<11, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<11, 16, MODE>(n, m, a, b, r, t, s);
-        case 12: return v32 ? launch_fast<12, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<12, 16, MODE>(n, m, a, b, r, t, s);
-        case 13: return v32 ? launch_fast<13, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<13, 16, MODE>(n, m, a, b, r, t, s);
-        case 14: return v32 ? launch_fast<14, 32, MODE>(n, m, a, b, r, t, s) : launch_fast<14, 16, MODE>(n, m, a, b, r, t, s);
+        case 10: return launch_fast<10, 16, MODE>(n, m, rl, a, b, r, t, s);
+        case 11: return v32 ? launch_fast<11, 32, MODE>(n, m, rl, a, b, r, t, s) : launch_fast<11, 16, MODE>(n, m, rl, a, b, r, t, s);
+        case 12: return v32 ? launch_fast<12, 32, MODE>(n, m, rl, a, b, r, t, s) : launch_fast<12, 16, MODE>(n, m, rl, a, b, r, t, s);
+        case 13: return v32 ? launch_fast<13, 32, MODE>(n, m, rl, a, b, r, t, s) : launch_fast<13, 16, MODE>(n, m, rl, a, b, r, t, s);
+        case 14: return v32 ? launch_fast<14, 32, MODE>(n, m, rl, a, b, r, t, s) : launch_fast<14, 16, MODE>(n, m, rl, a, b, r, t, s);
     }
     return hipErrorInvalidValue;
 }
@@ -395,17 +396,18 @@ static uint32_t ntt_threads(uint32_t lg) {
 }
 
 hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                          const NttTables& t, hipStream_t s) {
+                          const NttTables& t, hipStream_t s, bool lo) {
+    const uint32_t rl = lo ? m : 2 * m;
     if (t.lg > kNttMaxLg || t.lg < 6) return hipErrorInvalidValue;
     const uint32_t T = ntt_threads(t.lg);
     const size_t lds = ((size_t)1 << t.lg) * 8;
     if ((1u << t.lg) / T > kNttMaxPerThread) return hipErrorInvalidValue;
     if (2ull * m > (uint64_t)kNttMaxLimbsPerThread * T) return hipErrorInvalidValue;
-    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<0>(t.lg, n, m, a, b, r, t, s);
+    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<0>(t.lg, n, m, rl, a, b, r, t, s);
     hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
-    ntt_kernel<0><<<n, T, lds, s>>>(m, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    ntt_kernel<0><<<n, T, lds, s>>>(m, rl, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
     return hipGetLastError();
 }
 
@@ -414,12 +416,32 @@ hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uin
     if (t.lg > kNttMaxLg || t.lg < 6) return hipErrorInvalidValue;
     const uint32_t T = ntt_threads(t.lg);
     const size_t lds = ((size_t)1 << t.lg) * 8;
-    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<1>(t.lg, n, 0, a, b, r, t, s);
+    if (t.lg >= 10 && tune_get("ntt.fast", 1)) return launch_fast_lg<1>(t.lg, n, 0, 0, a, b, r, t, s);
     hipError_t e = hipFuncSetAttribute((const void*)ntt_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                        (int)lds);
     if (e != hipSuccess) return e;
-    ntt_kernel<1><<<n, T, lds, s>>>(0, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    ntt_kernel<1><<<n, T, lds, s>>>(0, 0, t.lg, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
     return hipGetLastError();
+}
+
+// r = ((a^2 + b)(b^2 + b) + ab) mod B^m with NTT products (reference
+// polyFttKer, cuda/ker-fft-mul.cu.h:456-509: the same sequence of three
+// additions and four products as polyKerQ, every step mod B^m).
+hipError_t launch_poly_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                           uint64_t* scratch, const NttTables& t, hipStream_t s) {
+    const size_t nm = (size_t)n * m;
+    uint64_t* t1 = scratch;
+    uint64_t* t2 = scratch + nm;
+    uint64_t* t3 = scratch + 2 * nm;
+    uint64_t* t4 = scratch + 3 * nm;
+    hipError_t e;
+    if ((e = launch_mul_ntt(n, m, a, a, t1, t, s, true)) != hipSuccess) return e;       // t1 = a^2
+    if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s)) != hipSuccess) return e;  // t2 = a^2 + b
+    if ((e = launch_mul_ntt(n, m, b, b, t3, t, s, true)) != hipSuccess) return e;      // t3 = b^2
+    if ((e = launch_add(n, m, t3, b, t4, nullptr, false, s)) != hipSuccess) return e;  // t4 = b^2 + b
+    if ((e = launch_mul_ntt(n, m, t2, t4, t1, t, s, true)) != hipSuccess) return e;    // t5 = t2 * t4
+    if ((e = launch_mul_ntt(n, m, a, b, t3, t, s, true)) != hipSuccess) return e;      // t6 = a * b
+    return launch_add(n, m, t1, t3, r, nullptr, false, s);                              // r = t5 + t6
 }
 
 }  // namespace midint

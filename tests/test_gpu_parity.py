@@ -149,6 +149,30 @@ def test_poly(ctx, torch_dev, orc, m):
     assert np.array_equal(host(dr), orc.poly(a, b, m))
 
 
+@pytest.mark.parametrize("m", [1, 3, 64, 100, 1024, 2048])
+def test_poly_ntt(ctx, torch_dev, orc, m):
+    """NTT composite (polyFttKer): same closed form as the classical one."""
+    torch = torch_dev
+    n = 3 if m >= 1024 else 9
+    a, b = operands(m * 11 + 5, n, m)
+    dr = empty(torch, n * m)
+    ctx.poly_ntt(dev(torch, a), dev(torch, b), dr, m)
+    torch.cuda.synchronize()
+    want = orc.poly(a, b, m)
+    assert np.array_equal(host(dr), want)
+    if m <= midint_max_mul(ctx):
+        dc = empty(torch, n * m)
+        ctx.poly(dev(torch, a), dev(torch, b), dc, m)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dc), host(dr))
+
+
+def midint_max_mul(ctx):
+    from __graft_entry__ import load_package
+
+    return load_package().lib().midint_max_limbs_mul()
+
+
 NTT_SIZES = [1, 2, 3, 8, 16, 64, 100, 128, 256, 1000, 1024, 2048]
 
 
