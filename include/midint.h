@@ -116,6 +116,40 @@ midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
  * on first use). */
 midint_status midint_ntt_prepare(midint_ctx* ctx, uint32_t m);
 
+/* ------------------------------------------------------ strided layouts */
+/* The same operations on instances placed at arbitrary limb strides: a's
+ * instance i starts at d_a + i*sa, b's at d_b + i*sb, r's at d_r + i*sr
+ * (0 selects the batch-major stride: m, or 2m for mul_full). Strides must be
+ * at least the instance length. This runs directly on interleaved
+ * array-of-structs layouts (see midint_cgbn) without a re-layout pass.
+ * The Futhark entry layout [m][ipb][4n]u64 (futhark-clean/big-add.fut:
+ * 124-128, sqr-mul.fut) flattened row-major is the batch-major layout itself
+ * (m*ipb instances of 4n limbs), so it needs no stride. */
+midint_status midint_add_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                 const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, uint8_t* d_carry,
+                                 void* stream);
+midint_status midint_add6_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                  const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream);
+midint_status midint_mul_lo_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                    const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream);
+midint_status midint_mul_full_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                      const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr,
+                                      void* stream);
+/* Uses 4*n*m limbs of context scratch for its temporaries. */
+midint_status midint_poly_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                  const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream);
+
+/* CGBN's instance_t array (cuda/cgbn-tests/cgbn-kers.cu.h:5-9): per
+ * instance three fields of bits/32 u32 words, {a, b, sum}. The op writes
+ * sum = (a + b), (6a + 10b), (a * b) or ((a^2 + b)(b^2 + b) + ab), each mod
+ * 2^bits, like kernel_add / kernel_6adds / kernel_mul / kernel_poly
+ * (:19-110). bits must be a multiple of 32; a multiple of 64 with an
+ * 8-byte aligned array runs in place on the strided u64 path, other sizes
+ * go through the u32 repack of the next section. */
+enum { MIDINT_CGBN_ADD = 0, MIDINT_CGBN_ADD6 = 1, MIDINT_CGBN_MUL = 2, MIDINT_CGBN_POLY = 3 };
+midint_status midint_cgbn(midint_ctx* ctx, uint32_t op, uint32_t n, uint32_t bits, void* d_instances,
+                          void* stream);
+
 /* ------------------------------------------------------------ u32 limbs */
 /* The same operations on integers of m32 u32 words (the reference's U32bits
  * base, cuda/ker-helpers.cu.h:21-28, run by runAdditions / runNaiveMuls<

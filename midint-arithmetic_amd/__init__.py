@@ -32,6 +32,7 @@ _u64p = ctypes.POINTER(ctypes.c_uint64)
 _u32p = ctypes.POINTER(ctypes.c_uint32)
 _vp = ctypes.c_void_p
 _u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
 _int = ctypes.c_int
 _dblp = ctypes.POINTER(ctypes.c_double)
 
@@ -53,6 +54,12 @@ _SIGS = {
     "midint_ntt_conv": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_transform": (_int, [_vp, _u32, _u32, _u32, _u32, _vp, _vp, _vp]),
     "midint_poly_ntt": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_add_strided": (_int, [_vp, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _u64, _vp, _vp]),
+    "midint_add6_strided": (_int, [_vp, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _u64, _vp]),
+    "midint_mul_lo_strided": (_int, [_vp, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _u64, _vp]),
+    "midint_mul_full_strided": (_int, [_vp, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _u64, _vp]),
+    "midint_poly_strided": (_int, [_vp, _u32, _u32, _vp, _u64, _vp, _u64, _vp, _u64, _vp]),
+    "midint_cgbn": (_int, [_vp, _u32, _u32, _u32, _vp, _vp]),
     "midint_add_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp, _vp]),
     "midint_add6_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_lo_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
@@ -157,6 +164,7 @@ def _stream(stream):
 PRIME_GOLDILOCKS, PRIME_P32, PRIME_P64 = 0, 1, 2
 PRIMES = {PRIME_GOLDILOCKS: 0xFFFFFFFF00000001, PRIME_P32: 3221225473, PRIME_P64: 4179340454199820289}
 NTT_INVERSE, NTT_REFERENCE_ROOT = 1, 2
+CGBN_ADD, CGBN_ADD6, CGBN_MUL, CGBN_POLY = 0, 1, 2, 3
 
 
 class Context:
@@ -216,6 +224,21 @@ class Context:
         n = a.numel() >> lg
         check(lib().midint_ntt_conv(self._h, n, lg, _ptr(a), _ptr(b), _ptr(r), _stream(stream)),
               "midint_ntt_conv")
+
+    # strided layouts: instance i of a at a + i*sa limbs (0: batch-major)
+    def strided(self, op, n, m, a, sa, b, sb, r, sr, carry=None, stream=None):
+        """op in 'add', 'add6', 'mul_lo', 'mul_full', 'poly'; a, b, r are
+        device buffers (anything with data_ptr()) already offset to
+        instance 0 of each operand."""
+        fn = getattr(lib(), f"midint_{op}_strided")
+        args = [self._h, n, m, _ptr(a), sa, _ptr(b), sb, _ptr(r), sr]
+        if op == "add":
+            args.append(_ptr(carry))
+        check(fn(*args, _stream(stream)), f"midint_{op}_strided")
+
+    def cgbn(self, op, instances, n, bits, stream=None):
+        """CGBN instance_t array (a, b, sum per instance); op in CGBN_*."""
+        check(lib().midint_cgbn(self._h, op, n, bits, _ptr(instances), _stream(stream)), "midint_cgbn")
 
     # u32-limb layout: a, b, r hold n * m32 u32 words (int32 tensors or any
     # buffer with data_ptr()); counts are in words

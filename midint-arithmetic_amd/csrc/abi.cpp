@@ -251,82 +251,152 @@ midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
     return hip_status(launch_poly_ntt(n, m, d_a, d_b, d_r, ctx->scratch, t, (hipStream_t)stream));
 }
 
-// ------------------------------------------------------------ u32 limbs
-enum class U32Op { Add, Add6, MulLo, MulFull, Poly };
+// ------------------------------------------------- strided u64 layouts
+enum class Op { Add, Add6, MulLo, MulFull, Poly };
 
-static midint_status u32_op(midint_ctx* ctx, U32Op op, uint32_t n, uint32_t m32, const uint32_t* a,
-                            const uint32_t* b, uint32_t* r, uint8_t* carry, void* stream) {
+static uint32_t op_out_limbs(Op op, uint32_t m) { return op == Op::MulFull ? 2 * m : m; }
+
+// u64 operation on instances at limb strides sa, sb, sr (0 = batch-major).
+static midint_status u64_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m, const uint64_t* a, uint64_t sa,
+                            const uint64_t* b, uint64_t sb, uint64_t* r, uint64_t sr, uint8_t* carry,
+                            void* stream) {
+    midint_status st = check_binary(ctx, n, m, a, b, r);
+    if (st != MIDINT_OK || n == 0) return st;
+    const bool is_add = op == Op::Add || op == Op::Add6;
+    if (m > (is_add ? midint_max_limbs_add() : midint_max_limbs_mul())) return MIDINT_EUNSUPPORTED;
+    if ((sa && sa < m) || (sb && sb < m) || (sr && sr < op_out_limbs(op, m))) return MIDINT_EINVAL;
+    // the batch-major strides keep the 16-byte vector path
+    if (sa == m) sa = 0;
+    if (sb == m) sb = 0;
+    if (sr == op_out_limbs(op, m)) sr = 0;
+    if ((st = bind_device(ctx)) != MIDINT_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    const Strides str{sa, sb, sr};
+    switch (op) {
+        case Op::Add: return hip_status(launch_add(n, m, a, b, r, carry, false, s, str));
+        case Op::Add6: return hip_status(launch_add(n, m, a, b, r, nullptr, true, s, str));
+        case Op::MulLo: return hip_status(launch_mul(n, m, a, b, r, false, s, str));
+        case Op::MulFull: return hip_status(launch_mul(n, m, a, b, r, true, s, str));
+        case Op::Poly: {
+            std::lock_guard<std::mutex> lk(ctx->mu);
+            if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
+            return hip_status(launch_poly(n, m, a, b, r, ctx->scratch, s, str));
+        }
+    }
+    return MIDINT_EINVAL;
+}
+
+midint_status midint_add_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                 const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, uint8_t* d_carry,
+                                 void* stream) {
+    return u64_op(ctx, Op::Add, n, m, d_a, sa, d_b, sb, d_r, sr, d_carry, stream);
+}
+midint_status midint_add6_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                  const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream) {
+    return u64_op(ctx, Op::Add6, n, m, d_a, sa, d_b, sb, d_r, sr, nullptr, stream);
+}
+midint_status midint_mul_lo_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                    const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream) {
+    return u64_op(ctx, Op::MulLo, n, m, d_a, sa, d_b, sb, d_r, sr, nullptr, stream);
+}
+midint_status midint_mul_full_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                      const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr,
+                                      void* stream) {
+    return u64_op(ctx, Op::MulFull, n, m, d_a, sa, d_b, sb, d_r, sr, nullptr, stream);
+}
+midint_status midint_poly_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
+                                  const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream) {
+    return u64_op(ctx, Op::Poly, n, m, d_a, sa, d_b, sb, d_r, sr, nullptr, stream);
+}
+
+// ------------------------------------------------------------ u32 limbs
+// u32 operation on instances at word strides sa, sb, sr (0 = batch-major).
+static midint_status u32_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m32, const uint32_t* a, uint64_t sa,
+                            const uint32_t* b, uint64_t sb, uint32_t* r, uint64_t sr, uint8_t* carry,
+                            void* stream) {
     if (!ctx || m32 == 0) return MIDINT_EINVAL;
     if (n == 0) return MIDINT_OK;
     if (!a || !b || !r || ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 3u)) return MIDINT_EINVAL;
     const uint32_t m = (m32 + 1) / 2;
-    const bool is_add = op == U32Op::Add || op == U32Op::Add6;
+    const bool is_add = op == Op::Add || op == Op::Add6;
     if (m > (is_add ? midint_max_limbs_add() : midint_max_limbs_mul())) return MIDINT_EUNSUPPORTED;
-    const uint64_t* a64 = (const uint64_t*)a;
-    const uint64_t* b64 = (const uint64_t*)b;
-    uint64_t* r64 = (uint64_t*)r;
-    if (!(m32 & 1u) && aligned8(a) && aligned8(b) && aligned8(r)) {  // same bytes as the u64 layout
-        switch (op) {
-            case U32Op::Add: return midint_add(ctx, n, m, a64, b64, r64, carry, stream);
-            case U32Op::Add6: return midint_add6(ctx, n, m, a64, b64, r64, stream);
-            case U32Op::MulLo: return midint_mul_lo(ctx, n, m, a64, b64, r64, stream);
-            case U32Op::MulFull: return midint_mul_full(ctx, n, m, a64, b64, r64, stream);
-            case U32Op::Poly: return midint_poly(ctx, n, m, a64, b64, r64, stream);
-        }
-    }
+    const uint32_t out_w32 = op == Op::MulFull ? 2 * m32 : m32;
+    sa = sa ? sa : m32;
+    sb = sb ? sb : m32;
+    sr = sr ? sr : out_w32;
+    if (sa < m32 || sb < m32 || sr < out_w32) return MIDINT_EINVAL;
+    if (!(m32 & 1u) && !((sa | sb | sr) & 1u) && aligned8(a) && aligned8(b) && aligned8(r))
+        // same bytes as the u64 layout at half the strides
+        return u64_op(ctx, op, n, m, (const uint64_t*)a, sa / 2, (const uint64_t*)b, sb / 2, (uint64_t*)r, sr / 2,
+                      carry, stream);
     midint_status st = bind_device(ctx);
     if (st != MIDINT_OK) return st;
-    // zero-extend each instance to mp limbs, run the u64 kernel, keep the low
-    // words: every operation here commutes with truncation mod 2^(32 m32). An
-    // addition keeps one spare word above m32 for its carry.
-    const uint32_t mp = op == U32Op::Add ? m32 / 2 + 1 : m;
-    if (op == U32Op::Add && mp > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
-    const uint32_t out_limbs = op == U32Op::MulFull ? 2 * mp : mp;
-    const uint32_t out_w32 = op == U32Op::MulFull ? 2 * m32 : m32;
+    // zero-extend each instance to mp batch-major limbs, run the u64 kernel,
+    // keep the low words: every operation here commutes with truncation mod
+    // 2^(32 m32). An addition keeps one spare word above m32 for its carry.
+    const uint32_t mp = op == Op::Add ? m32 / 2 + 1 : m;
+    if (op == Op::Add && mp > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
+    const uint32_t out_limbs = op_out_limbs(op, mp);
     const size_t nm = (size_t)n * mp;
     std::lock_guard<std::mutex> lk(ctx->mu);
-    if ((st = ensure_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == U32Op::Poly ? 4 * nm : 0))) != MIDINT_OK)
+    if ((st = ensure_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == Op::Poly ? 4 * nm : 0))) != MIDINT_OK)
         return st;
-    uint64_t* sa = ctx->scratch;
-    uint64_t* sb = sa + nm;
-    uint64_t* sr = sb + nm;
-    uint64_t* extra = sr + (size_t)n * out_limbs;
+    uint64_t* pa = ctx->scratch;
+    uint64_t* pb = pa + nm;
+    uint64_t* pr = pb + nm;
+    uint64_t* extra = pr + (size_t)n * out_limbs;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = launch_pack_u32(n, m32, mp, a, sa, s);
-    if (e == hipSuccess) e = launch_pack_u32(n, m32, mp, b, sb, s);
+    hipError_t e = launch_pack_u32(n, m32, mp, a, sa, pa, s);
+    if (e == hipSuccess) e = launch_pack_u32(n, m32, mp, b, sb, pb, s);
     if (e == hipSuccess) {
         switch (op) {
-            case U32Op::Add: e = launch_add(n, mp, sa, sb, sr, nullptr, false, s); break;
-            case U32Op::Add6: e = launch_add(n, mp, sa, sb, sr, nullptr, true, s); break;
-            case U32Op::MulLo: e = launch_mul(n, mp, sa, sb, sr, false, s); break;
-            case U32Op::MulFull: e = launch_mul(n, mp, sa, sb, sr, true, s); break;
-            case U32Op::Poly: e = launch_poly(n, mp, sa, sb, sr, extra, s); break;
+            case Op::Add: e = launch_add(n, mp, pa, pb, pr, nullptr, false, s); break;
+            case Op::Add6: e = launch_add(n, mp, pa, pb, pr, nullptr, true, s); break;
+            case Op::MulLo: e = launch_mul(n, mp, pa, pb, pr, false, s); break;
+            case Op::MulFull: e = launch_mul(n, mp, pa, pb, pr, true, s); break;
+            case Op::Poly: e = launch_poly(n, mp, pa, pb, pr, extra, s); break;
         }
     }
-    if (e == hipSuccess)
-        e = launch_unpack_u32(n, out_w32, out_limbs, sr, r, op == U32Op::Add ? carry : nullptr, s);
+    if (e == hipSuccess) e = launch_unpack_u32(n, out_w32, out_limbs, pr, r, sr, op == Op::Add ? carry : nullptr, s);
     return hip_status(e);
 }
 
 midint_status midint_add_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                              uint32_t* d_r, uint8_t* d_carry, void* stream) {
-    return u32_op(ctx, U32Op::Add, n, m32, d_a, d_b, d_r, d_carry, stream);
+    return u32_op(ctx, Op::Add, n, m32, d_a, 0, d_b, 0, d_r, 0, d_carry, stream);
 }
 midint_status midint_add6_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                               uint32_t* d_r, void* stream) {
-    return u32_op(ctx, U32Op::Add6, n, m32, d_a, d_b, d_r, nullptr, stream);
+    return u32_op(ctx, Op::Add6, n, m32, d_a, 0, d_b, 0, d_r, 0, nullptr, stream);
 }
 midint_status midint_mul_lo_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
                                 const uint32_t* d_b, uint32_t* d_r, void* stream) {
-    return u32_op(ctx, U32Op::MulLo, n, m32, d_a, d_b, d_r, nullptr, stream);
+    return u32_op(ctx, Op::MulLo, n, m32, d_a, 0, d_b, 0, d_r, 0, nullptr, stream);
 }
 midint_status midint_mul_full_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
                                   const uint32_t* d_b, uint32_t* d_r, void* stream) {
-    return u32_op(ctx, U32Op::MulFull, n, m32, d_a, d_b, d_r, nullptr, stream);
+    return u32_op(ctx, Op::MulFull, n, m32, d_a, 0, d_b, 0, d_r, 0, nullptr, stream);
 }
 midint_status midint_poly_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                               uint32_t* d_r, void* stream) {
-    return u32_op(ctx, U32Op::Poly, n, m32, d_a, d_b, d_r, nullptr, stream);
+    return u32_op(ctx, Op::Poly, n, m32, d_a, 0, d_b, 0, d_r, 0, nullptr, stream);
+}
+
+// ------------------------------------------------------------ CGBN layout
+midint_status midint_cgbn(midint_ctx* ctx, uint32_t op, uint32_t n, uint32_t bits, void* d_instances,
+                          void* stream) {
+    if (!ctx || bits == 0 || bits % 32 != 0) return MIDINT_EINVAL;
+    Op o;
+    switch (op) {
+        case MIDINT_CGBN_ADD: o = Op::Add; break;
+        case MIDINT_CGBN_ADD6: o = Op::Add6; break;
+        case MIDINT_CGBN_MUL: o = Op::MulLo; break;
+        case MIDINT_CGBN_POLY: o = Op::Poly; break;
+        default: return MIDINT_EINVAL;
+    }
+    const uint32_t w = bits / 32;  // words per field; instance = {a, b, sum}
+    uint32_t* base = (uint32_t*)d_instances;
+    return u32_op(ctx, o, n, w, base, 3ull * w, base + w, 3ull * w, base + 2 * w, 3ull * w, nullptr, stream);
 }
 
 // ------------------------------------------------- prime-field variants

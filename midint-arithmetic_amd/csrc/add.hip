@@ -52,20 +52,46 @@ __device__ __forceinline__ void store_pair(uint64_t* __restrict__ p, uint32_t k,
     }
 }
 
+// Limb k of a strided window, given its (instance, offset) position.
+template <bool NT>
+__device__ __forceinline__ uint64_t load_strided(const uint64_t* __restrict__ p, uint32_t k, uint32_t L,
+                                                 uint32_t inst, uint32_t off, uint64_t stride) {
+    if (k >= L) return 0ull;
+    const uint64_t* q = p + (uint64_t)inst * stride + off;
+    return NT ? __builtin_nontemporal_load(q) : *q;
+}
+
+// Position of the limb after (inst, off) in instances of m limbs.
+__device__ __forceinline__ void next_pos(uint32_t inst, uint32_t off, uint32_t m, uint32_t& inst1, uint32_t& off1) {
+    const bool wrap = off + 1 == m;
+    off1 = wrap ? 0u : off + 1;
+    inst1 = inst + (wrap ? 1u : 0u);
+}
+
 // Loads both operands of the block window into rows and marks segment starts.
-template <int R, bool VEC, bool NT = false>
+// STRIDED: instance i of a (b) starts at limb i * sa (i * sb) instead of i * m.
+template <int R, bool VEC, bool NT = false, bool STRIDED = false>
 __device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restrict__ a,
                                             const uint64_t* __restrict__ b, uint32_t L, uint32_t m,
                                             uint32_t step_q, uint32_t step_r, uint32_t* off0,
-                                            uint32_t* inst0) {
+                                            uint32_t* inst0, uint64_t sa = 0, uint64_t sb = 0) {
     const uint32_t wave = threadIdx.x / kWave;
     const uint32_t k_first = wave * R * kRowLimbs + 2 * lane_id();
     SegPos sp = segpos_init(k_first, m);
 #pragma unroll
     for (int v = 0; v < R; v++) {
         const uint32_t k = k_first + v * kRowLimbs;
-        load_pair<VEC, NT>(a, k, L, t.x0[v], t.x1[v]);
-        load_pair<VEC, NT>(b, k, L, t.y0[v], t.y1[v]);
+        if (STRIDED) {
+            uint32_t i1, o1;
+            next_pos(sp.inst, sp.off, m, i1, o1);
+            t.x0[v] = load_strided<NT>(a, k, L, sp.inst, sp.off, sa);
+            t.x1[v] = load_strided<NT>(a, k + 1, L, i1, o1, sa);
+            t.y0[v] = load_strided<NT>(b, k, L, sp.inst, sp.off, sb);
+            t.y1[v] = load_strided<NT>(b, k + 1, L, i1, o1, sb);
+        } else {
+            load_pair<VEC, NT>(a, k, L, t.x0[v], t.x1[v]);
+            load_pair<VEC, NT>(b, k, L, t.y0[v], t.y1[v]);
+        }
         t.s0[v] = (sp.off == 0) && (k < L);
         t.s1[v] = (sp.off + 1 == m) && (k + 1 < L);
         off0[v] = sp.off;
@@ -74,11 +100,31 @@ __device__ __forceinline__ void load_window(Rows<R>& t, const uint64_t* __restri
     }
 }
 
-template <int R, bool VEC, bool NT>
+// Stores the rows of a window (strided or batch-major).
+template <int R, bool VEC, bool NT, bool STRIDED>
+__device__ __forceinline__ void store_window(uint64_t* __restrict__ r, const Rows<R>& t, uint32_t L, uint32_t m,
+                                             const uint32_t* off0, const uint32_t* inst0, uint64_t sr) {
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        if (STRIDED) {
+            uint32_t i1, o1;
+            next_pos(inst0[v], off0[v], m, i1, o1);
+            if (k < L) r[(uint64_t)inst0[v] * sr + off0[v]] = t.x0[v];
+            if (k + 1 < L) r[(uint64_t)i1 * sr + o1] = t.x1[v];
+        } else {
+            store_pair<VEC, NT>(r, k, L, t.x0[v], t.x1[v]);
+        }
+    }
+}
+
+template <int R, bool VEC, bool NT, bool STRIDED>
 __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
                                                    uint32_t step_r, const uint64_t* __restrict__ a,
                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
-                                                   uint8_t* __restrict__ carry) {
+                                                   uint8_t* __restrict__ carry, uint64_t sa, uint64_t sb,
+                                                   uint64_t sr) {
     __shared__ uint32_t lds_gp[16];
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
@@ -87,7 +133,11 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
 
     Rows<R> t;
     uint32_t off0[R], inst0[R];
-    load_window<R, VEC, NT>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+    if (STRIDED)
+        load_window<R, false, NT, true>(t, a + inst_base * sa, b + inst_base * sb, L, m, step_q, step_r, off0,
+                                        inst0, sa, sb);
+    else
+        load_window<R, VEC, NT>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
 
     uint64_t ax0[R], ax1[R];
     if (carry) {
@@ -101,8 +151,7 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
     block_add_rows<R>(t, lds_gp);
 
     const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
-#pragma unroll
-    for (int v = 0; v < R; v++) store_pair<VEC, NT>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
+    store_window<R, VEC, NT, STRIDED>(STRIDED ? r + inst_base * sr : r + base, t, L, m, off0, inst0, sr);
 
     if (carry) {
 #pragma unroll
@@ -122,10 +171,11 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
     }
 }
 
-template <int R, bool VEC>
+template <int R, bool VEC, bool STRIDED>
 __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
                                                     uint32_t step_r, const uint64_t* __restrict__ a,
-                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r) {
+                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                    uint64_t sa, uint64_t sb, uint64_t sr) {
     __shared__ uint32_t lds_gp[16];
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
@@ -134,7 +184,11 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
 
     Rows<R> t;
     uint32_t off0[R], inst0[R];
-    load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+    if (STRIDED)
+        load_window<R, false, false, true>(t, a + inst_base * sa, b + inst_base * sb, L, m, step_q, step_r, off0,
+                                           inst0, sa, sb);
+    else
+        load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
     uint64_t b0[R], b1[R];
 #pragma unroll
     for (int v = 0; v < R; v++) {
@@ -181,10 +235,7 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
         t.y1[v] = b1[v];
     }
     block_add_rows<R>(t, lds_gp);
-
-    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
-#pragma unroll
-    for (int v = 0; v < R; v++) store_pair<VEC>(r + base, k_first + v * kRowLimbs, L, t.x0[v], t.x1[v]);
+    store_window<R, VEC, false, STRIDED>(STRIDED ? r + inst_base * sr : r + base, t, L, m, off0, inst0, sr);
 }
 
 // ---------------------------------------------------------------------------
@@ -227,38 +278,48 @@ AddGeom add_geometry(uint32_t m) {
 
 template <int R>
 static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n, uint32_t m, const uint64_t* a,
-                               const uint64_t* b, uint64_t* r, uint8_t* carry, hipStream_t s) {
+                               const uint64_t* b, uint64_t* r, uint8_t* carry, const Strides& st, hipStream_t s) {
     const bool nt = tune_get_class("add.nt", m, 1) != 0;
+    const bool strided = st.a || st.b || st.r;
+    const uint64_t sa = st.a ? st.a : m, sb = st.b ? st.b : m, sr_ = st.r ? st.r : m;
     const uint32_t blocks = (uint32_t)(((uint64_t)n + g.ipb - 1) / g.ipb);
     const uint32_t sq = kRowLimbs / m, sr = kRowLimbs % m;
     if (six) {
-        if (vec)
-            add6_kernel<R, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r);
-        else
-            add6_kernel<R, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r);
-    } else {
-        if (vec && nt)
-            add_kernel<R, true, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+        if (strided)
+            add6_kernel<R, false, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
         else if (vec)
-            add_kernel<R, true, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+            add6_kernel<R, true, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
         else
-            add_kernel<R, false, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry);
+            add6_kernel<R, false, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
+    } else {
+        if (strided)
+            add_kernel<R, false, true, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry, sa,
+                                                                          sb, sr_);
+        else if (vec && nt)
+            add_kernel<R, true, true, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry, sa,
+                                                                          sb, sr_);
+        else if (vec)
+            add_kernel<R, true, false, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry, sa,
+                                                                           sb, sr_);
+        else
+            add_kernel<R, false, false, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, carry,
+                                                                            sa, sb, sr_);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, uint8_t* carry,
-                      bool six, hipStream_t s) {
+                      bool six, hipStream_t s, Strides st) {
     AddGeom g = add_geometry(m);
     if (g.ipb == 0) return hipErrorInvalidValue;
     const bool aligned = ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 15u) == 0;
     const bool vec = aligned && ((uint64_t)g.ipb * m) % 2 == 0;
     switch (g.rows) {
-        case 1: return launch_add_r<1>(g, vec, six, n, m, a, b, r, carry, s);
-        case 2: return launch_add_r<2>(g, vec, six, n, m, a, b, r, carry, s);
-        case 4: return launch_add_r<4>(g, vec, six, n, m, a, b, r, carry, s);
-        case 8: return launch_add_r<8>(g, vec, six, n, m, a, b, r, carry, s);
-        case 16: return launch_add_r<16>(g, vec, six, n, m, a, b, r, carry, s);
+        case 1: return launch_add_r<1>(g, vec, six, n, m, a, b, r, carry, st, s);
+        case 2: return launch_add_r<2>(g, vec, six, n, m, a, b, r, carry, st, s);
+        case 4: return launch_add_r<4>(g, vec, six, n, m, a, b, r, carry, st, s);
+        case 8: return launch_add_r<8>(g, vec, six, n, m, a, b, r, carry, st, s);
+        case 16: return launch_add_r<16>(g, vec, six, n, m, a, b, r, carry, st, s);
     }
     return hipErrorInvalidValue;
 }

@@ -19,18 +19,24 @@ struct AddGeom {
 };
 AddGeom add_geometry(uint32_t m);
 
+// Instance strides (limbs) of the operands and the result; 0 = the
+// batch-major stride (m, or the output length).
+struct Strides {
+    uint64_t a = 0, b = 0, r = 0;
+};
+
 // r = (a + b) mod B^m (six == false) or (6a + 10b) mod B^m (six == true).
 hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                      uint8_t* carry, bool six, hipStream_t s);
+                      uint8_t* carry, bool six, hipStream_t s, Strides st = {});
 
 // Largest operand size (limbs) of the classical multiplication kernels.
 constexpr uint32_t kMulMaxLimbs = 4096;
 // full: r[0..2m) = a*b ; otherwise r[0..m) = a*b mod B^m.
 hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
-                      hipStream_t s);
+                      hipStream_t s, Strides st = {});
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ).
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                       uint64_t* scratch, hipStream_t s);
+                       uint64_t* scratch, hipStream_t s, Strides st = {});
 
 // NTT multiplication.
 struct NttTables {
@@ -53,13 +59,14 @@ hipError_t launch_poly_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint
 hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
                            hipStream_t s);
 
-// u32-limb layouts: zero-extend m32 words per instance to m limbs (2m >=
-// m32), and copy the low w32 words of each ms-limb instance back (carry:
+// u32-limb layouts: zero-extend m32 words per instance (instance stride ss
+// words) to m batch-major limbs (2m >= m32), and copy the low w32 words of
+// each ms-limb instance back to a u32 array of instance stride ds (carry:
 // bit 0 of word w32, optional; needs 2 ms > w32).
-hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* src, uint64_t* dst,
+hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* src, uint64_t ss, uint64_t* dst,
                            hipStream_t s);
 hipError_t launch_unpack_u32(uint64_t n, uint32_t w32, uint32_t ms, const uint64_t* src, uint32_t* dst,
-                             uint8_t* carry, hipStream_t s);
+                             uint64_t ds, uint8_t* carry, hipStream_t s);
 
 // NTT over the reference's primes (p32, p64) and Goldilocks: tables in the
 // field's multiplier form (Montgomery for p32/p64), and the launcher.

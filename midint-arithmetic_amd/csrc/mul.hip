@@ -45,7 +45,10 @@ template <bool FULL, int Q, bool PF>
 __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
                                                    uint32_t P, uint32_t words, uint32_t ipb, uint32_t out_len,
                                                    int32_t m_shift, const uint64_t* __restrict__ a,
-                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ r) {
+                                                   const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                   uint64_t sa, uint64_t sb, uint64_t sr) {
+    // sa, sb, sr: instance strides (limbs) of a, b and r; m, m, out_len for
+    // the batch-major layout
     extern __shared__ __align__(16) uint64_t lds64[];
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
     __shared__ uint32_t lds_gp[16];
@@ -75,12 +78,12 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     }
     {
         const uint32_t total = ninst * m;
-        const uint64_t* ga = a + inst_base * m;
-        const uint64_t* gb = b + inst_base * m;
+        const uint64_t* ga = a + inst_base * sa;
+        const uint64_t* gb = b + inst_base * sb;
         for (uint32_t f = tid; f < total; f += blockDim.x) {
             const uint32_t s = m_shift >= 0 ? (f >> m_shift) : f / m;
             const uint32_t i = f - s * m;
-            const uint64_t va = ga[f], vb = gb[f];
+            const uint64_t va = ga[s * sa + i], vb = gb[s * sb + i];
             uint32_t* A = lds + s * words;
             uint32_t* Bw = A + n_pad + (Q - 1);
             A[2 * i] = (uint32_t)va;
@@ -156,11 +159,11 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
         const uint32_t k = k_first + v * kRowLimbs;
         const uint32_t o0 = off0[v];
         const uint32_t i0 = inst0[v];
-        if (k < Lw && i0 < ninst && o0 < out_len) r[(inst_base + i0) * out_len + o0] = rows.x0[v];
+        if (k < Lw && i0 < ninst && o0 < out_len) r[(inst_base + i0) * sr + o0] = rows.x0[v];
         const bool wrap = (o0 + 1 == P);
         const uint32_t o1 = wrap ? 0u : o0 + 1;
         const uint32_t i1 = i0 + (wrap ? 1u : 0u);
-        if (k + 1 < Lw && i1 < ninst && o1 < out_len) r[(inst_base + i1) * out_len + o1] = rows.x1[v];
+        if (k + 1 < Lw && i1 < ninst && o1 < out_len) r[(inst_base + i1) * sr + o1] = rows.x1[v];
     }
 }
 
@@ -193,7 +196,7 @@ static size_t mul_lds_bytes(const MulLayout& L) {
 
 template <bool FULL, int Q, bool PF>
 static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
-                               uint64_t* r, hipStream_t s) {
+                               uint64_t* r, const Strides& st, hipStream_t s) {
     const size_t lds = mul_lds_bytes(L);
     if (lds > 160 * 1024) return hipErrorInvalidValue;
     const void* fn = (const void*)mul_kernel<FULL, Q, PF>;
@@ -205,13 +208,14 @@ static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const
     const uint32_t out_len = FULL ? 2 * m : m;
     const int32_t m_shift = (m & (m - 1)) == 0 ? __builtin_ctz(m) : -1;
     mul_kernel<FULL, Q, PF><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len,
-                                                           m_shift, a, b, r);
+                                                           m_shift, a, b, r, st.a ? st.a : m, st.b ? st.b : m,
+                                                           st.r ? st.r : out_len);
     return hipGetLastError();
 }
 
 template <bool FULL>
 static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                                        hipStream_t s) {
+                                        const Strides& st, hipStream_t s) {
     uint32_t Q = tune_get_class("mul.q", m, 8) == 8 ? 8 : 4;
     const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
     MulLayout L;
@@ -219,33 +223,37 @@ static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* 
         Q = 4;  // fall back to the smaller footprint
         if (!mul_layout(m, FULL, Q, &L)) return hipErrorInvalidValue;
     }
-    if (Q == 8) return pf ? launch_mul_t<FULL, 8, true>(L, n, m, a, b, r, s)
-                          : launch_mul_t<FULL, 8, false>(L, n, m, a, b, r, s);
-    return pf ? launch_mul_t<FULL, 4, true>(L, n, m, a, b, r, s) : launch_mul_t<FULL, 4, false>(L, n, m, a, b, r, s);
+    if (Q == 8) return pf ? launch_mul_t<FULL, 8, true>(L, n, m, a, b, r, st, s)
+                          : launch_mul_t<FULL, 8, false>(L, n, m, a, b, r, st, s);
+    return pf ? launch_mul_t<FULL, 4, true>(L, n, m, a, b, r, st, s)
+              : launch_mul_t<FULL, 4, false>(L, n, m, a, b, r, st, s);
 }
 
 hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
-                      hipStream_t s) {
-    return full ? launch_mul_full_or_lo<true>(n, m, a, b, r, s) : launch_mul_full_or_lo<false>(n, m, a, b, r, s);
+                      hipStream_t s, Strides st) {
+    return full ? launch_mul_full_or_lo<true>(n, m, a, b, r, st, s)
+                : launch_mul_full_or_lo<false>(n, m, a, b, r, st, s);
 }
 
 // Reference polyKerQ (cuda/ker-classic-mul.cu.h:533-559) as a chain of
 // launches over caller-provided scratch of 4*n*m limbs.
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                       uint64_t* scratch, hipStream_t s) {
+                       uint64_t* scratch, hipStream_t s, Strides st) {
     const size_t nm = (size_t)n * m;
     uint64_t* t1 = scratch;
     uint64_t* t2 = scratch + nm;
     uint64_t* t3 = scratch + 2 * nm;
     uint64_t* t4 = scratch + 3 * nm;
+    const uint64_t sa = st.a ? st.a : m, sb = st.b ? st.b : m;
+    // strided operands / result only at the ends of the chain; temporaries are batch-major
     hipError_t e;
-    if ((e = launch_mul(n, m, a, a, t1, false, s)) != hipSuccess) return e;          // t1 = a^2
-    if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s)) != hipSuccess) return e; // t2 = a^2 + b
-    if ((e = launch_mul(n, m, b, b, t3, false, s)) != hipSuccess) return e;          // t3 = b^2
-    if ((e = launch_add(n, m, t3, b, t4, nullptr, false, s)) != hipSuccess) return e; // t4 = b^2 + b
-    if ((e = launch_mul(n, m, t2, t4, t1, false, s)) != hipSuccess) return e;        // t5 = t2 * t4
-    if ((e = launch_mul(n, m, a, b, t3, false, s)) != hipSuccess) return e;          // t6 = a * b
-    return launch_add(n, m, t1, t3, r, nullptr, false, s);                            // r = t5 + t6
+    if ((e = launch_mul(n, m, a, a, t1, false, s, {sa, sa, 0})) != hipSuccess) return e;         // t1 = a^2
+    if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s, {0, sb, 0})) != hipSuccess) return e; // t2 = a^2 + b
+    if ((e = launch_mul(n, m, b, b, t3, false, s, {sb, sb, 0})) != hipSuccess) return e;         // t3 = b^2
+    if ((e = launch_add(n, m, t3, b, t4, nullptr, false, s, {0, sb, 0})) != hipSuccess) return e; // t4 = b^2 + b
+    if ((e = launch_mul(n, m, t2, t4, t1, false, s)) != hipSuccess) return e;                    // t5 = t2 * t4
+    if ((e = launch_mul(n, m, a, b, t3, false, s, {sa, sb, 0})) != hipSuccess) return e;         // t6 = a * b
+    return launch_add(n, m, t1, t3, r, nullptr, false, s, {0, 0, st.r});                         // r = t5 + t6
 }
 
 }  // namespace midint

@@ -8,32 +8,32 @@
 
 namespace midint {
 
-// dst[i][0..m) (u64) = src[i][0..m32) (u32), zero-extended; 2m >= m32.
-__global__ void pack_u32_kernel(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* __restrict__ src,
+// dst[i][0..m) (u64) = src[i*ss .. i*ss + m32) (u32), zero-extended; 2m >= m32.
+__global__ void pack_u32_kernel(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* __restrict__ src, uint64_t ss,
                                 uint64_t* __restrict__ dst) {
     const uint64_t total = n * m;
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = t / m;
         const uint32_t j = (uint32_t)(t - i * m);
-        const uint32_t* s = src + i * m32;
+        const uint32_t* s = src + i * ss;
         const uint32_t lo = 2 * j < m32 ? s[2 * j] : 0u;
         const uint32_t hi = 2 * j + 1 < m32 ? s[2 * j + 1] : 0u;
         dst[t] = ((uint64_t)hi << 32) | lo;
     }
 }
 
-// dst[i][0..w32) (u32) = the low w32 words of src[i] (u64, stride ms limbs);
-// carry[i] (optional) = bit 0 of word w32 of src[i].
+// dst[i*ds .. i*ds + w32) (u32) = the low w32 words of src[i] (u64, stride
+// ms limbs); carry[i] (optional) = bit 0 of word w32 of src[i].
 __global__ void unpack_u32_kernel(uint64_t n, uint32_t w32, uint32_t ms, const uint64_t* __restrict__ src,
-                                  uint32_t* __restrict__ dst, uint8_t* __restrict__ carry) {
+                                  uint32_t* __restrict__ dst, uint64_t ds, uint8_t* __restrict__ carry) {
     const uint64_t total = n * w32;
     const uint32_t* s32 = reinterpret_cast<const uint32_t*>(src);
     for (uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; t < total;
          t += (uint64_t)gridDim.x * blockDim.x) {
         const uint64_t i = t / w32;
         const uint32_t j = (uint32_t)(t - i * w32);
-        dst[t] = s32[i * 2 * ms + j];
+        dst[i * ds + j] = s32[i * 2 * ms + j];
         if (carry && j == 0) carry[i] = (uint8_t)(s32[i * 2 * ms + w32] & 1u);
     }
 }
@@ -43,17 +43,17 @@ static uint32_t grid_for(uint64_t total) {
     return (uint32_t)(g < 8192 ? (g ? g : 1) : 8192);
 }
 
-hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* src, uint64_t* dst,
+hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t* src, uint64_t ss, uint64_t* dst,
                            hipStream_t s) {
-    if (2ull * m < m32) return hipErrorInvalidValue;
-    pack_u32_kernel<<<grid_for(n * m), 256, 0, s>>>(n, m32, m, src, dst);
+    if (2ull * m < m32 || ss < m32) return hipErrorInvalidValue;
+    pack_u32_kernel<<<grid_for(n * m), 256, 0, s>>>(n, m32, m, src, ss, dst);
     return hipGetLastError();
 }
 
 hipError_t launch_unpack_u32(uint64_t n, uint32_t w32, uint32_t ms, const uint64_t* src, uint32_t* dst,
-                             uint8_t* carry, hipStream_t s) {
-    if (2ull * ms < (uint64_t)w32 + (carry ? 1 : 0)) return hipErrorInvalidValue;  // carry word must exist
-    unpack_u32_kernel<<<grid_for(n * w32), 256, 0, s>>>(n, w32, ms, src, dst, carry);
+                             uint64_t ds, uint8_t* carry, hipStream_t s) {
+    if (2ull * ms < (uint64_t)w32 + (carry ? 1 : 0) || ds < w32) return hipErrorInvalidValue;  // carry word must exist
+    unpack_u32_kernel<<<grid_for(n * w32), 256, 0, s>>>(n, w32, ms, src, dst, ds, carry);
     return hipGetLastError();
 }
 
