@@ -303,6 +303,11 @@ midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, 
                                   const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
 midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
                                       const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
+/* The same through the chosen prime (midint_mul_ntt_prime): the reference's
+ * gpuMulFFT<FftPrime32 / FftPrime64, m> (cuda/main.cu:415-579, selected in
+ * cuda/main.cu:765-766), exact. */
+midint_status midint_gpu_mul_fft_prime_host(int device, uint32_t prime, uint32_t n, uint32_t m, const uint64_t* h_a,
+                                            const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
 
 /* ------------------------------------------------------------- tuning */
 /* Process-wide launch parameters used by the kernels' launchers, set by the

@@ -70,6 +70,7 @@ _SIGS = {
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_mul_fft_prime_host": (_int, [_int, _u32, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_probe_valu": (_int, [_int, _dblp, _dblp]),
     "midint_probe_mac": (_int, [_int, _dblp]),
     "midint_max_limbs_ragged": (_u32, []),

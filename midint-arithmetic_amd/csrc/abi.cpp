@@ -471,7 +471,8 @@ midint_status midint_mul_ntt_prime(midint_ctx* ctx, uint32_t prime, uint32_t n, 
 enum class HostOp { Add, MulLo, MulFull, MulNtt };
 
 static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, const uint64_t* h_a,
-                              const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
+                              const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec,
+                              uint32_t prime = MIDINT_PRIME_GOLDILOCKS) {
     if (m == 0 || (n > 0 && (!h_a || !h_b || !h_r))) return MIDINT_EINVAL;
     if (n == 0) return MIDINT_OK;
     midint_ctx* ctx = nullptr;
@@ -495,7 +496,7 @@ static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, con
             case HostOp::Add: return midint_add(ctx, n, m, d_a, d_b, d_r, nullptr, nullptr);
             case HostOp::MulLo: return midint_mul_lo(ctx, n, m, d_a, d_b, d_r, nullptr);
             case HostOp::MulFull: return midint_mul_full(ctx, n, m, d_a, d_b, d_r, nullptr);
-            case HostOp::MulNtt: return midint_mul_ntt(ctx, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::MulNtt: return midint_mul_ntt_prime(ctx, prime, n, m, d_a, d_b, d_r, nullptr);
         }
         return MIDINT_EINVAL;
     };
@@ -533,6 +534,12 @@ midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, 
 midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
                                       const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
     return run_host(device, HostOp::MulNtt, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_mul_fft_prime_host(int device, uint32_t prime, uint32_t n, uint32_t m, const uint64_t* h_a,
+                                            const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
+    if (!valid_prime(prime)) return MIDINT_EINVAL;
+    return run_host(device, HostOp::MulNtt, n, m, h_a, h_b, h_r, runs, usec, prime);
 }
 
 // ----------------------------------------------------------- generators

@@ -167,6 +167,179 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// Digit kernel: the same column-group schoolbook on 28-bit digits (Q = 8).
+// Every product is below 2^56, so a column accumulator takes one
+// v_mad_u64_u32 per product and no carry instruction; 255 / 8 = 31 blocks
+// after each flush the high half moves to the overflow word. For 8192-bit
+// operands that is 304^2 (padded) mads instead of 2 x 256^2 mad + addc.
+// Column groups are 224 bits wide and tile two limb vectors E (even groups)
+// and O (odd groups); the product is E + O (digit_group_store).
+
+__device__ __forceinline__ uint32_t digit_at(const uint64_t* __restrict__ g, uint32_t m, uint32_t j) {
+    const uint32_t bit = kDigitBits * j;
+    const uint32_t q = bit >> 6, sh = bit & 63;
+    uint64_t v = g[q] >> sh;
+    if (sh > 64 - kDigitBits && q + 1 < m) v |= g[q + 1] << (64 - sh);
+    return (uint32_t)v & ((1u << kDigitBits) - 1);
+}
+
+template <bool FULL, bool PF>
+__global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32_t m, uint32_t D, uint32_t n_pad,
+                                                         uint32_t T, uint32_t words, uint32_t ipb, uint32_t P,
+                                                         const uint64_t* __restrict__ a,
+                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                         uint64_t sa, uint64_t sb, uint64_t sr) {
+    constexpr int Q = 8;
+    extern __shared__ __align__(16) uint64_t lds64[];
+    uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+    __shared__ uint32_t lds_gp[16];
+
+    const uint32_t tid = threadIdx.x;
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+
+    // ---- stage digits: A[0, n_pad) and W[x] = digit x - (Q-1) of b for
+    // x < n_pad + 2Q, zero outside the operand (and for slots past the grid)
+    {
+        const uint32_t per = 2 * n_pad + 2 * Q;
+        for (uint32_t f = tid; f < ipb * per; f += blockDim.x) {
+            const uint32_t s = f / per;
+            const uint32_t x = f - s * per;
+            uint32_t v = 0;
+            if (s < ninst) {
+                if (x < n_pad) {
+                    if (x < D) v = digit_at(a + (inst_base + s) * sa, m, x);
+                } else {
+                    const int32_t j = (int32_t)(x - n_pad) - (Q - 1);
+                    if (j >= 0 && (uint32_t)j < D) v = digit_at(b + (inst_base + s) * sb, m, (uint32_t)j);
+                }
+            }
+            lds[s * words + x] = v;
+        }
+    }
+    __syncthreads();
+
+    // ---- convolution (same lane / group pairing as mul_kernel)
+    const uint32_t slot = tid / T;
+    const uint32_t t = tid - slot * T;
+    const bool active = slot < ipb;
+    uint32_t k1 = Q * t, k2, i2, NB;
+    if (FULL) {
+        k2 = n_pad + Q * t;
+        i2 = Q * t;
+        NB = T + 1;
+    } else {
+        k2 = n_pad - Q - Q * t;
+        i2 = 0;
+        NB = 2 * T + 1;
+    }
+    ColGroups<Q> g;
+    if (active) {
+        const uint32_t* A = lds + slot * words;
+        uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
+        conv_lane<Q, PF, true>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
+    }
+    __syncthreads();  // everyone is done with A/B/snap; reuse LDS for E/O
+
+    uint64_t* Eb = lds64;
+    uint64_t* Ob = lds64 + ipb * P;
+    if (active) digit_lane_store(g, true, k1, k2, Eb + slot * P, Ob + slot * P, P);
+    __syncthreads();
+
+    // ---- normalisation: E + O over ipb segments of P limbs (4 rows per wave)
+    constexpr int R = 4;
+    const uint32_t Lw = ipb * P;
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<R> rows;
+    uint32_t off0[R], inst0[R];
+    {
+        SegPos sp = segpos_init(k_first, P);
+        const uint32_t sq = kRowLimbs / P, srr = kRowLimbs % P;
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t k = k_first + v * kRowLimbs;
+            rows.x0[v] = k < Lw ? Eb[k] : 0ull;
+            rows.x1[v] = k + 1 < Lw ? Eb[k + 1] : 0ull;
+            rows.y0[v] = k < Lw ? Ob[k] : 0ull;
+            rows.y1[v] = k + 1 < Lw ? Ob[k + 1] : 0ull;
+            rows.s0[v] = (sp.off == 0) && (k < Lw);
+            rows.s1[v] = (sp.off + 1 == P) && (k + 1 < Lw);
+            off0[v] = sp.off;
+            inst0[v] = sp.inst;
+            segpos_next(&sp, P, sq, srr);
+        }
+    }
+    block_add_rows<R>(rows, lds_gp);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        const uint32_t o0 = off0[v];
+        const uint32_t i0 = inst0[v];
+        if (k < Lw && i0 < ninst) r[(inst_base + i0) * sr + o0] = rows.x0[v];
+        const bool wrap = (o0 + 1 == P);
+        const uint32_t o1 = wrap ? 0u : o0 + 1;
+        const uint32_t i1 = i0 + (wrap ? 1u : 0u);
+        if (k + 1 < Lw && i1 < ninst) r[(inst_base + i1) * sr + o1] = rows.x1[v];
+    }
+}
+
+struct DigitLayout {
+    uint32_t D, n_pad, T, words, ipb, threads, P;
+};
+
+static bool digit_layout(uint32_t m, bool full, DigitLayout* L) {
+    constexpr uint32_t Q = 8;
+    if (m == 0 || m > kMulMaxLimbs) return false;
+    L->D = (64 * m + kDigitBits - 1) / kDigitBits;
+    L->n_pad = (L->D + 2 * Q - 1) / (2 * Q) * (2 * Q);
+    L->T = full ? L->n_pad / Q : L->n_pad / (2 * Q);
+    L->P = full ? 2 * m : m;
+    L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
+    if (L->T <= 256) {  // pack instances so that few lanes of the last wave idle
+        uint32_t best = 1;
+        double best_u = 0;
+        for (uint32_t k = 1; k * L->T <= 256; k++) {
+            const uint32_t th = (k * L->T + 63) / 64 * 64;
+            const double u = (double)(k * L->T) / th;
+            if (u >= best_u) {
+                best_u = u;
+                best = k;
+            }
+        }
+        L->ipb = best;
+        L->threads = (best * L->T + 63) / 64 * 64;
+    } else {
+        L->ipb = 1;
+        L->threads = (L->T + 63) / 64 * 64;
+    }
+    if (L->threads > 1024) return false;
+    // the normalisation covers 4 rows of 128 limbs per wave
+    if ((uint64_t)(L->threads / 64) * 4 * 128 < (uint64_t)L->ipb * L->P) return false;
+    return true;
+}
+
+static size_t digit_lds_bytes(const DigitLayout& L) {
+    const size_t conv = (size_t)L.ipb * L.words * 4;
+    const size_t norm = (size_t)L.ipb * L.P * 16;
+    return conv > norm ? conv : norm;
+}
+
+template <bool FULL, bool PF>
+static hipError_t launch_digit_t(const DigitLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
+                                 uint64_t* r, const Strides& st, hipStream_t s) {
+    const size_t lds = digit_lds_bytes(L);
+    const void* fn = (const void*)mul_digit_kernel<FULL, PF>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
+    mul_digit_kernel<FULL, PF><<<blocks, L.threads, lds, s>>>(n, m, L.D, L.n_pad, L.T, L.words, L.ipb, L.P, a, b, r,
+                                                             st.a ? st.a : m, st.b ? st.b : m, st.r ? st.r : L.P);
+    return hipGetLastError();
+}
+
 static bool mul_layout(uint32_t m, bool full, uint32_t Q, MulLayout* L) {
     if (m == 0 || m > kMulMaxLimbs) return false;
     L->Q = Q;
@@ -176,8 +349,12 @@ static bool mul_layout(uint32_t m, bool full, uint32_t Q, MulLayout* L) {
     L->P = full ? L->n_pad : L->n_pad / 2;
     L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
     if (L->T <= 256) {
-        L->threads = 256;
-        L->ipb = 256 / L->T;
+        // block size: tunable per size class ("mul.threads"); smaller blocks
+        // pack the per-wave LDS footprint more tightly into 160 KB
+        uint32_t th = (uint32_t)tune_get_class("mul.threads", m, 256);
+        if (th < 64 || th > 256 || th % 64 != 0 || th < L->T) th = 256;
+        L->threads = th;
+        L->ipb = th / L->T;
     } else {
         L->threads = (L->T + 63) / 64 * 64;
         L->ipb = 1;
@@ -216,6 +393,14 @@ static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const
 template <bool FULL>
 static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                                         const Strides& st, hipStream_t s) {
+    if (tune_get_class("mul.digits", m, 0) != 0) {
+        DigitLayout D;
+        if (digit_layout(m, FULL, &D) && digit_lds_bytes(D) <= 160 * 1024) {
+            const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
+            return pf ? launch_digit_t<FULL, true>(D, n, m, a, b, r, st, s)
+                      : launch_digit_t<FULL, false>(D, n, m, a, b, r, st, s);
+        }
+    }
     uint32_t Q = tune_get_class("mul.q", m, 8) == 8 ? 8 : 4;
     const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
     MulLayout L;

@@ -120,9 +120,39 @@ __device__ __forceinline__ void load_block(const uint32_t* pa, const uint32_t* p
     }
 }
 
-// Column q, a-word u uses window word q - u + Q - 1.
+// Digit mode: operands are d-bit digits (d <= 28), so every product is
+// below 2^56 and a 64-bit accumulator absorbs 255 of them on top of a value
+// below 2^40: one v_mad_u64_u32 per product and no carry handling. Every
+// DigitFlush::blocks blocks the bits from 2^40 up move into the overflow
+// word, which so counts units of 2^40 (a column sum stays below 2^70, so
+// the count stays below 2^30): sum = ovf * 2^40 + acc.
+constexpr uint32_t kDigitBits = 28;
 template <int Q>
+struct DigitFlush {
+    static constexpr uint32_t blocks = 255 / Q;  // Q products per accumulator per block
+};
+
+constexpr uint32_t kDigitFlushBit = 40;
+
+template <int Q>
+__device__ __forceinline__ void flush(ColGroups<Q>& g) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        g.ovf[q] += (uint32_t)(g.acc[q] >> kDigitFlushBit);
+        g.acc[q] &= (1ull << kDigitFlushBit) - 1;
+    }
+}
+
+// Column q, a-word u uses window word q - u + Q - 1.
+template <int Q, bool DIG = false>
 __device__ __forceinline__ void mac_block(const Operands<Q>& o, ColGroups<Q>& g) {
+    if (DIG) {
+#pragma unroll
+        for (int u = 0; u < Q; u++)
+#pragma unroll
+            for (int q = 0; q < Q; q++) g.acc[q] += (uint64_t)o.a[u] * o.b[q - u + Q - 1];
+        return;
+    }
     if (Q == 8) {
 #pragma unroll
         for (int u = 0; u < Q; u++) mac8(o.a[u], &o.b[Q - 1 - u], g.acc, g.ovf);
@@ -149,9 +179,10 @@ __device__ __forceinline__ void park(uint32_t* snap, const ColGroups<Q>& g) {
     }
 }
 
-template <int Q, bool PF>
+template <int Q, bool PF, bool DIG = false>
 __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, uint32_t k1, uint32_t k2,
                                           uint32_t i2, uint32_t sw, uint32_t NB, uint32_t* snap, ColGroups<Q>& g) {
+    constexpr uint32_t F = DigitFlush<Q>::blocks;
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         g.acc[q] = 0;
@@ -177,7 +208,8 @@ __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, 
             }
             load_block<Q>(pa, pb, x1);
             if (beta == sw) park<Q>(snap, g);
-            mac_block<Q>(x0, g);
+            mac_block<Q, DIG>(x0, g);
+            if (DIG && (beta + 1) % F == 0) flush<Q>(g);
             if (beta + 1 >= NB) break;
             if (beta + 2 == sw) {
                 pa = S + i2;
@@ -188,7 +220,8 @@ __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, 
             }
             load_block<Q>(pa, pb, x0);
             if (beta + 1 == sw) park<Q>(snap, g);
-            mac_block<Q>(x1, g);
+            mac_block<Q, DIG>(x1, g);
+            if (DIG && (beta + 2) % F == 0) flush<Q>(g);
         }
     } else {
         const uint32_t* pa = S;
@@ -201,7 +234,8 @@ __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, 
             }
             Operands<Q> cur;
             load_block<Q>(pa, pb, cur);
-            mac_block<Q>(cur, g);
+            mac_block<Q, DIG>(cur, g);
+            if (DIG && (beta + 1) % F == 0) flush<Q>(g);
             pa += Q;
             pb -= Q;
         }
@@ -266,6 +300,103 @@ __device__ __forceinline__ void lane_store(const ColGroups<Q>& g, bool two, uint
     } else {
         group_store<Q>(g.acc, g.ovf, k1, L, H, P);
     }
+}
+
+// Digit mode, Q = 8: the group of columns [kc, kc + 8) holds
+// V = sum_q S_q 2^(28 q), S_q = s_hi 2^64 + s_lo (< 2^96), so V < 2^293,
+// at bit 28 kc = 224 g (g = kc / 8). Even groups start on a limb (limb
+// 3.5 g), odd groups 32 bits into limb 3.5 g - 0.5. Even groups go to
+// vector E, odd ones to O; in each vector consecutive groups are 7 limbs
+// apart and V spans at most 6 limbs, so every group writes its 7-limb slot
+// (value and zero fill) and the slots tile the vector: the product is E + O.
+// The writer of group 0 also clears O's first 3 limbs. Limbs >= P dropped.
+__device__ __forceinline__ void digit_group_store(const uint64_t (&s_lo)[8], const uint64_t (&s_hi)[8],
+                                                  uint32_t kc, uint64_t* E, uint64_t* O, uint32_t P) {
+    static_assert(kDigitBits == 28, "slot geometry assumes 28-bit digits and 8 columns");
+    uint64_t V[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const int o = 28 * q;
+        const int l = o >> 6, r = o & 63;
+        const uint64_t p0 = s_lo[q] << r;
+        const uint64_t p1 = r ? (s_hi[q] << r) | (s_lo[q] >> (64 - r)) : s_hi[q];
+        const uint64_t p2 = r ? (s_hi[q] >> (64 - r)) : 0ull;
+        // V[l..] += (p2:p1:p0) with carries
+        uint64_t x = V[l] + p0;
+        uint64_t c = x < p0;
+        V[l] = x;
+        x = V[l + 1] + p1;
+        uint64_t c2 = x < p1;
+        x += c;
+        c2 += x < c;
+        V[l + 1] = x;
+        x = V[l + 2] + p2;
+        uint64_t c3 = x < p2;
+        x += c2;
+        c3 += x < c2;
+        V[l + 2] = x;
+#pragma unroll
+        for (int i = l + 3; i < 6; i++) {
+            x = V[i] + c3;
+            c3 = x < c3;
+            V[i] = x;
+        }
+    }
+    const uint32_t g = kc / 8;
+    if (g & 1u) {
+        const uint32_t base = 7 * (g >> 1) + 3;
+        const uint64_t W[7] = {V[0] << 32,
+                               (V[1] << 32) | (V[0] >> 32),
+                               (V[2] << 32) | (V[1] >> 32),
+                               (V[3] << 32) | (V[2] >> 32),
+                               (V[4] << 32) | (V[3] >> 32),
+                               (V[5] << 32) | (V[4] >> 32),
+                               V[5] >> 32};
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+            if (base + i < P) O[base + i] = W[i];
+    } else {
+        const uint32_t base = 7 * (g >> 1);
+#pragma unroll
+        for (int i = 0; i < 7; i++)
+            if (base + i < P) E[base + i] = i < 6 ? V[i] : 0ull;
+        if (g == 0) {
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                if (i < (int)P) O[i] = 0;
+        }
+    }
+}
+
+// Column sum acc + ovf 2^40 (digit mode) as s_hi 2^64 + s_lo.
+__device__ __forceinline__ void digit_canon(uint64_t acc, uint32_t ovf, uint64_t& s_lo, uint64_t& s_hi) {
+    const uint64_t t = (acc >> kDigitFlushBit) + (uint64_t)ovf;  // < 2^33
+    s_lo = (acc & ((1ull << kDigitFlushBit) - 1)) | (t << kDigitFlushBit);
+    s_hi = t >> (64 - kDigitFlushBit);
+}
+
+// Stores the one or two digit column groups of a lane after conv_lane; the
+// second group is the 96-bit difference of the final and the parked sums.
+__device__ __forceinline__ void digit_lane_store(const ColGroups<8>& g, bool two, uint32_t k1, uint32_t k2,
+                                                 uint64_t* E, uint64_t* O, uint32_t P) {
+    uint64_t fl[8], fh[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) digit_canon(g.acc[q], g.ovf[q], fl[q], fh[q]);
+    if (!two) {
+        digit_group_store(fl, fh, k1, E, O, P);
+        return;
+    }
+    uint64_t pl[8], ph[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) digit_canon(g.sacc[q], g.sovf[q], pl[q], ph[q]);
+    digit_group_store(pl, ph, k1, E, O, P);
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        const uint64_t lo = fl[q] - pl[q];
+        fh[q] = fh[q] - ph[q] - (fl[q] < pl[q] ? 1ull : 0ull);
+        fl[q] = lo;
+    }
+    digit_group_store(fl, fh, k2, E, O, P);
 }
 
 // The ragged kernel uses Q = 4.

@@ -3,6 +3,8 @@
 //
 //   mul.q         columns per lane group of the classical kernel (4 or 8)
 //   mul.prefetch  1: double-buffer the operand LDS reads
+//   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
+//   mul.threads   threads per block of the word kernel (64..256)
 //   add.rows      rows of 128 limbs per wave for small-m additions (1..8)
 //   add.threads   threads per block for small-m additions (64..1024)
 //   add.nt        1: non-temporal (streaming) 16-byte loads and stores
@@ -28,7 +30,7 @@ namespace {
 std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
-        {"mul.q", 8}, {"mul.prefetch", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
+        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
         {"ntt.threads", 0}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256},
     };
     return t;

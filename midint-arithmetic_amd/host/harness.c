@@ -4,7 +4,8 @@
  *
  *   runAdditions  -> run_additions   (cuda/main.cu:661-681)
  *   runNaiveMuls  -> run_naive_muls  (cuda/main.cu:683-709)
- *   runFFTMuls    -> run_fft_muls    (cuda/main.cu:711-746), validated here
+ *   runFFTMuls    -> run_fft_muls    (cuda/main.cu:711-746), validated here;
+ *                    Goldilocks, then FftPrime32 as cuda/main.cu:766 runs it
  *
  * For each size (in u32 words, as the reference counts them) the inputs are
  * generated with the reference's ourMkRandom<m,m> (glibc rand, seed 1), the
@@ -60,7 +61,7 @@ static int validate_exact(const uint64_t* want, const uint64_t* got, size_t coun
 
 enum { OP_ADD, OP_MUL_LO, OP_MUL_FFT };
 
-static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int gmp) {
+static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int gmp, uint32_t prime) {
     const uint32_t m = m_u32 / 2;                    /* u64 limbs */
     const uint32_t n = (uint32_t)(total_work / m_u32);
     const size_t in_limbs = (size_t)n * m;
@@ -85,14 +86,17 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int 
     else if (op == OP_MUL_LO)
         st = midint_gpu_mul_host(device, n, m, 0, a, b, r, runs, &usec);
     else
-        st = midint_gpu_mul_fft_host(device, n, m, a, b, r, runs, &usec);
+        st = midint_gpu_mul_fft_prime_host(device, prime, n, m, a, b, r, runs, &usec);
     if (st != MIDINT_OK) {
         printf("midint error: %s (hip %d)\n", midint_strerror(st), midint_last_hip_error());
         free(a), free(b), free(r), free(want);
         return 0;
     }
     const unsigned bits = m_u32 * 32;
-    const char* name = op == OP_ADD ? "add" : op == OP_MUL_LO ? "mul_lo" : "mul_ntt";
+    const char* name = op == OP_ADD      ? "add"
+                       : op == OP_MUL_LO ? "mul_lo"
+                       : prime == MIDINT_PRIME_P32 ? "mul_ntt_p32"
+                                                   : "mul_ntt";
     double metric;
     if (op == OP_ADD) {
         metric = 3.0 * in_limbs * 8 / (usec * 1000.0);
@@ -143,10 +147,13 @@ int main(int argc, char** argv) {
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64, 32, 16};
     int ok = 1;
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_ADD, device, total_work, sizes[i], gmp);
+        ok &= run_one(OP_ADD, device, total_work, sizes[i], gmp, 0);
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], gmp);
+        ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], gmp, 0);
     for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp);
+        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp, MIDINT_PRIME_GOLDILOCKS);
+    /* p32 products take 8-bit coefficients: up to 2048 u32 words (1024 limbs) */
+    for (size_t i = 1; i < sizeof(sizes) / sizeof(sizes[0]); i++)
+        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp, MIDINT_PRIME_P32);
     return ok ? 0 : 1;
 }

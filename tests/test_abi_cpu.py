@@ -84,3 +84,27 @@ def test_tuning_table(mi, tmp_path):
     assert L.midint_tune_load(str(f).encode()) == 6
     mi.tune_set("add.rows", old)
     mi.tune_set("add.nt", 1)
+
+
+def test_new_entry_points_reject_bad_arguments(mi):
+    """Argument checks of the u32, strided, CGBN, prime-NTT and poly_ntt entry
+    points run before any HIP call."""
+    L = mi.lib()
+    # NULL context
+    assert L.midint_add_u32(None, 1, 4, None, None, None, None, None) == 1
+    assert L.midint_mul_full_strided(None, 1, 4, None, 0, None, 0, None, 0, None) == 1
+    assert L.midint_cgbn(None, 0, 1, 64, None, None) == 1
+    assert L.midint_ntt_transform(None, 0, 0, 1, 6, None, None, None) == 1
+    assert L.midint_mul_ntt_prime(None, 1, 1, 4, None, None, None, None) == 1
+    assert L.midint_poly_ntt(None, 1, 4, None, None, None, None) == 1
+    # n == 0 is a no-op only with a context; m == 0 is always an error
+    assert L.midint_poly_ntt(None, 0, 0, None, None, None, None) == 1
+
+
+def test_declared_prime_and_layout_constants(mi):
+    assert mi.PRIMES[mi.PRIME_P32] == 3 * 2**30 + 1
+    assert mi.PRIMES[mi.PRIME_P64] == 29 * 2**57 + 1
+    assert mi.PRIMES[mi.PRIME_GOLDILOCKS] == 2**64 - 2**32 + 1
+    text = open(mi.HEADER).read()
+    for name in ("MIDINT_PRIME_P32 = 1", "MIDINT_NTT_REFERENCE_ROOT = 2", "MIDINT_CGBN_POLY = 3"):
+        assert name in text

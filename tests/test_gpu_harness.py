@@ -1,0 +1,26 @@
+"""The C harness (host/harness.c, the reference main.cu's runAdditions /
+runNaiveMuls / runFFTMuls flow on the C ABI) end to end on the GPU: every
+size of every sweep validated against GMP, as validateExact does."""
+import json
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def test_harness_sweeps_validate(mi):
+    import torch
+
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    exe = os.path.join(mi.HERE, "bin", "midint-harness")
+    out = subprocess.run([exe, str(1 << 20)], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout[-2000:] + out.stderr[-2000:]
+    lines = [json.loads(x) for x in out.stdout.splitlines() if x.startswith("{")]
+    ops = {x["op"] for x in lines}
+    assert ops == {"add", "mul_lo", "mul_ntt", "mul_ntt_p32"}, ops
+    assert len(lines) == 9 * 3 + 8
+    if "not found" not in out.stdout:
+        assert all(x["valid"] is True for x in lines), [x for x in lines if x["valid"] is not True]

@@ -121,8 +121,20 @@ def test_add6(ctx, torch_dev, orc, m):
 MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256, 512, 777, 1024, 2048]
 
 
+@pytest.mark.parametrize("digits", [0, 1])
 @pytest.mark.parametrize("m", MUL_SIZES)
-def test_mul_full_and_lo(ctx, torch_dev, orc, m):
+def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, digits):
+    """Both classical kernels: u32 words with mad + carry (mul.digits 0) and
+    28-bit digits with one mad per product (mul.digits 1)."""
+    old = mi.tune_get("mul.digits")
+    mi.tune_set("mul.digits", digits)
+    try:
+        _check_mul(ctx, torch_dev, orc, m)
+    finally:
+        mi.tune_set("mul.digits", old)
+
+
+def _check_mul(ctx, torch_dev, orc, m):
     torch = torch_dev
     n = 6 if m >= 1024 else 11
     a, b = operands(m * 17, n, m)
