@@ -118,6 +118,8 @@ const void* ragged_reset_ptr();
 // 2048 x 2048 NTT products [P00 | P11 | P01 | P10] (4096 limbs each); the
 // combine kernel adds P01 + P10 into limbs [2048, 8192) of P00 | P11.
 hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s);
+// Copies `bytes` (a multiple of 8; both pointers 16-byte aligned) with a kernel on `s`.
+hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 // Autotuned launch parameter (see tuning.cpp), or dflt for unknown keys.
 int64_t tune_get(const char* key, int64_t dflt);

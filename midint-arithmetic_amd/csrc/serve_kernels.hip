@@ -7,6 +7,10 @@
 // and P00 | P11 already is P00 + P11 X^2 (disjoint limbs), so the combine
 // adds P01 and then P10 into limbs [2048, 8192): two chained carry-resolved
 // additions on register-resident rows (6144 limbs = 16 waves x 3 rows).
+//
+// stage_copy: the batch's host<->HBM transfers (below).
+#include <algorithm>
+
 #include "add_core.hpp"
 #include "midint_internal.h"
 
@@ -51,6 +55,43 @@ __global__ __launch_bounds__(1024) void bigmul_combine_kernel(uint64_t* __restri
 hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s) {
     if (n == 0) return hipSuccess;
     bigmul_combine_kernel<<<n, 1024, 0, s>>>(blocks16k);
+    return hipGetLastError();
+}
+
+}  // namespace midint
+
+namespace midint {
+
+// Staging copy between pinned host memory and HBM for the serving loop:
+// moves `n16` 16-byte words, then one trailing 8-byte word when `tail8` is
+// set. It runs on the batch's own stream in place of a copy-engine transfer;
+// each thread keeps 4 loads in flight so that host reads over the bus
+// overlap.
+__global__ __launch_bounds__(256) void stage_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                         size_t n16, uint32_t tail8) {
+    constexpr int U = 4;
+    const size_t stride = (size_t)gridDim.x * 256;
+    size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
+    for (; i + (U - 1) * stride < n16; i += U * stride) {
+        uint4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; u++) dst[i + u * stride] = v[u];
+    }
+    for (; i < n16; i += stride) dst[i] = src[i];
+    if (tail8 && blockIdx.x == 0 && threadIdx.x == 0)
+        reinterpret_cast<uint64_t*>(dst + n16)[0] = reinterpret_cast<const uint64_t*>(src + n16)[0];
+}
+
+hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    if (bytes % 8 || reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(dst) % 16)
+        return hipErrorInvalidValue;
+    const size_t n16 = bytes / 16;
+    const size_t blocks = std::min<size_t>(1024, std::max<size_t>(1, (n16 + 1023) / 1024));
+    stage_copy_kernel<<<(uint32_t)blocks, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16,
+                                                        (uint32_t)(bytes % 16 != 0));
     return hipGetLastError();
 }
 
