@@ -55,18 +55,26 @@ def _set(mi, params, cls):
         mi.tune_set(f"{k}@{cls}", v)
 
 
-def tune_op(mi, torch, ctx, op, cls, grid, defaults, n, rng, log, sets=1):
-    """Times every configuration of `grid` for operand size `cls` on n
-    instances (cycling through `sets` buffer sets, so that additions stream
-    from HBM rather than the Infinity Cache); returns the fastest params."""
+def grid_of(grid):
+    """Every combination of a {key: [values]} grid, as a list of dicts."""
+    return [dict(zip(grid.keys(), v)) for v in itertools.product(*grid.values())]
+
+
+def tune_op(mi, torch, ctx, op, cls, cands, defaults, n, rng, log, sets=1, key_cls=None):
+    """Times every configuration in `cands` (a list of {key: value} dicts) for
+    operand size `cls` limbs on n instances of `op` ("add", "mul" or "ntt"),
+    cycling through `sets` buffer sets so that additions stream from HBM
+    rather than the Infinity Cache; returns the fastest params. The keys are
+    set at class `key_cls` (default `cls`)."""
     m = cls
+    key_cls = cls if key_cls is None else key_cls
     bufs = []
     for _ in range(sets):
         a = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
         b = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
         a[: m] = np.uint64(2**64 - 1)
         b[: m] = np.uint64(2**64 - 1)
-        out_l = 2 * n * m if op == "mul" else n * m
+        out_l = n * m if op == "add" else 2 * n * m
         bufs.append((torch.from_numpy(a.view(np.int64)).to("cuda:0"), torch.from_numpy(b.view(np.int64)).to("cuda:0"),
                      torch.empty(out_l, dtype=torch.int64, device="cuda:0")))
     it = [0]
@@ -76,19 +84,20 @@ def tune_op(mi, torch, ctx, op, cls, grid, defaults, n, rng, log, sets=1):
         it[0] += 1
         if op == "mul":
             ctx.mul_full(da, db, dr, m)
+        elif op == "ntt":
+            ctx.mul_ntt(da, db, dr, m)
         else:
             ctx.add(da, db, dr, m)
 
-    _set(mi, defaults, cls)
+    _set(mi, defaults, key_cls)
     it[0] = 0
     run()
     torch.cuda.synchronize()
     ref = bufs[0][2].clone()
     rows = []
     best, best_us = dict(defaults), None
-    for values in itertools.product(*grid.values()):
-        params = dict(zip(grid.keys(), values))
-        _set(mi, params, cls)
+    for params in cands:
+        _set(mi, params, key_cls)
         bufs[0][2].zero_()
         it[0] = 0
         run()
@@ -100,8 +109,8 @@ def tune_op(mi, torch, ctx, op, cls, grid, defaults, n, rng, log, sets=1):
         rows.append((params, us))
         if best_us is None or us < best_us:
             best, best_us = params, us
-    _set(mi, best, cls)
-    log(f"  {op}@{cls}: best {best} {best_us:.2f} us over {len(rows)} candidates (n = {n})")
+    _set(mi, best, key_cls)
+    log(f"  {op}@{key_cls}: best {best} {best_us:.2f} us over {len(rows)} candidates (n = {n})")
     return best, rows
 
 
@@ -153,17 +162,29 @@ def main(argv=None):
     add_grid = {"add.rows": [1, 2, 4], "add.threads": [64, 128, 256, 512], "add.nt": [0, 1]}
     for cls in add_classes:
         # 32 MiB per operand, 4 rotating sets (384 MiB > the 256 MiB Infinity Cache)
-        best, _ = tune_op(mi, torch, ctx, "add", cls, add_grid, {"add.rows": 2, "add.threads": 256, "add.nt": 1},
+        best, _ = tune_op(mi, torch, ctx, "add", cls, grid_of(add_grid), {"add.rows": 2, "add.threads": 256, "add.nt": 1},
                           (1 << 22) // cls, rng, log, sets=4)
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
 
     mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
-    mul_grid = {"mul.q": [4, 8], "mul.prefetch": [0, 1]}
+    # word kernel: Q x prefetch x block size; digit kernel: prefetch
+    mul_cands = grid_of({"mul.digits": [0], "mul.q": [4, 8], "mul.prefetch": [0, 1], "mul.threads": [128, 256]})
+    mul_cands += grid_of({"mul.digits": [1], "mul.q": [8], "mul.prefetch": [0, 1], "mul.threads": [256]})
+    mul_default = {"mul.digits": 0, "mul.q": 8, "mul.prefetch": 0, "mul.threads": 256}
     for cls in mul_classes:
         # about 2^30 partial products per launch, never fewer than 512 instances
         n = min(1 << 20, max(512, (1 << 30) // (2 * cls) ** 2))
-        best, _ = tune_op(mi, torch, ctx, "mul", cls, mul_grid, {"mul.q": 4, "mul.prefetch": 0}, n, rng, log)
+        best, _ = tune_op(mi, torch, ctx, "mul", cls, mul_cands, mul_default, n, rng, log)
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
+
+    # NTT values per thread, keyed by transform length L = 8m (2^11..2^14)
+    ntt_classes = [2048] if args.quick else [256, 512, 1024, 2048]
+    for cls in ntt_classes:
+        L = 8 * cls
+        n = max(256, (1 << 21) // cls)
+        best, _ = tune_op(mi, torch, ctx, "ntt", cls, grid_of({"ntt.vals": [16, 32]}), {"ntt.vals": 0}, n, rng, log,
+                          key_cls=L)
+        lines += [f"{k}@{L} {v}" for k, v in best.items()]
 
     lines.append(f"ragged.threads {tune_ragged(mi, torch, ctx, log)}")
 
