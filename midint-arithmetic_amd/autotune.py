@@ -167,10 +167,15 @@ def main(argv=None):
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
 
     mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
-    # word kernel: Q x prefetch x block size; digit kernel: prefetch
-    mul_cands = grid_of({"mul.digits": [0], "mul.q": [4, 8], "mul.prefetch": [0, 1], "mul.threads": [128, 256]})
-    mul_cands += grid_of({"mul.digits": [1], "mul.q": [8], "mul.prefetch": [0, 1], "mul.threads": [256]})
-    mul_default = {"mul.digits": 0, "mul.q": 8, "mul.prefetch": 0, "mul.threads": 256}
+    # word kernel: Q x register park x prefetch x block size; digit kernel:
+    # register park x prefetch
+    mul_cands = grid_of({"mul.digits": [0], "mul.q": [8, 16], "mul.rpark": [0, 1], "mul.prefetch": [0, 1],
+                         "mul.threads": [128, 256]})
+    mul_cands += grid_of({"mul.digits": [0], "mul.q": [4], "mul.rpark": [0], "mul.prefetch": [0, 1],
+                          "mul.threads": [256]})
+    mul_cands += grid_of({"mul.digits": [1], "mul.q": [8], "mul.rpark": [0, 1], "mul.prefetch": [0, 1],
+                          "mul.threads": [256]})
+    mul_default = {"mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 0, "mul.threads": 256}
     for cls in mul_classes:
         # about 2^30 partial products per launch, never fewer than 512 instances
         n = min(1 << 20, max(512, (1 << 30) // (2 * cls) ** 2))

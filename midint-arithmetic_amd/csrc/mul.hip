@@ -12,14 +12,18 @@
 // walked in ONE loop of equal trip count for all lanes: lane t works on its
 // first group for blocks 0..t and on its second group afterwards, so no lane
 // of a wavefront idles while another finishes a longer column range. At the
-// switch the lane parks its accumulators in LDS (one masked LDS store) and
-// keeps accumulating; the second group is recovered by subtraction.
+// switch the lane parks its accumulators (in registers with RPARK, else in
+// an LDS snapshot area) and keeps accumulating; the second group is
+// recovered by subtraction.
 //
 // Each loop block processes Q a-words x Q columns = Q^2 32x32->64 products
-// from one 16-byte broadcast LDS read of a and two 16-byte LDS reads of a
-// sliding window of b (lane-contiguous, bank-conflict free). Column sums are
-// 64-bit accumulators plus a 32-bit overflow count (the reference's
-// computeIter64, :11-26, at u32 granularity).
+// (Q = 4, 8 or 16) from Q/4 16-byte broadcast LDS reads of a and Q/2
+// 16-byte reads of a sliding 2Q-word window of b. For Q >= 8 the window
+// operand is stored skewed (4 pad words per Q words, skew_idx) so that the
+// lanes' window reads hit distinct LDS banks; unskewed, Q = 8 windows 32
+// bytes apart conflict pairwise, which cost a fifth of the kernel's time.
+// Column sums are 64-bit accumulators plus a 32-bit overflow count (the
+// reference's computeIter64, :11-26, at u32 granularity).
 //
 // Column sums are split into low words L and high words H shifted by one
 // group (reference from4Reg2ShmQ, :299-350) and normalised with one
@@ -31,6 +35,10 @@
 
 namespace midint {
 
+// Dynamic LDS a block may request: the 160 KB of a CU minus room for the
+// kernels' static LDS (the 64-byte row-adder scratch).
+constexpr size_t kMulLdsMax = 160 * 1024 - 256;
+
 struct MulLayout {
     uint32_t Q;       // columns per lane group
     uint32_t n_pad;   // padded operand length in u32 words (multiple of 2Q)
@@ -39,10 +47,17 @@ struct MulLayout {
     uint32_t words;   // LDS u32 words per instance slot (conv phase)
     uint32_t ipb;     // instances per block
     uint32_t threads; // block size
+    bool rpark;       // first column group parked in registers (no LDS snapshot area)
 };
 
-template <bool FULL, int Q, bool PF>
-__global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
+// Words of the W region of a slot: 2Q-padded window operand, skewed for Q >= 8.
+template <int Q>
+__host__ __device__ constexpr uint32_t mul_w_len(uint32_t n_pad) {
+    return Q >= 8 ? skew_len<Q>(n_pad + 2 * Q) : n_pad + 2 * Q;
+}
+
+template <bool FULL, int Q, bool PF, bool RPARK>
+__global__ __launch_bounds__(Q == 16 ? 256 : 1024) void mul_kernel(uint32_t n_inst, uint32_t m, uint32_t n_pad, uint32_t T,
                                                    uint32_t P, uint32_t words, uint32_t ipb, uint32_t out_len,
                                                    int32_t m_shift, const uint64_t* __restrict__ a,
                                                    const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
@@ -57,9 +72,12 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
 
-    // ---- stage operands into LDS: A[n_pad], B[(Q-1) + j] with zero pads.
-    // Only the pad words are cleared: A[2m, n_pad), B[0, Q-1) and
-    // B[Q-1+2m, n_pad+2Q) of every slot (the snapshot area needs no init).
+    // ---- stage operands into LDS: A[n_pad], then the window operand W with
+    // W[(Q-1) + j] = b_j, in the skewed layout for Q >= 8 (skew_idx). Only
+    // the pad words are cleared: A[2m, n_pad), W[0, Q-1) and
+    // W[Q-1+2m, n_pad+2Q) of every slot (the snapshot area needs no init).
+    constexpr bool SKEW = Q >= 8;
+    auto wi = [](uint32_t y) { return SKEW ? skew_idx<Q>(y) : y; };
     {
         const uint32_t padA = n_pad - 2 * m;
         const uint32_t padB = (Q - 1) + (n_pad + Q + 1 - 2 * m);
@@ -68,12 +86,13 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
             const uint32_t s = f / per;
             const uint32_t x = f - s * per;
             uint32_t* A = lds + s * words;
+            uint32_t* Wp = A + n_pad;
             if (x < padA)
                 A[2 * m + x] = 0u;
             else if (x < padA + (Q - 1))
-                A[n_pad + (x - padA)] = 0u;
+                Wp[wi(x - padA)] = 0u;
             else
-                A[n_pad + (Q - 1) + 2 * m + (x - padA - (Q - 1))] = 0u;
+                Wp[wi((Q - 1) + 2 * m + (x - padA - (Q - 1)))] = 0u;
         }
     }
     {
@@ -85,20 +104,20 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
             const uint32_t i = f - s * m;
             const uint64_t va = ga[s * sa + i], vb = gb[s * sb + i];
             uint32_t* A = lds + s * words;
-            uint32_t* Bw = A + n_pad + (Q - 1);
-            A[2 * i] = (uint32_t)va;
-            A[2 * i + 1] = (uint32_t)(va >> 32);
-            Bw[2 * i] = (uint32_t)vb;
-            Bw[2 * i + 1] = (uint32_t)(vb >> 32);
+            uint32_t* Wp = A + n_pad;
+            *reinterpret_cast<uint64_t*>(A + 2 * i) = va;
+            Wp[wi((Q - 1) + 2 * i)] = (uint32_t)vb;
+            Wp[wi((Q - 1) + 2 * i + 1)] = (uint32_t)(vb >> 32);
         }
         // slots past the last instance of the grid hold zero operands
         for (uint32_t f = ninst * m + tid; f < ipb * m; f += blockDim.x) {
             const uint32_t s = m_shift >= 0 ? (f >> m_shift) : f / m;
             const uint32_t i = f - s * m;
             uint32_t* A = lds + s * words;
-            uint32_t* Bw = A + n_pad + (Q - 1);
-            A[2 * i] = A[2 * i + 1] = 0u;
-            Bw[2 * i] = Bw[2 * i + 1] = 0u;
+            uint32_t* Wp = A + n_pad;
+            *reinterpret_cast<uint64_t*>(A + 2 * i) = 0ull;
+            Wp[wi((Q - 1) + 2 * i)] = 0u;
+            Wp[wi((Q - 1) + 2 * i + 1)] = 0u;
         }
     }
     __syncthreads();
@@ -120,10 +139,10 @@ __global__ __launch_bounds__(1024) void mul_kernel(uint32_t n_inst, uint32_t m, 
     ColGroups<Q> g;
     if (active) {
         const uint32_t* A = lds + slot * words;
-        uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
-        conv_lane<Q, PF>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
+        uint32_t* snap = lds + slot * words + n_pad + mul_w_len<Q>(n_pad) + 3 * Q * t;
+        conv_lane<Q, PF, false, SKEW, RPARK>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
     }
-    __syncthreads();  // everyone is done with A/B/snap; reuse LDS for L/H
+    __syncthreads();  // everyone is done with A/W/snap; reuse LDS for L/H
 
     uint64_t* Lb = lds64;
     uint64_t* Hb = lds64 + ipb * P;
@@ -184,7 +203,7 @@ __device__ __forceinline__ uint32_t digit_at(const uint64_t* __restrict__ g, uin
     return (uint32_t)v & ((1u << kDigitBits) - 1);
 }
 
-template <bool FULL, bool PF>
+template <bool FULL, bool PF, bool RPARK>
 __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32_t m, uint32_t D, uint32_t n_pad,
                                                          uint32_t T, uint32_t words, uint32_t ipb, uint32_t P,
                                                          const uint64_t* __restrict__ a,
@@ -199,8 +218,9 @@ __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
 
-    // ---- stage digits: A[0, n_pad) and W[x] = digit x - (Q-1) of b for
-    // x < n_pad + 2Q, zero outside the operand (and for slots past the grid)
+    // ---- stage digits: A[0, n_pad) and W[y] = digit y - (Q-1) of b for
+    // y < n_pad + 2Q in the skewed layout (skew_idx), zero outside the
+    // operand (and for slots past the grid)
     {
         const uint32_t per = 2 * n_pad + 2 * Q;
         for (uint32_t f = tid; f < ipb * per; f += blockDim.x) {
@@ -215,7 +235,7 @@ __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32
                     if (j >= 0 && (uint32_t)j < D) v = digit_at(b + (inst_base + s) * sb, m, (uint32_t)j);
                 }
             }
-            lds[s * words + x] = v;
+            lds[s * words + (x < n_pad ? x : n_pad + skew_idx<Q>(x - n_pad))] = v;
         }
     }
     __syncthreads();
@@ -237,8 +257,8 @@ __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32
     ColGroups<Q> g;
     if (active) {
         const uint32_t* A = lds + slot * words;
-        uint32_t* snap = lds + slot * words + 2 * n_pad + 2 * Q + 3 * Q * t;
-        conv_lane<Q, PF, true>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
+        uint32_t* snap = lds + slot * words + n_pad + mul_w_len<Q>(n_pad) + 3 * Q * t;
+        conv_lane<Q, PF, true, true, RPARK>(A, A + n_pad, k1, k2, i2, t + 1, NB, snap, g);
     }
     __syncthreads();  // everyone is done with A/B/snap; reuse LDS for E/O
 
@@ -286,16 +306,18 @@ __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32
 
 struct DigitLayout {
     uint32_t D, n_pad, T, words, ipb, threads, P;
+    bool rpark;
 };
 
-static bool digit_layout(uint32_t m, bool full, DigitLayout* L) {
+static bool digit_layout(uint32_t m, bool full, bool rpark, DigitLayout* L) {
     constexpr uint32_t Q = 8;
     if (m == 0 || m > kMulMaxLimbs) return false;
     L->D = (64 * m + kDigitBits - 1) / kDigitBits;
     L->n_pad = (L->D + 2 * Q - 1) / (2 * Q) * (2 * Q);
     L->T = full ? L->n_pad / Q : L->n_pad / (2 * Q);
     L->P = full ? 2 * m : m;
-    L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
+    L->rpark = rpark;
+    L->words = L->n_pad + mul_w_len<Q>(L->n_pad) + (rpark ? 0 : 3 * Q * L->T);
     if (L->T <= 256) {  // pack instances so that few lanes of the last wave idle
         uint32_t best = 1;
         double best_u = 0;
@@ -325,29 +347,32 @@ static size_t digit_lds_bytes(const DigitLayout& L) {
     return conv > norm ? conv : norm;
 }
 
-template <bool FULL, bool PF>
+template <bool FULL, bool PF, bool RPARK>
 static hipError_t launch_digit_t(const DigitLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
                                  uint64_t* r, const Strides& st, hipStream_t s) {
     const size_t lds = digit_lds_bytes(L);
-    const void* fn = (const void*)mul_digit_kernel<FULL, PF>;
+    const void* fn = (const void*)mul_digit_kernel<FULL, PF, RPARK>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
     const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
-    mul_digit_kernel<FULL, PF><<<blocks, L.threads, lds, s>>>(n, m, L.D, L.n_pad, L.T, L.words, L.ipb, L.P, a, b, r,
+    mul_digit_kernel<FULL, PF, RPARK><<<blocks, L.threads, lds, s>>>(n, m, L.D, L.n_pad, L.T, L.words, L.ipb, L.P, a, b, r,
                                                              st.a ? st.a : m, st.b ? st.b : m, st.r ? st.r : L.P);
     return hipGetLastError();
 }
 
-static bool mul_layout(uint32_t m, bool full, uint32_t Q, MulLayout* L) {
+static bool mul_layout(uint32_t m, bool full, uint32_t Q, bool rpark, MulLayout* L) {
     if (m == 0 || m > kMulMaxLimbs) return false;
+    if (Q != 4 && Q != 8 && Q != 16) return false;
     L->Q = Q;
+    L->rpark = rpark && Q >= 8;
     const uint32_t n = 2 * m;
     L->n_pad = (n + 2 * Q - 1) / (2 * Q) * (2 * Q);
     L->T = full ? L->n_pad / Q : L->n_pad / (2 * Q);
     L->P = full ? L->n_pad : L->n_pad / 2;
-    L->words = 2 * L->n_pad + 2 * Q + 3 * Q * L->T;
+    const uint32_t wlen = Q == 16 ? mul_w_len<16>(L->n_pad) : Q == 8 ? mul_w_len<8>(L->n_pad) : mul_w_len<4>(L->n_pad);
+    L->words = L->n_pad + wlen + (L->rpark ? 0 : 3 * Q * L->T);
     if (L->T <= 256) {
         // block size: tunable per size class ("mul.threads"); smaller blocks
         // pack the per-wave LDS footprint more tightly into 160 KB
@@ -359,7 +384,7 @@ static bool mul_layout(uint32_t m, bool full, uint32_t Q, MulLayout* L) {
         L->threads = (L->T + 63) / 64 * 64;
         L->ipb = 1;
     }
-    if (L->threads > 1024) return false;
+    if (L->threads > (Q == 16 ? 256u : 1024u)) return false;  // the kernel's launch bound
     // normalisation: Q/2 rows of 128 limbs per wave must cover ipb*P limbs
     if ((uint64_t)(L->threads / 64) * (Q / 2) * 128 < (uint64_t)L->ipb * L->P) return false;
     return true;
@@ -371,12 +396,12 @@ static size_t mul_lds_bytes(const MulLayout& L) {
     return conv > norm ? conv : norm;
 }
 
-template <bool FULL, int Q, bool PF>
+template <bool FULL, int Q, bool PF, bool RPARK>
 static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
                                uint64_t* r, const Strides& st, hipStream_t s) {
     const size_t lds = mul_lds_bytes(L);
-    if (lds > 160 * 1024) return hipErrorInvalidValue;
-    const void* fn = (const void*)mul_kernel<FULL, Q, PF>;
+    if (lds > kMulLdsMax) return hipErrorInvalidValue;
+    const void* fn = (const void*)mul_kernel<FULL, Q, PF, RPARK>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -384,9 +409,9 @@ static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const
     const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
     const uint32_t out_len = FULL ? 2 * m : m;
     const int32_t m_shift = (m & (m - 1)) == 0 ? __builtin_ctz(m) : -1;
-    mul_kernel<FULL, Q, PF><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb, out_len,
-                                                           m_shift, a, b, r, st.a ? st.a : m, st.b ? st.b : m,
-                                                           st.r ? st.r : out_len);
+    mul_kernel<FULL, Q, PF, RPARK><<<blocks, L.threads, lds, s>>>(n, m, L.n_pad, L.T, L.P, L.words, L.ipb,
+                                                                  out_len, m_shift, a, b, r, st.a ? st.a : m,
+                                                                  st.b ? st.b : m, st.r ? st.r : out_len);
     return hipGetLastError();
 }
 
@@ -395,23 +420,41 @@ static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* 
                                         const Strides& st, hipStream_t s) {
     if (tune_get_class("mul.digits", m, 0) != 0) {
         DigitLayout D;
-        if (digit_layout(m, FULL, &D) && digit_lds_bytes(D) <= 160 * 1024) {
+        const bool rp = tune_get_class("mul.rpark", m, 1) != 0;
+        if (digit_layout(m, FULL, rp, &D) && digit_lds_bytes(D) <= kMulLdsMax) {
             const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
-            return pf ? launch_digit_t<FULL, true>(D, n, m, a, b, r, st, s)
-                      : launch_digit_t<FULL, false>(D, n, m, a, b, r, st, s);
+            if (rp)
+                return pf ? launch_digit_t<FULL, true, true>(D, n, m, a, b, r, st, s)
+                          : launch_digit_t<FULL, false, true>(D, n, m, a, b, r, st, s);
+            return pf ? launch_digit_t<FULL, true, false>(D, n, m, a, b, r, st, s)
+                      : launch_digit_t<FULL, false, false>(D, n, m, a, b, r, st, s);
         }
     }
-    uint32_t Q = tune_get_class("mul.q", m, 8) == 8 ? 8 : 4;
+    const int64_t tq = tune_get_class("mul.q", m, 8);
+    uint32_t Q = tq == 16 ? 16 : tq == 4 ? 4 : 8;
     const bool pf = tune_get_class("mul.prefetch", m, 0) != 0;
+    const bool rp = tune_get_class("mul.rpark", m, 1) != 0;
     MulLayout L;
-    if (!mul_layout(m, FULL, Q, &L) || mul_lds_bytes(L) > 160 * 1024) {
-        Q = 4;  // fall back to the smaller footprint
-        if (!mul_layout(m, FULL, Q, &L)) return hipErrorInvalidValue;
+    if (!mul_layout(m, FULL, Q, rp, &L) || mul_lds_bytes(L) > kMulLdsMax) {
+        Q = 4;  // fall back to the smallest footprint
+        if (!mul_layout(m, FULL, Q, false, &L)) return hipErrorInvalidValue;
     }
-    if (Q == 8) return pf ? launch_mul_t<FULL, 8, true>(L, n, m, a, b, r, st, s)
-                          : launch_mul_t<FULL, 8, false>(L, n, m, a, b, r, st, s);
-    return pf ? launch_mul_t<FULL, 4, true>(L, n, m, a, b, r, st, s)
-              : launch_mul_t<FULL, 4, false>(L, n, m, a, b, r, st, s);
+    if (Q == 16) {
+        if (L.rpark)
+            return pf ? launch_mul_t<FULL, 16, true, true>(L, n, m, a, b, r, st, s)
+                      : launch_mul_t<FULL, 16, false, true>(L, n, m, a, b, r, st, s);
+        return pf ? launch_mul_t<FULL, 16, true, false>(L, n, m, a, b, r, st, s)
+                  : launch_mul_t<FULL, 16, false, false>(L, n, m, a, b, r, st, s);
+    }
+    if (Q == 8) {
+        if (L.rpark)
+            return pf ? launch_mul_t<FULL, 8, true, true>(L, n, m, a, b, r, st, s)
+                      : launch_mul_t<FULL, 8, false, true>(L, n, m, a, b, r, st, s);
+        return pf ? launch_mul_t<FULL, 8, true, false>(L, n, m, a, b, r, st, s)
+                  : launch_mul_t<FULL, 8, false, false>(L, n, m, a, b, r, st, s);
+    }
+    return pf ? launch_mul_t<FULL, 4, true, false>(L, n, m, a, b, r, st, s)
+              : launch_mul_t<FULL, 4, false, false>(L, n, m, a, b, r, st, s);
 }
 
 hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,

@@ -100,7 +100,26 @@ struct Operands {
     uint32_t b[2 * Q];
 };
 
+// Skewed layout of the sliding-window operand W (fixed-size kernels, Q >= 8):
+// logical word y is stored at y + 4 floor(y / Q), so every Q-word segment
+// is followed by 4 pad words. Lane windows start on segment boundaries, Q
+// logical words apart, hence Q + 4 physical words apart: an odd number of
+// 4-bank quads (3 for Q = 8, 5 for Q = 16), which puts the 16-byte reads of
+// any 16 consecutive lanes on distinct bank quads. Unskewed, lanes 8 words
+// apart share quads pairwise and every window read is a two-way bank
+// conflict. A window (2Q logical words) is two whole segments: physical
+// [p, p + Q) and [p + Q + 4, p + 2Q + 4), and moving it down by one block
+// moves p down by Q + 4.
 template <int Q>
+__host__ __device__ constexpr uint32_t skew_idx(uint32_t y) {
+    return y + 4 * (y / Q);
+}
+template <int Q>
+__host__ __device__ constexpr uint32_t skew_len(uint32_t n) {
+    return n + 4 * ((n + Q - 1) / Q);
+}
+
+template <int Q, bool SKEW = false>
 __device__ __forceinline__ void load_block(const uint32_t* pa, const uint32_t* pb, Operands<Q>& o) {
 #pragma unroll
     for (int i = 0; i < Q; i += 4) {
@@ -112,7 +131,8 @@ __device__ __forceinline__ void load_block(const uint32_t* pa, const uint32_t* p
     }
 #pragma unroll
     for (int i = 0; i < 2 * Q; i += 4) {
-        const uint4 v = *reinterpret_cast<const uint4*>(pb + i);
+        const uint32_t off = SKEW && i >= Q ? i + 4 : i;
+        const uint4 v = *reinterpret_cast<const uint4*>(pb + off);
         o.b[i] = v.x;
         o.b[i + 1] = v.y;
         o.b[i + 2] = v.z;
@@ -153,9 +173,11 @@ __device__ __forceinline__ void mac_block(const Operands<Q>& o, ColGroups<Q>& g)
             for (int q = 0; q < Q; q++) g.acc[q] += (uint64_t)o.a[u] * o.b[q - u + Q - 1];
         return;
     }
-    if (Q == 8) {
+    if (Q % 8 == 0) {
 #pragma unroll
-        for (int u = 0; u < Q; u++) mac8(o.a[u], &o.b[Q - 1 - u], g.acc, g.ovf);
+        for (int u = 0; u < Q; u++)
+#pragma unroll
+            for (int q0 = 0; q0 < Q; q0 += 8) mac8(o.a[u], &o.b[q0 + Q - 1 - u], &g.acc[q0], &g.ovf[q0]);
         return;
     }
 #pragma unroll
@@ -179,10 +201,26 @@ __device__ __forceinline__ void park(uint32_t* snap, const ColGroups<Q>& g) {
     }
 }
 
-template <int Q, bool PF, bool DIG = false>
+// Parks the running sums as the lane's first column group (registers).
+template <int Q>
+__device__ __forceinline__ void park_regs(ColGroups<Q>& g) {
+#pragma unroll
+    for (int q = 0; q < Q; q++) {
+        g.sacc[q] = g.acc[q];
+        g.sovf[q] = g.ovf[q];
+    }
+}
+
+// SKEW: W is in the skewed layout (skew_idx). RPARK: the first group is
+// parked in registers (sacc/sovf) instead of the LDS snapshot, trading 3Q
+// VGPRs and 3Q moves per block for the 3Q words per lane of LDS.
+template <int Q, bool PF, bool DIG = false, bool SKEW = false, bool RPARK = false>
 __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, uint32_t k1, uint32_t k2,
                                           uint32_t i2, uint32_t sw, uint32_t NB, uint32_t* snap, ColGroups<Q>& g) {
+    static_assert(!SKEW || Q >= 8, "the skewed layout needs Q >= 8");
     constexpr uint32_t F = DigitFlush<Q>::blocks;
+    constexpr uint32_t step = SKEW ? Q + 4 : Q;  // physical move of the window per block
+    auto wpos = [&](uint32_t y) { return W + (SKEW ? skew_idx<Q>(y) : y); };
 #pragma unroll
     for (int q = 0; q < Q; q++) {
         g.acc[q] = 0;
@@ -193,54 +231,67 @@ __device__ __forceinline__ void conv_lane(const uint32_t* S, const uint32_t* W, 
         // of block beta+1 is issued before the products of block beta. The
         // prefetch past the last block reads one block beyond the lane's
         // range, which stays inside the LDS staging area (pa <= S + ns + 2Q,
-        // pb >= W - Q >= S); its values are never used.
+        // pb >= W - step >= S); its values are never used.
         const uint32_t* pa = S;
-        const uint32_t* pb = W + k1;
+        const uint32_t* pb = wpos(k1);
         Operands<Q> x0, x1;
-        load_block<Q>(pa, pb, x0);
+        load_block<Q, SKEW>(pa, pb, x0);
         for (uint32_t beta = 0; beta < NB; beta += 2) {
             if (beta + 1 == sw) {
                 pa = S + i2;
-                pb = W + (k2 - i2);
+                pb = wpos(k2 - i2);
             } else {
                 pa += Q;
-                pb -= Q;
+                pb -= step;
             }
-            load_block<Q>(pa, pb, x1);
-            if (beta == sw) park<Q>(snap, g);
+            load_block<Q, SKEW>(pa, pb, x1);
+            if (beta == sw) {
+                if (RPARK)
+                    park_regs<Q>(g);
+                else
+                    park<Q>(snap, g);
+            }
             mac_block<Q, DIG>(x0, g);
             if (DIG && (beta + 1) % F == 0) flush<Q>(g);
             if (beta + 1 >= NB) break;
             if (beta + 2 == sw) {
                 pa = S + i2;
-                pb = W + (k2 - i2);
+                pb = wpos(k2 - i2);
             } else {
                 pa += Q;
-                pb -= Q;
+                pb -= step;
             }
-            load_block<Q>(pa, pb, x0);
-            if (beta + 1 == sw) park<Q>(snap, g);
+            load_block<Q, SKEW>(pa, pb, x0);
+            if (beta + 1 == sw) {
+                if (RPARK)
+                    park_regs<Q>(g);
+                else
+                    park<Q>(snap, g);
+            }
             mac_block<Q, DIG>(x1, g);
             if (DIG && (beta + 2) % F == 0) flush<Q>(g);
         }
     } else {
         const uint32_t* pa = S;
-        const uint32_t* pb = W + k1;
+        const uint32_t* pb = wpos(k1);
         for (uint32_t beta = 0; beta < NB; beta++) {
             if (beta == sw) {
-                park<Q>(snap, g);
+                if (RPARK)
+                    park_regs<Q>(g);
+                else
+                    park<Q>(snap, g);
                 pa = S + i2;
-                pb = W + (k2 - i2);
+                pb = wpos(k2 - i2);
             }
             Operands<Q> cur;
-            load_block<Q>(pa, pb, cur);
+            load_block<Q, SKEW>(pa, pb, cur);
             mac_block<Q, DIG>(cur, g);
             if (DIG && (beta + 1) % F == 0) flush<Q>(g);
             pa += Q;
-            pb -= Q;
+            pb -= step;
         }
     }
-    if (sw < NB) {
+    if (!RPARK && sw < NB) {
 #pragma unroll
         for (int q = 0; q < Q; q++) {
             g.sacc[q] = (uint64_t)snap[3 * q] | ((uint64_t)snap[3 * q + 1] << 32);

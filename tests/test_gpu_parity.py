@@ -5,6 +5,7 @@ fixed-size workloads are compared with the SHA-256 of GMP's results
 (tests/golden/digests.json) and with size-independent properties.
 Bit-exact everywhere: this is integer arithmetic.
 """
+import contextlib
 import hashlib
 import json
 import os
@@ -121,17 +122,64 @@ def test_add6(ctx, torch_dev, orc, m):
 MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256, 512, 777, 1024, 2048]
 
 
-@pytest.mark.parametrize("digits", [0, 1])
-@pytest.mark.parametrize("m", MUL_SIZES)
-def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, digits):
-    """Both classical kernels: u32 words with mad + carry (mul.digits 0) and
-    28-bit digits with one mad per product (mul.digits 1)."""
-    old = mi.tune_get("mul.digits")
-    mi.tune_set("mul.digits", digits)
+# Classical-kernel configurations (tuning keys): u32 words with mad + carry
+# at Q = 4, 8, 16 columns per lane group, the first group parked in LDS or
+# in registers, with and without operand prefetch; 28-bit digits with one
+# mad per product.
+MUL_CFGS = [
+    {"mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 0},
+    {"mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 1},
+    {"mul.digits": 0, "mul.q": 8, "mul.rpark": 0, "mul.prefetch": 0},
+    {"mul.digits": 0, "mul.q": 8, "mul.rpark": 0, "mul.prefetch": 1},
+    {"mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 0},
+    {"mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 1},
+    {"mul.digits": 0, "mul.q": 16, "mul.rpark": 0, "mul.prefetch": 0},
+    {"mul.digits": 0, "mul.q": 16, "mul.rpark": 1, "mul.prefetch": 0},
+    {"mul.digits": 0, "mul.q": 16, "mul.rpark": 1, "mul.prefetch": 1},
+    {"mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 0},
+    {"mul.digits": 1, "mul.rpark": 1, "mul.prefetch": 0},
+    {"mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 1},
+]
+
+
+def _cfg_id(c):
+    return "-".join(f"{k.split('.')[1]}{v}" for k, v in c.items())
+
+
+def _size_class(m):
+    c = 1
+    while c < m:
+        c <<= 1
+    return c
+
+
+@contextlib.contextmanager
+def tuned(mi, m, cfg):
+    """Sets `cfg` for the size class of m (class keys take precedence over
+    the global ones, and the tuning file sets class keys). Afterwards every
+    touched key holds its previous value, or the global value where the
+    class key was unset (the same lookup result)."""
+    saved = {}
+    for k, v in cfg.items():
+        key = f"{k}@{_size_class(m)}"
+        try:
+            saved[key] = mi.tune_get(key)
+        except Exception:
+            saved[key] = mi.tune_get(k)
+        mi.tune_set(key, v)
     try:
-        _check_mul(ctx, torch_dev, orc, m)
+        yield
     finally:
-        mi.tune_set("mul.digits", old)
+        for key, v in saved.items():
+            mi.tune_set(key, v)
+
+
+@pytest.mark.parametrize("cfg", MUL_CFGS, ids=_cfg_id)
+@pytest.mark.parametrize("m", MUL_SIZES)
+def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, cfg):
+    """Every classical-kernel configuration against the oracle."""
+    with tuned(mi, m, cfg):
+        _check_mul(ctx, torch_dev, orc, m)
 
 
 def _check_mul(ctx, torch_dev, orc, m):
