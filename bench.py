@@ -1,7 +1,8 @@
 """Benchmark of the midint HIP kernels on one MI355X.
 
 Headline (``value``): big-integer multiplications per second at 8192 bits --
-16384 pairs of 128-limb operands, full 256-limb products (classical kernel),
+16384 pairs of 128-limb operands, full 256-limb products (classical
+schoolbook, on the kernel the tuning file selects for 128 limbs),
 inputs resident in HBM, one step = one batch. The same JSON line also
 carries the batched-addition bandwidth (65536 x 64 limbs, HBM roofline) and
 the NTT multiplication rate (1024 x 2048 limbs).
@@ -23,11 +24,39 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POP/s
 MIB = 1 << 20
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def tuned_value(mi, key, m, default):
+    """The launch parameter the library uses for operands of m limbs: the
+    key at m's power-of-two class, else the global key, else `default`."""
+    c = 1
+    while c < m:
+        c <<= 1
+    for k in (f"{key}@{c}", key):
+        try:
+            return mi.tune_get(k)
+        except Exception:
+            pass
+    return default
+
+
+def mfma_work(m):
+    """(useful, issued) int8 ops of one m x m-limb full product on the matrix
+    cores: 7-bit digits D = ceil(64m / 7), useful = 2 D^2 digit products;
+    issued = 2 x 32768 MACs per v_mfma_i32_32x32x32_i8 over the kernel's
+    column tiles (each tile's d range)."""
+    D = (64 * m + 6) // 7
+    P = (D + 31) // 32
+    blocks = (64 * 2 * m + 223) // 224
+    NT = (blocks + 31) // 32
+    mfmas = sum(min(P, 32 * ct + 31) - max(0, 32 * ct - P + 1) + 1 for ct in range(NT))
+    return 2 * D * D, 2 * 32768 * mfmas
 
 
 def timed(torch, fn, steps, warmup):
@@ -192,8 +221,26 @@ def main():
     pp_per_launch = n * (2 * m) ** 2  # 32x32->64 partial products of the full product
     mad_rate, add_rate = mi.probe_valu(0)
     achieved_pp = pp_per_launch / (ms * 1e-3)
-    log(f"mul_full: {ms * 1e3:.1f} us/batch, {muls_per_s / 1e6:.2f} M muls/s; "
-        f"probe v_mad_u64_u32 {mad_rate / 1e12:.2f} T/s")
+    on_mfma = tuned_value(mi, "mul.mfma", m, 0) != 0 and tuned_value(mi, "mul.digits", m, 0) == 0
+    log(f"mul_full: {ms * 1e3:.1f} us/batch, {muls_per_s / 1e6:.2f} M muls/s "
+        f"({'int8 matrix-core' if on_mfma else 'VALU'} kernel); probe v_mad_u64_u32 {mad_rate / 1e12:.2f} T/s")
+    valu_roofline = {"bound": "valu", "achieved": achieved_pp / 1e12, "peak": mad_rate / 1e12,
+                     "unit": "T partial-products/s (32x32->64 v_mad_u64_u32)",
+                     "frac": achieved_pp / mad_rate, "traffic": measured_traffic("mul_full_16384x128"),
+                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"}
+    if on_mfma:
+        useful, issued = mfma_work(m)
+        tops = n * useful / (ms * 1e-3) / 1e12
+        roofline = {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
+                    "frac": tops / INT8_PEAK_TOPS, "traffic": measured_traffic("mul_full_16384x128"),
+                    "work": "int8 ops per launch = 16384 x 2 x 1171^2 (7-bit digit products of the 8192-bit "
+                            "operands); MFMA-issued ops incl. tile padding: "
+                            f"{n * issued / (ms * 1e-3) / 1e12:.0f} TOP/s",
+                    "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
+                    "peak_source": "MI355X published dense int8 MFMA rate"}
+    else:
+        roofline = valu_roofline
 
     # ---------------------------------------------------------- NTT
     nn, mn = 1024, 2048
@@ -259,15 +306,12 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u64",
+        "dtype": "i8" if on_mfma else "u64",
         "data": "synthetic (xoshiro256** limbs, seed 0x6D6964696E74 + 1)",
         "config": {"workload": "mul_16384x128: 16384 pairs of 8192-bit ints, full 256-limb products, "
-                               "classical kernel", "instances": n, "limbs": m},
-        "roofline": {"bound": "valu", "achieved": achieved_pp / 1e12, "peak": mad_rate / 1e12,
-                     "unit": "T partial-products/s (32x32->64 v_mad_u64_u32)",
-                     "frac": achieved_pp / mad_rate, "traffic": measured_traffic("mul_full_16384x128"),
-                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
-                     "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"},
+                               + ("classical kernel on the int8 matrix cores (7-bit digits)" if on_mfma
+                                  else "classical kernel (u32 words, VALU)"), "instances": n, "limbs": m},
+        "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": {
             "add_65536x64": {"us_per_launch": ms_add * 1e3, "GB_per_s": add_gbs,
@@ -282,6 +326,7 @@ def main():
                                                    "per product)",
                                            "frac": ntt_pp / (ms_ntt * 1e-3) / mad_rate,
                                            "traffic": measured_traffic("mul_ntt_1024x2048")}},
+            "mul_valu_equivalent": valu_roofline,
             "ragged_50000": ragged,
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
             "serving": serving,

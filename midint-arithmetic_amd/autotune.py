@@ -168,14 +168,18 @@ def main(argv=None):
 
     mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
     # word kernel: Q x register park x prefetch x block size; digit kernel:
-    # register park x prefetch
-    mul_cands = grid_of({"mul.digits": [0], "mul.q": [8, 16], "mul.rpark": [0, 1], "mul.prefetch": [0, 1],
-                         "mul.threads": [128, 256]})
-    mul_cands += grid_of({"mul.digits": [0], "mul.q": [4], "mul.rpark": [0], "mul.prefetch": [0, 1],
+    # register park x prefetch; matrix-core kernel: waves per instance
+    mul_cands = grid_of({"mul.mfma": [0], "mul.digits": [0], "mul.q": [8, 16], "mul.rpark": [0, 1],
+                         "mul.prefetch": [0, 1], "mul.threads": [128, 256]})
+    mul_cands += grid_of({"mul.mfma": [0], "mul.digits": [0], "mul.q": [4], "mul.rpark": [0], "mul.prefetch": [0, 1],
                           "mul.threads": [256]})
-    mul_cands += grid_of({"mul.digits": [1], "mul.q": [8], "mul.rpark": [0, 1], "mul.prefetch": [0, 1],
-                          "mul.threads": [256]})
-    mul_default = {"mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 0, "mul.threads": 256}
+    mul_cands += grid_of({"mul.mfma": [0], "mul.digits": [1], "mul.q": [8], "mul.rpark": [0, 1],
+                          "mul.prefetch": [0, 1], "mul.threads": [256]})
+    # int8 matrix-core kernel: waves per instance (0 = its own choice)
+    mul_cands += grid_of({"mul.mfma": [1], "mul.mfma_wpi": [0, 1, 2, 4, 8, 16], "mul.digits": [0], "mul.q": [8],
+                          "mul.rpark": [1], "mul.prefetch": [0], "mul.threads": [256]})
+    mul_default = {"mul.mfma": 0, "mul.mfma_wpi": 0, "mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 0,
+                   "mul.threads": 256}
     for cls in mul_classes:
         # about 2^30 partial products per launch, never fewer than 512 instances
         n = min(1 << 20, max(512, (1 << 30) // (2 * cls) ** 2))

@@ -34,6 +34,11 @@ constexpr uint32_t kMulMaxLimbs = 4096;
 // full: r[0..2m) = a*b ; otherwise r[0..m) = a*b mod B^m.
 hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                       hipStream_t s, Strides st = {});
+// The int8 matrix-core classical kernel (mul_mfma.hip): whether its layout
+// fits this size, and the launch (same contract as launch_mul).
+bool mul_mfma_supported(uint32_t m, bool full);
+hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                           hipStream_t s, Strides st = {});
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ).
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                        uint64_t* scratch, hipStream_t s, Strides st = {});

@@ -195,12 +195,15 @@ __global__ __launch_bounds__(Q == 16 ? 256 : 1024) void mul_kernel(uint32_t n_in
 // Column groups are 224 bits wide and tile two limb vectors E (even groups)
 // and O (odd groups); the product is E + O (digit_group_store).
 
-__device__ __forceinline__ uint32_t digit_at(const uint64_t* __restrict__ g, uint32_t m, uint32_t j) {
+// Digit j (< D) of an operand of w32 = 2m u32 words: bits [28j, 28j + 28),
+// one funnel shift of the two words that hold them (the word past the end
+// reads as zero).
+__device__ __forceinline__ uint32_t digit_at(const uint32_t* __restrict__ g32, uint32_t w32, uint32_t j) {
     const uint32_t bit = kDigitBits * j;
-    const uint32_t q = bit >> 6, sh = bit & 63;
-    uint64_t v = g[q] >> sh;
-    if (sh > 64 - kDigitBits && q + 1 < m) v |= g[q + 1] << (64 - sh);
-    return (uint32_t)v & ((1u << kDigitBits) - 1);
+    const uint32_t w = bit >> 5;
+    const uint32_t lo = g32[w];
+    const uint32_t hi = w + 1 < w32 ? g32[w + 1] : 0u;
+    return __builtin_amdgcn_alignbit(hi, lo, bit & 31) & ((1u << kDigitBits) - 1);
 }
 
 template <bool FULL, bool PF, bool RPARK>
@@ -221,21 +224,16 @@ __global__ __launch_bounds__(1024) void mul_digit_kernel(uint32_t n_inst, uint32
     // ---- stage digits: A[0, n_pad) and W[y] = digit y - (Q-1) of b for
     // y < n_pad + 2Q in the skewed layout (skew_idx), zero outside the
     // operand (and for slots past the grid)
-    {
-        const uint32_t per = 2 * n_pad + 2 * Q;
-        for (uint32_t f = tid; f < ipb * per; f += blockDim.x) {
-            const uint32_t s = f / per;
-            const uint32_t x = f - s * per;
-            uint32_t v = 0;
-            if (s < ninst) {
-                if (x < n_pad) {
-                    if (x < D) v = digit_at(a + (inst_base + s) * sa, m, x);
-                } else {
-                    const int32_t j = (int32_t)(x - n_pad) - (Q - 1);
-                    if (j >= 0 && (uint32_t)j < D) v = digit_at(b + (inst_base + s) * sb, m, (uint32_t)j);
-                }
-            }
-            lds[s * words + (x < n_pad ? x : n_pad + skew_idx<Q>(x - n_pad))] = v;
+    for (uint32_t s = 0; s < ipb; s++) {
+        const bool live = s < ninst;
+        const uint32_t* ga = reinterpret_cast<const uint32_t*>(a + (inst_base + s) * sa);
+        const uint32_t* gb = reinterpret_cast<const uint32_t*>(b + (inst_base + s) * sb);
+        uint32_t* A = lds + s * words;
+        uint32_t* Wp = A + n_pad;
+        for (uint32_t x = tid; x < n_pad; x += blockDim.x) A[x] = live && x < D ? digit_at(ga, 2 * m, x) : 0u;
+        for (uint32_t y = tid; y < n_pad + 2 * Q; y += blockDim.x) {
+            const uint32_t j = y - (Q - 1);  // wraps for y < Q-1
+            Wp[skew_idx<Q>(y)] = live && j < D ? digit_at(gb, 2 * m, j) : 0u;
         }
     }
     __syncthreads();
@@ -418,6 +416,8 @@ static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const
 template <bool FULL>
 static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                                         const Strides& st, hipStream_t s) {
+    if (tune_get_class("mul.mfma", m, 0) != 0 && mul_mfma_supported(m, FULL))
+        return launch_mul_mfma(n, m, a, b, r, FULL, s, st);
     if (tune_get_class("mul.digits", m, 0) != 0) {
         DigitLayout D;
         const bool rp = tune_get_class("mul.rpark", m, 1) != 0;

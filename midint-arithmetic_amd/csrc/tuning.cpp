@@ -3,6 +3,8 @@
 //
 //   mul.q         columns per lane group of the classical kernel (4, 8 or 16)
 //   mul.rpark     1: park the first column group in registers, not LDS (Q >= 8)
+//   mul.mfma      1: int8 matrix-core classical kernel (7-bit digits)
+//   mul.mfma_wpi  waves per instance of that kernel (0: half the column tiles)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)
@@ -31,7 +33,7 @@ namespace {
 std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
-        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
+        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
         {"ntt.threads", 0}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256},
     };
     return t;
