@@ -216,6 +216,15 @@ def main():
     da, db = to_dev(torch, a), to_dev(torch, b)
     dr = torch.empty(2 * n * m, dtype=torch.int64, device="cuda:0")
     log(f"mul_full {n} x {m} limbs: warmup {args.warmup}, steps {args.steps}")
+    # Untimed lead-in: an idle MI355X needs on the order of 100 ms of load to
+    # reach its working clocks (measured: 88 us per batch after 20 warm-up
+    # steps, 78 us after 2000). Run the step for 250 ms first, then the W
+    # warm-up steps, then the K timed steps.
+    t_end = time.perf_counter() + 0.25
+    while time.perf_counter() < t_end:
+        for _ in range(50):
+            ctx.mul_full(da, db, dr, m)
+        torch.cuda.synchronize()
     ms = timed(torch, lambda: ctx.mul_full(da, db, dr, m), args.steps, args.warmup)
     muls_per_s = n / (ms * 1e-3)
     pp_per_launch = n * (2 * m) ** 2  # 32x32->64 partial products of the full product
