@@ -15,9 +15,11 @@
 // sum times a power of two below 2^32 stays below 2^61.2.
 //
 // LDS: a's digits as 32-byte blocks of 32 digits, the instances of a block
-// one after another with 31 zero blocks before, between and after them
-// (B operand: one aligned 16-byte read per lane; a block index may run 31
-// past either end of its instance), and per instance b's digits reversed (A operand: row r of T_d is
+// one after another with a zero block before, between and after them (B
+// operand: one aligned 16-byte read per lane of block med3(J, -1, P), so a
+// block index outside the instance reads zeros and its MFMA adds nothing;
+// the loop then needs no per-tile range tests), and per instance b's
+// digits reversed (A operand: row r of T_d is
 // br[y0 .. y0+15], y0 = X0 - 32d - r + 16kg; the misalignment of y0 is fixed
 // per lane, so each row is five aligned dword reads and four funnel shifts).
 // Since 32 digits x 7 bits = 224 bits = 7 u32 words, block J of an operand
@@ -45,13 +47,21 @@ namespace midint {
 typedef int mf_v4i __attribute__((ext_vector_type(4)));
 typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kMfTiles = 3;  // column tiles (accumulator sets) per wave pass
+constexpr int kMfTiles = 2;      // column tiles (accumulator sets) per wave pass
+constexpr int kMfRawUnroll = 8;  // operand limbs per thread per staging round
 
-// Bytes of the a region of ipb instances (31 zero blocks, then per instance
-// P digit blocks and 31 zero blocks) and of one instance's br region (P + 3
+// Bytes of the a region of ipb instances (a zero block, then per instance
+// P digit blocks and a zero block) and of one instance's br region (P + 3
 // blocks, the last three zero).
-__host__ __device__ constexpr uint32_t mf_a_bytes(uint32_t P, uint32_t ipb) { return (31 + ipb * (P + 31)) * 32; }
+__host__ __device__ constexpr uint32_t mf_a_bytes(uint32_t P, uint32_t ipb) { return (1 + ipb * (P + 1)) * 32; }
 __host__ __device__ constexpr uint32_t mf_br_bytes(uint32_t P) { return (P + 3) * 32; }
+
+// floor(t / d) from mg = ceil(2^32 / d) (mf_magic, on the host) whenever
+// t * d < 2^32: t mg / 2^32 = t / d + t e / (d 2^32) with e = mg d - 2^32 < d,
+// and t e < 2^32 keeps the excess below 1 / d, which cannot carry past the
+// next multiple of 1 / d. mfma_layout checks the bound for every division.
+__device__ __forceinline__ uint32_t mf_div(uint32_t t, uint32_t mg) { return __umulhi(t, mg); }
+__host__ constexpr uint32_t mf_magic(uint32_t d) { return (uint32_t)((0xFFFFFFFFull + d) / d); }
 
 __device__ __forceinline__ uint32_t mf_ld(const uint32_t* __restrict__ g, uint32_t w, uint32_t W) {
     return w < W ? g[w] : 0u;
@@ -74,7 +84,7 @@ __device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint
 #pragma unroll
     for (int v = 0; v < 16; v++) {
         const int bit = 7 * (8 * (v / 4) + (v % 4));
-        A[bit >> 5] += (uint64_t)(uint32_t)c[v] * (uint64_t)(1u << (bit & 31));
+        A[bit >> 5] += (uint64_t)(uint32_t)c[v] << (bit & 31);
     }
     uint32_t u[8];
     uint64_t carry = 0;
@@ -133,7 +143,8 @@ __device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint
 
 template <bool FULL, int R>
 __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t NT,
-                                                        uint32_t wpi, uint32_t ipb, uint32_t PL,
+                                                        uint32_t wpi, uint32_t ipb, uint32_t PL, uint32_t mg_2m,
+                                                        uint32_t mg_p1, uint32_t mg_nbr,
                                                         const uint64_t* __restrict__ a,
                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
                                                         uint64_t sa, uint64_t sb, uint64_t sr) {
@@ -158,13 +169,24 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
     // 7 words from there to packed digits: a (zero-padded blocks) and b
     // reversed.
     {
+        // kMfRawUnroll limbs per thread per round, all loads issued before
+        // the stores
         uint64_t* raw = lds64;
         const uint32_t nl = ipb * 2 * m;
-        for (uint32_t t = tid; t < nl; t += blockDim.x) {
-            const uint32_t si = t / (2 * m), j = t - si * 2 * m;
-            uint64_t v = 0;
-            if (si < ninst) v = j < m ? a[(inst_base + si) * sa + j] : b[(inst_base + si) * sb + (j - m)];
-            raw[t] = v;
+        for (uint32_t t0 = tid; t0 < nl; t0 += kMfRawUnroll * blockDim.x) {
+            uint64_t v[kMfRawUnroll];
+#pragma unroll
+            for (int u = 0; u < kMfRawUnroll; u++) {
+                const uint32_t t = t0 + u * blockDim.x;
+                const uint32_t si = mf_div(t, mg_2m), j = t - si * 2 * m;
+                v[u] = 0;
+                if (t < nl && si < ninst) v[u] = j < m ? a[(inst_base + si) * sa + j] : b[(inst_base + si) * sb + (j - m)];
+            }
+#pragma unroll
+            for (int u = 0; u < kMfRawUnroll; u++) {
+                const uint32_t t = t0 + u * blockDim.x;
+                if (t < nl) raw[t] = v[u];
+            }
         }
     }
     __syncthreads();
@@ -176,15 +198,15 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
             uint8_t* dst8;
             const bool br = t >= NA;
             const uint32_t* g;
-            if (!br) {  // a-block t: instance (t-31) / (P+31), block (t-31) % (P+31)
-                const int32_t q = (int32_t)t - 31;
-                si = q < 0 ? 0u : (uint32_t)q / (P + 31);
-                J = q < 0 ? -1 : q - (int32_t)(si * (P + 31));
+            if (!br) {  // a-block t: instance (t-1) / (P+1), block (t-1) % (P+1)
+                const int32_t q = (int32_t)t - 1;
+                si = q < 0 ? 0u : mf_div((uint32_t)q, mg_p1);
+                J = q < 0 ? -1 : q - (int32_t)(si * (P + 1));
                 g = reinterpret_cast<const uint32_t*>(lds64 + (size_t)si * 2 * m);
                 dst8 = Areg + t * 32;
             } else {  // br block Y of instance si holds b-block P - Y, byte-reversed
                 const uint32_t u = t - NA;
-                si = u / NBR;
+                si = mf_div(u, mg_nbr);
                 const uint32_t Y = u - si * NBR;
                 J = (int32_t)P - (int32_t)Y;
                 g = reinterpret_cast<const uint32_t*>(lds64 + (size_t)si * 2 * m) + W32;
@@ -218,17 +240,17 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
     __syncthreads();
 
     // ---- MFMA phase: wave wi of instance s takes tiles wi, wi + wpi, ...
-    // in passes of up to kMfTiles tiles. Wave, tile and d indices are
-    // scalar; each d step loads the A operand (five dwords of br) and every
-    // tile's B operand (clamped to the tile's own d range, so always in
-    // bounds), and the loads of step d-1 are issued before the MFMAs of
-    // step d (double buffering; MFMAs only for tiles whose range holds d).
+    // in passes of up to kMfTiles tiles over the union of their d ranges.
+    // Wave and d indices are scalar; each d step loads the A operand (five
+    // dwords of br) and every tile's B operand (block clamped into
+    // [-1, P]: zeros outside the instance), and the loads of step d-1 are
+    // issued before the MFMAs of step d.
     {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
         const uint32_t lane = tid & 63;
         const uint32_t s = wv / wpi, wi = wv - s * wpi;
         const uint32_t n = lane & 31, kg = lane >> 5;
-        const uint8_t* Ab = Areg + (size_t)(31 + s * (P + 31)) * 32;  // block 0 of instance s
+        const uint8_t* Ab = Areg + (size_t)(1 + s * (P + 1)) * 32;  // block 0 of instance s
         const uint32_t* BR = reinterpret_cast<const uint32_t*>(BRreg + (size_t)s * BR_BYTES);
         const uint32_t ybase = X0 - n + 16 * kg;  // y0 at d = 0
         const uint32_t sh = (ybase & 3) * 8;
@@ -236,56 +258,51 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
         uint64_t* E = Eb + (size_t)s * PL;
         uint64_t* O = Ob + (size_t)s * PL;
         for (uint32_t c0 = wi; c0 < NT; c0 += kMfTiles * wpi) {
-            int32_t lo[kMfTiles], hi[kMfTiles];
-            const uint8_t* bb[kMfTiles];  // B operand of tile k at d = 0
+            // d range of the pass: union of its tiles' ranges
             int32_t dlo = (int32_t)P, dhi = 0;
+            int32_t J0[kMfTiles];  // lane's a-block index at d = 0
 #pragma unroll
             for (int k = 0; k < kMfTiles; k++) {
                 const uint32_t ct = c0 + k * wpi;
                 const int32_t I0 = 32 * (int32_t)ct;
                 if (ct < NT) {
-                    lo[k] = max(0, I0 - (int32_t)P + 1);
-                    hi[k] = min((int32_t)P, I0 + 31);
-                    dlo = min(dlo, lo[k]);
-                    dhi = max(dhi, hi[k]);
-                } else {
-                    lo[k] = 0;  // an empty range whose clamped loads stay in bounds
-                    hi[k] = -1;
+                    dlo = min(dlo, max(0, I0 - (int32_t)P + 1));
+                    dhi = max(dhi, min((int32_t)P, I0 + 31));
                 }
-                const int32_t J0 = (ct < NT ? I0 : 0) + (int32_t)n;
-                bb[k] = Ab + J0 * 32 + 16 * kg;
+                J0[k] = I0 + (int32_t)n;
             }
             mf_v16i acc[kMfTiles];
 #pragma unroll
             for (int k = 0; k < kMfTiles; k++) acc[k] = mf_v16i{};
-            uint32_t qa[5];
-            mf_v4i bq[kMfTiles];
-            auto issue = [&](int32_t d) {
+            // two operand sets in ping-pong: the loads of step d-1 go to one
+            // set while the MFMAs of step d read the other (no copies)
+            uint32_t qa0[5], qa1[5];
+            mf_v4i bq0[kMfTiles], bq1[kMfTiles];
+            auto issue = [&](int32_t d, uint32_t (&qa)[5], mf_v4i (&bq)[kMfTiles]) {
                 const uint32_t* pa = Abase - 8 * d;
 #pragma unroll
                 for (int i = 0; i < 5; i++) qa[i] = pa[i];
 #pragma unroll
                 for (int k = 0; k < kMfTiles; k++) {
-                    const int32_t dd = hi[k] < lo[k] ? 0 : min(max(d, lo[k]), hi[k]);
-                    bq[k] = *reinterpret_cast<const mf_v4i*>(bb[k] - dd * 32);
+                    const int32_t J = min(max(J0[k] - d, -1), (int32_t)P);  // v_med3_i32
+                    bq[k] = *reinterpret_cast<const mf_v4i*>(Ab + J * 32 + 16 * kg);
                 }
             };
-            issue(dhi);
-            for (int32_t d = dhi; d >= dlo; d--) {
-                uint32_t qc[5];
-                mf_v4i bc[kMfTiles];
-#pragma unroll
-                for (int i = 0; i < 5; i++) qc[i] = qa[i];
-#pragma unroll
-                for (int k = 0; k < kMfTiles; k++) bc[k] = bq[k];
-                issue(max(d - 1, dlo));
+            auto mma = [&](const uint32_t (&qc)[5], const mf_v4i (&bc)[kMfTiles]) {
                 const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qc[1], qc[0], sh),
                                    (int)__builtin_amdgcn_alignbit(qc[2], qc[1], sh),
                                    (int)__builtin_amdgcn_alignbit(qc[3], qc[2], sh),
                                    (int)__builtin_amdgcn_alignbit(qc[4], qc[3], sh)};
 #pragma unroll
-                for (int k = 0; k < kMfTiles; k++)
-                    if (d >= lo[k] && d <= hi[k]) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
+                for (int k = 0; k < kMfTiles; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
+            };
+            issue(dhi, qa0, bq0);
+            for (int32_t d = dhi; d >= dlo; d -= 2) {
+                issue(max(d - 1, dlo), qa1, bq1);
+                mma(qa0, bq0);
+                if (d - 1 < dlo) break;
+                issue(max(d - 2, dlo), qa0, bq0);
+                mma(qa1, bq1);
             }
 #pragma unroll
             for (int k = 0; k < kMfTiles; k++) {
@@ -357,6 +374,10 @@ bool mfma_layout(uint32_t m, bool full, MfmaLayout* L) {
     if (R > 8) return false;
     L->R = R;
     L->lds = (size_t)16 * L->ipb * L->PL + mf_a_bytes(L->P, L->ipb) + (size_t)L->ipb * mf_br_bytes(L->P);
+    // mf_div bounds: limb index x 2m, staging item x (P + 1) and x (P + 3)
+    const uint64_t limbs2 = (uint64_t)L->ipb * 2 * m;
+    const uint64_t items = 1 + (uint64_t)L->ipb * (2 * L->P + 4);
+    if (limbs2 * 2 * m >= (1ull << 32) || items * (L->P + 3) >= (1ull << 32)) return false;
     return L->lds <= 160 * 1024 - 256;
 }
 
@@ -369,7 +390,9 @@ hipError_t launch_mfma_t(const MfmaLayout& L, uint32_t n, uint32_t m, const uint
         if (e != hipSuccess) return e;
     }
     const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
-    mul_mfma_kernel<FULL, R><<<blocks, 64 * L.waves, L.lds, s>>>(n, m, L.P, L.NT, L.wpi, L.ipb, L.PL, a, b, r,
+    mul_mfma_kernel<FULL, R><<<blocks, 64 * L.waves, L.lds, s>>>(n, m, L.P, L.NT, L.wpi, L.ipb, L.PL,
+                                                                 mf_magic(2 * m), mf_magic(L.P + 1),
+                                                                 mf_magic(L.P + 3), a, b, r,
                                                                  st.a ? st.a : m, st.b ? st.b : m,
                                                                  st.r ? st.r : L.PL);
     return hipGetLastError();
