@@ -51,11 +51,11 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
     __shared__ uint32_t s_item;
     __shared__ uint32_t lds_gp[16];
 
-    // LDS: [conv area / L,H window] [snap: THREADS x 3Q words] [owner: window bytes] [lane_op: lanes bytes]
+    // LDS: [conv area / L,H window] [owner: window bytes] [lane_op: lanes bytes]; the
+    // first column group of a lane is parked in registers (no snapshot area)
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
     const uint32_t area_words = conv_cap > 4u * kRgWindow ? conv_cap : 4u * kRgWindow;
-    uint32_t* snap_all = lds + area_words;
-    uint8_t* owner = reinterpret_cast<uint8_t*>(snap_all + THREADS * 3 * Q);
+    uint8_t* owner = reinterpret_cast<uint8_t*>(lds + area_words);
     uint8_t* lane_op = owner + kRgWindow;
     uint64_t* Lb = lds64;
     uint64_t* Hb = lds64 + kRgWindow;
@@ -135,7 +135,7 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
                         two = false;
                     }
                     const uint32_t* S = lds + o.lds_base;
-                    conv_lane<Q, false>(S, S + o.ns_w + Q, k1, k2, i2, sw, NB, snap_all + 3 * Q * tid, g[rr]);
+                    conv_lane<Q, false, false, false, true>(S, S + o.ns_w + Q, k1, k2, i2, sw, NB, nullptr, g[rr]);
                     gk1[rr] = k1;
                     gk2[rr] = k2;
                     gop[rr] = k;
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
 template <int THREADS>
 static size_t ragged_lds_bytes(uint32_t conv_cap) {
     const uint32_t area_words = conv_cap > 4u * kRgWindow ? conv_cap : 4u * kRgWindow;
-    return (size_t)area_words * 4 + (size_t)THREADS * 3 * kMulQ * 4 + kRgWindow + kRgMaxLanes;
+    return (size_t)area_words * 4 + kRgWindow + kRgMaxLanes;
 }
 
 size_t ragged_lds_size(uint32_t threads, uint32_t conv_cap) {
