@@ -63,6 +63,13 @@ __host__ __device__ constexpr uint32_t mf_br_bytes(uint32_t P) { return (P + 3) 
 __device__ __forceinline__ uint32_t mf_div(uint32_t t, uint32_t mg) { return __umulhi(t, mg); }
 __host__ constexpr uint32_t mf_magic(uint32_t d) { return (uint32_t)((0xFFFFFFFFull + d) / d); }
 
+// min(max(x, lo), hi) for lo <= hi in one instruction.
+__device__ __forceinline__ uint32_t mf_med3(uint32_t x, uint32_t lo, uint32_t hi) {
+    uint32_t r;
+    asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
+    return r;
+}
+
 __device__ __forceinline__ uint32_t mf_ld(const uint32_t* __restrict__ g, uint32_t w, uint32_t W) {
     return w < W ? g[w] : 0u;
 }
@@ -251,6 +258,10 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
         const uint32_t s = wv / wpi, wi = wv - s * wpi;
         const uint32_t n = lane & 31, kg = lane >> 5;
         const uint8_t* Ab = Areg + (size_t)(1 + s * (P + 1)) * 32;  // block 0 of instance s
+        // B-operand byte addresses (from the start of LDS) of block 0 and of
+        // the zero blocks at -1 and P for this lane's K half
+        const uint32_t b_base = (uint32_t)(Ab - lds8) + 16 * kg;
+        const uint32_t b_lo = b_base - 32, b_hi = b_base + 32 * P;
         const uint32_t* BR = reinterpret_cast<const uint32_t*>(BRreg + (size_t)s * BR_BYTES);
         const uint32_t ybase = X0 - n + 16 * kg;  // y0 at d = 0
         const uint32_t sh = (ybase & 3) * 8;
@@ -260,7 +271,7 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
         for (uint32_t c0 = wi; c0 < NT; c0 += kMfTiles * wpi) {
             // d range of the pass: union of its tiles' ranges
             int32_t dlo = (int32_t)P, dhi = 0;
-            int32_t J0[kMfTiles];  // lane's a-block index at d = 0
+            uint32_t bb[kMfTiles];  // LDS byte address of the lane's B operand at d = 0
 #pragma unroll
             for (int k = 0; k < kMfTiles; k++) {
                 const uint32_t ct = c0 + k * wpi;
@@ -269,7 +280,7 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
                     dlo = min(dlo, max(0, I0 - (int32_t)P + 1));
                     dhi = max(dhi, min((int32_t)P, I0 + 31));
                 }
-                J0[k] = I0 + (int32_t)n;
+                bb[k] = b_base + (uint32_t)(32 * (I0 + (int32_t)n));
             }
             mf_v16i acc[kMfTiles];
 #pragma unroll
@@ -284,8 +295,9 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
                 for (int i = 0; i < 5; i++) qa[i] = pa[i];
 #pragma unroll
                 for (int k = 0; k < kMfTiles; k++) {
-                    const int32_t J = min(max(J0[k] - d, -1), (int32_t)P);  // v_med3_i32
-                    bq[k] = *reinterpret_cast<const mf_v4i*>(Ab + J * 32 + 16 * kg);
+                    // block J = J0 - d, clamped to the zero blocks at -1 and P
+                    const uint32_t addr = mf_med3(bb[k] - 32u * (uint32_t)d, b_lo, b_hi);
+                    bq[k] = *reinterpret_cast<const mf_v4i*>(lds8 + addr);
                 }
             };
             auto mma = [&](const uint32_t (&qc)[5], const mf_v4i (&bc)[kMfTiles]) {
@@ -296,14 +308,17 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
 #pragma unroll
                 for (int k = 0; k < kMfTiles; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
             };
+            // steps dhi, dhi-1, ..., dlo: pairs, then a last single step
+            const int32_t steps = dhi - dlo + 1;
             issue(dhi, qa0, bq0);
-            for (int32_t d = dhi; d >= dlo; d -= 2) {
-                issue(max(d - 1, dlo), qa1, bq1);
+            int32_t d = dhi;
+            for (int32_t i = 0; i + 1 < steps; i += 2, d -= 2) {
+                issue(d - 1, qa1, bq1);
                 mma(qa0, bq0);
-                if (d - 1 < dlo) break;
                 issue(max(d - 2, dlo), qa0, bq0);
                 mma(qa1, bq1);
             }
+            if (steps & 1) mma(qa0, bq0);
 #pragma unroll
             for (int k = 0; k < kMfTiles; k++) {
                 const uint32_t ct = c0 + k * wpi;
