@@ -48,7 +48,7 @@ typedef int mf_v4i __attribute__((ext_vector_type(4)));
 typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMfTiles = 2;      // column tiles (accumulator sets) per wave pass
-constexpr int kMfRawUnroll = 8;  // operand limbs per thread per staging round
+constexpr int kMfRawUnroll = 4;  // operand limbs per thread per staging round
 
 // Bytes of the a region of ipb instances (a zero block, then per instance
 // P digit blocks and a zero block) and of one instance's br region (P + 3
