@@ -46,16 +46,26 @@ def tuned_value(mi, key, m, default):
     return default
 
 
-def mfma_work(m):
+def mfma_work(m, wpi=0):
     """(useful, issued) int8 ops of one m x m-limb full product on the matrix
     cores: 7-bit digits D = ceil(64m / 7), useful = 2 D^2 digit products;
-    issued = 2 x 32768 MACs per v_mfma_i32_32x32x32_i8 over the kernel's
-    column tiles (each tile's d range)."""
+    issued = 2 x 32768 MACs per v_mfma_i32_32x32x32_i8, one per column tile
+    and d step of each pass (two tiles c0, c0 + wpi per pass over the union
+    of their d ranges, as the kernel runs them)."""
     D = (64 * m + 6) // 7
     P = (D + 31) // 32
     blocks = (64 * 2 * m + 223) // 224
     NT = (blocks + 31) // 32
-    mfmas = sum(min(P, 32 * ct + 31) - max(0, 32 * ct - P + 1) + 1 for ct in range(NT))
+    if wpi <= 0:
+        wpi = (NT + 2) // 4
+    wpi = max(1, min(wpi, 16, NT))
+    mfmas = 0
+    for wi in range(wpi):
+        for c0 in range(wi, NT, 2 * wpi):
+            tiles = [c0 + k * wpi for k in range(2) if c0 + k * wpi < NT]
+            lo = min(max(0, 32 * ct - P + 1) for ct in tiles)
+            hi = max(min(P, 32 * ct + 31) for ct in tiles)
+            mfmas += (hi - lo + 1) * len(tiles)
     return 2 * D * D, 2 * 32768 * mfmas
 
 
@@ -239,7 +249,7 @@ def main():
                      "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"}
     if on_mfma:
-        useful, issued = mfma_work(m)
+        useful, issued = mfma_work(m, tuned_value(mi, "mul.mfma_wpi", m, 0))
         tops = n * useful / (ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": tops / INT8_PEAK_TOPS, "traffic": measured_traffic("mul_full_16384x128"),
