@@ -48,14 +48,15 @@ def tuned_value(mi, key, m, default):
 
 def mfma_work(m, wpi=0):
     """(useful, issued) int8 ops of one m x m-limb full product on the matrix
-    cores: 7-bit digits D = ceil(64m / 7), useful = 2 D^2 digit products;
-    issued = 2 x 32768 MACs per v_mfma_i32_32x32x32_i8, one per column tile
-    and d step of each pass (two tiles c0, c0 + wpi per pass over the union
-    of their d ranges, as the kernel runs them)."""
-    D = (64 * m + 6) // 7
-    P = (D + 31) // 32
-    blocks = (64 * 2 * m + 223) // 224
-    NT = (blocks + 31) // 32
+    cores: balanced base-256 digits, D = 8m + 1 per operand, useful = 2 D^2
+    digit products; issued = 2 x 32768 MACs per v_mfma_i32_32x32x32_i8, one
+    per column tile and d step of each pass (up to two tiles c0, c0 + wpi per
+    pass over the union of their d ranges, as the kernel runs them)."""
+    D = 8 * m + 1
+    P = m // 4 + 1
+    dmax = (D + 30) // 32
+    N = (2 * m + 3) // 4
+    NT = (N + 31) // 32
     if wpi <= 0:
         wpi = (NT + 2) // 4
     wpi = max(1, min(wpi, 16, NT))
@@ -64,8 +65,8 @@ def mfma_work(m, wpi=0):
         for c0 in range(wi, NT, 2 * wpi):
             tiles = [c0 + k * wpi for k in range(2) if c0 + k * wpi < NT]
             lo = min(max(0, 32 * ct - P + 1) for ct in tiles)
-            hi = max(min(P, 32 * ct + 31) for ct in tiles)
-            mfmas += (hi - lo + 1) * len(tiles)
+            hi = max(min(dmax, 32 * ct + 31) for ct in tiles)
+            mfmas += max(0, hi - lo + 1) * len(tiles)
     return 2 * D * D, 2 * 32768 * mfmas
 
 
@@ -253,8 +254,8 @@ def main():
         tops = n * useful / (ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": tops / INT8_PEAK_TOPS, "traffic": measured_traffic("mul_full_16384x128"),
-                    "work": "int8 ops per launch = 16384 x 2 x 1171^2 (7-bit digit products of the 8192-bit "
-                            "operands); MFMA-issued ops incl. tile padding: "
+                    "work": f"int8 ops per launch = {n} x 2 x {8 * m + 1}^2 (balanced base-256 digit products "
+                            f"of the {64 * m}-bit operands); MFMA-issued ops incl. tile padding: "
                             f"{n * issued / (ms * 1e-3) / 1e12:.0f} TOP/s",
                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                     "peak_source": "MI355X published dense int8 MFMA rate"}
@@ -328,7 +329,7 @@ def main():
         "dtype": "i8" if on_mfma else "u64",
         "data": "synthetic (xoshiro256** limbs, seed 0x6D6964696E74 + 1)",
         "config": {"workload": "mul_16384x128: 16384 pairs of 8192-bit ints, full 256-limb products, "
-                               + ("classical kernel on the int8 matrix cores (7-bit digits)" if on_mfma
+                               + ("classical kernel on the int8 matrix cores (balanced 8-bit digits)" if on_mfma
                                   else "classical kernel (u32 words, VALU)"), "instances": n, "limbs": m},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -1,42 +1,53 @@
 // Classical (schoolbook) batched multiplication on the int8 matrix cores.
 //
-// Operands are split into 7-bit digits, non-negative in the signed int8
-// operands of v_mfma_i32_32x32x32_i8. With blocks of 32 digits
-// (a = sum_J a_J X^J, X = 2^224, D digits, P = ceil(D / 32) blocks), output
-// block I (digits 32I .. 32I+31) of the column sums is
+// Digits. An operand of m limbs is written in balanced base 256: with
+// x = a + 0x8080...80 (128 added to each of its 8m bytes, carries
+// propagated), digit i (i < 8m) is byte i of x minus 128, in [-128, 127],
+// stored as the int8 byte (byte i of x) ^ 0x80, and digit 8m is the carry out
+// of x (0 or 1). Then a = sum_i digit_i 256^i exactly, with D = 8m + 1
+// digits.
 //
-//     c_I = sum_{d=0..P} T_d a_{I-d},     T_d[r][s] = b_{32d + r - s},
+// Blocks of 32 digits are 256 bits = 4 limbs (a = sum_J a_J X^J, X = 2^256,
+// P = floor(m / 4) + 1 blocks). Output block I (digits 32I .. 32I+31) of the
+// column sums is
 //
-// a 32 x 32 Toeplitz matrix of b times a 32-digit block of a. One MFMA
-// evaluates T_d [a_{I0-d} .. a_{I0+31-d}] for the 32 output blocks of a
-// "column tile" I0 .. I0+31 and accumulates over d in its int32 C registers,
-// so no partial sum leaves the matrix core. A column sum is below
-// D * 127^2 < 2^29.2 for m <= 4096 limbs (D <= 37450), inside int32, and a
-// sum times a power of two below 2^32 stays below 2^61.2.
+//     c_I = sum_d T_d a_{I-d},     T_d[r][s] = b_{32d + r - s},
 //
-// LDS: a's digits as 32-byte blocks of 32 digits, the instances of a block
-// one after another with a zero block before, between and after them (B
-// operand: one aligned 16-byte read per lane of block med3(J, -1, P), so a
-// block index outside the instance reads zeros and its MFMA adds nothing;
-// the loop then needs no per-tile range tests), and per instance b's
-// digits reversed (A operand: row r of T_d is
+// a 32 x 32 Toeplitz matrix of b times a 32-digit block of a; T_d is zero
+// for d > dmax = floor((D + 30) / 32). One v_mfma_i32_32x32x32_i8 evaluates
+// T_d [a_{I0-d} .. a_{I0+31-d}] for the 32 output blocks of a "column tile"
+// I0 .. I0+31 and accumulates over d in its int32 C registers, so no partial
+// sum leaves the matrix core.
+//
+// Bias. A column sum is a sum of at most D digit products in
+// [-16256, 16384]. Every accumulator starts at 255 B (B = 2^21), which makes
+// each column sum non-negative and below 2^30 for m <= 4096 limbs. That
+// adds B (2^256 - 1) to every block value; over the N output blocks the
+// total is B 2^(256 N) - B, and 2^(256 N) is zero modulo the output length
+// (4N >= output limbs), so the lowest column of block 0 starts at 256 B
+// instead and the blocks sum to the product exactly.
+//
+// LDS: a's digits as 32-byte blocks, the instances of a workgroup one after
+// another with a zero block before, between and after them (B operand: one
+// aligned 16-byte read per lane of block med3(J, -1, P), so a block index
+// outside the instance reads zeros and its MFMA adds nothing; the loop needs
+// no per-tile range tests), and per instance b's digits reversed, br[y] =
+// b_digit[X0 - y] with X0 = 32P + 31 (A operand: row r of T_d is
 // br[y0 .. y0+15], y0 = X0 - 32d - r + 16kg; the misalignment of y0 is fixed
 // per lane, so each row is five aligned dword reads and four funnel shifts).
-// Since 32 digits x 7 bits = 224 bits = 7 u32 words, block J of an operand
-// is words 7J .. 7J+6 and its digits sit at fixed shifts; br is b's blocks
-// in reverse order, each byte-reversed. The product is written to two limb
-// vectors E and O shared by the block's instances.
+// Limb j of an operand is digits 8j .. 8j+7: one 8-byte word of the a region
+// and one byte-swapped 8-byte word of br, so staging needs no bit shuffling.
 //
 // Column sums to limbs: lane (n, kg) of a tile holds the 16 sums of output
-// block I = I0 + n at rows 8q + 4kg + t (q, t < 4). It folds them into
-// 32-bit-word accumulators (one v_mad_u64_u32 by a power of two per sum),
-// carries those into 32-bit words, shifts by 28 bits when kg = 1 and adds
-// its partner lane (lane ^ 32): the block's value V < 2^252 at bit 224 I.
-// Even blocks start on a limb (3.5 I) and go to E, odd blocks start 32 bits
-// into limb 3.5 I - 0.5 and go to O; every block writes its whole 7-limb
-// slot, the slots of each vector tile it, and the product is E + O: one
-// carry-resolving pass of the row adder (the same split as the 28-bit digit
-// kernel in mul.hip).
+// block I = I0 + n at rows 8q + 4kg + t (q, t < 4), i.e. at bits
+// 64q + 32kg + 8t of the block. G_q = sum_t c_{8q+4kg+t} 2^(8t) < 2^55
+// (three v_mad_u64_u32), so the lane's part is sum_q G_q 2^(64q) shifted up
+// by 32 kg bits, with no carries inside the lane. The kg = 0 lane takes its
+// partner's G_q (ds_bpermute) and adds them 32 bits up in one 9-word carry
+// chain: the block value V < 2^279 (5 limbs). Even blocks write vector E,
+// odd blocks vector O, each an 8-limb slot at limb 4I (the kg = 1 lane
+// writes its 3 zero limbs); the slots of each vector tile it and the product
+// is E + O, one carry-resolving pass of the row adder.
 #include <algorithm>
 
 #include "add_core.hpp"
@@ -47,12 +58,13 @@ namespace midint {
 typedef int mf_v4i __attribute__((ext_vector_type(4)));
 typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
-constexpr int kMfTiles = 2;      // column tiles (accumulator sets) per wave pass
-constexpr int kMfRawUnroll = 4;  // operand limbs per thread per staging round
+constexpr int kMfTiles = 2;                          // column tiles (accumulator sets) per wave pass
+constexpr uint64_t kMfHalf = 0x8080808080808080ull;  // 128 in every byte
+constexpr uint32_t kMfBias = 1u << 21;               // B
 
 // Bytes of the a region of ipb instances (a zero block, then per instance
 // P digit blocks and a zero block) and of one instance's br region (P + 3
-// blocks, the last three zero).
+// blocks).
 __host__ __device__ constexpr uint32_t mf_a_bytes(uint32_t P, uint32_t ipb) { return (1 + ipb * (P + 1)) * 32; }
 __host__ __device__ constexpr uint32_t mf_br_bytes(uint32_t P) { return (P + 3) * 32; }
 
@@ -60,8 +72,16 @@ __host__ __device__ constexpr uint32_t mf_br_bytes(uint32_t P) { return (P + 3) 
 // t * d < 2^32: t mg / 2^32 = t / d + t e / (d 2^32) with e = mg d - 2^32 < d,
 // and t e < 2^32 keeps the excess below 1 / d, which cannot carry past the
 // next multiple of 1 / d. mfma_layout checks the bound for every division.
-__device__ __forceinline__ uint32_t mf_div(uint32_t t, uint32_t mg) { return __umulhi(t, mg); }
-__host__ constexpr uint32_t mf_magic(uint32_t d) { return (uint32_t)((0xFFFFFFFFull + d) / d); }
+// d = 1 (2^32 does not fit) is encoded as mg = 0.
+__device__ __forceinline__ uint32_t mf_div(uint32_t t, uint32_t mg) { return mg ? __umulhi(t, mg) : t; }
+__host__ constexpr uint32_t mf_magic(uint32_t d) { return d == 1 ? 0u : (uint32_t)((0xFFFFFFFFull + d) / d); }
+
+__device__ __forceinline__ SegPos mf_segpos(uint32_t k, uint32_t seg, uint32_t mg) {
+    SegPos s;
+    s.inst = mf_div(k, mg);
+    s.off = k - s.inst * seg;
+    return s;
+}
 
 // min(max(x, lo), hi) for lo <= hi in one instruction.
 __device__ __forceinline__ uint32_t mf_med3(uint32_t x, uint32_t lo, uint32_t hi) {
@@ -70,88 +90,149 @@ __device__ __forceinline__ uint32_t mf_med3(uint32_t x, uint32_t lo, uint32_t hi
     return r;
 }
 
-__device__ __forceinline__ uint32_t mf_ld(const uint32_t* __restrict__ g, uint32_t w, uint32_t W) {
-    return w < W ? g[w] : 0u;
-}
+// 2^8, 2^16, 2^24 in SGPRs the compiler cannot see through (built from a
+// kernel argument), so that c * 2^k + G stays one v_mad_u64_u32 instead of
+// a 64-bit shift and add.
+struct MfPow {
+    uint32_t p8, p16, p24;
+};
 
-// Digits x0 .. x0+3 of an operand of W u32 words, one per byte (byte e =
-// digit x0 + e); digits past the operand read as zero.
-__device__ __forceinline__ uint32_t mf_digits4(const uint32_t* __restrict__ g, uint32_t W, uint32_t x0) {
-    const uint32_t bit = 7 * x0;
-    const uint32_t w = bit >> 5;
-    const uint32_t v = __builtin_amdgcn_alignbit(mf_ld(g, w + 1, W), mf_ld(g, w, W), bit & 31);
-    return (v & 0x7Fu) | ((v << 1) & 0x7F00u) | ((v << 2) & 0x7F0000u) | ((v << 3) & 0x7F000000u);
-}
-
-// Folds a finished tile into the E / O slots of its 32 output blocks.
-__device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint32_t kg, uint64_t* E, uint64_t* O,
-                                              uint32_t PL) {
-    // row 8q + t (t < 4) of the lane's half sits at bit 7 (8q + t) (+ 28 kg)
-    uint64_t A[6] = {0, 0, 0, 0, 0, 0};
+// Folds a finished tile into the E / O slots of its 32 output blocks (only
+// blocks below N are written).
+__device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint32_t kg, uint32_t N, uint64_t* E,
+                                              uint64_t* O, uint32_t paddr, const MfPow& pw) {
+    uint32_t g[8];
 #pragma unroll
-    for (int v = 0; v < 16; v++) {
-        const int bit = 7 * (8 * (v / 4) + (v % 4));
-        A[bit >> 5] += (uint64_t)(uint32_t)c[v] << (bit & 31);
+    for (int q = 0; q < 4; q++) {
+        uint64_t G = (uint64_t)(uint32_t)c[4 * q];
+        G += (uint64_t)(uint32_t)c[4 * q + 1] * pw.p8;
+        G += (uint64_t)(uint32_t)c[4 * q + 2] * pw.p16;
+        G += (uint64_t)(uint32_t)c[4 * q + 3] * pw.p24;
+        g[2 * q] = (uint32_t)G;
+        g[2 * q + 1] = (uint32_t)(G >> 32);
     }
-    uint32_t u[8];
-    uint64_t carry = 0;
+    // the partner lane's words (kg = 1 rows, 32 bits up)
+    uint32_t p[8];
 #pragma unroll
-    for (int w = 0; w < 6; w++) {
-        const uint64_t t = A[w] + carry;
-        u[w] = (uint32_t)t;
-        carry = t >> 32;
-    }
-    u[6] = (uint32_t)carry;
-    u[7] = 0;
-    uint32_t V[8];
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-        const uint32_t sh = __builtin_amdgcn_alignbit(u[w], w ? u[w - 1] : 0u, 4);  // (U << 28) word w
-        V[w] = kg ? sh : u[w];
-    }
-    uint32_t S[9];
-    uint32_t cy = 0;
-#pragma unroll
-    for (int w = 0; w < 8; w++) {
-        const uint32_t p = (uint32_t)__shfl_xor((int)V[w], 32);
-        const uint64_t t = (uint64_t)V[w] + p + cy;
-        S[w] = (uint32_t)t;
-        cy = (uint32_t)(t >> 32);
-    }
-    S[8] = cy;
-    uint64_t L[7];
-    uint64_t* dst;
-    uint32_t base;
-    if ((I & 1u) == 0) {
-        base = 7 * (I >> 1);
-        dst = E;
-#pragma unroll
-        for (int i = 0; i < 4; i++) L[i] = (uint64_t)S[2 * i] | ((uint64_t)S[2 * i + 1] << 32);
-        L[4] = S[8];
-    } else {
-        base = 7 * (I >> 1) + 3;
-        dst = O;
-        L[0] = (uint64_t)S[0] << 32;
-#pragma unroll
-        for (int i = 1; i < 5; i++) L[i] = (uint64_t)S[2 * i - 1] | ((uint64_t)S[2 * i] << 32);
-    }
-    L[5] = 0;
-    L[6] = 0;
-    // the lane pair splits the slot: kg = 0 limbs 0..3, kg = 1 limbs 4..6
-#pragma unroll
-    for (int i = 0; i < 7; i++)
-        if ((i < 4) == (kg == 0) && base + i < PL) dst[base + i] = L[i];
-    if (I == 0 && kg == 1) {
-#pragma unroll
-        for (int i = 0; i < 3; i++)
-            if ((uint32_t)i < PL) O[i] = 0;
+    for (int i = 0; i < 8; i++) p[i] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)paddr, (int)g[i]);
+    uint32_t w1, w2, w3, w4, w5, w6, w7, w8;
+    uint64_t cy;
+    asm("v_add_co_u32_e64 %[w1], %[c], %[g1], %[p0]\n\t"
+        "v_addc_co_u32_e64 %[w2], %[c], %[g2], %[p1], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w3], %[c], %[g3], %[p2], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w4], %[c], %[g4], %[p3], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w5], %[c], %[g5], %[p4], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w6], %[c], %[g6], %[p5], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w7], %[c], %[g7], %[p6], %[c]\n\t"
+        "v_addc_co_u32_e64 %[w8], %[c], %[p7], 0, %[c]"
+        : [w1] "=&v"(w1), [w2] "=&v"(w2), [w3] "=&v"(w3), [w4] "=&v"(w4), [w5] "=&v"(w5), [w6] "=&v"(w6),
+          [w7] "=&v"(w7), [w8] "=&v"(w8), [c] "=&s"(cy)
+        : [g1] "v"(g[1]), [g2] "v"(g[2]), [g3] "v"(g[3]), [g4] "v"(g[4]), [g5] "v"(g[5]), [g6] "v"(g[6]),
+          [g7] "v"(g[7]), [p0] "v"(p[0]), [p1] "v"(p[1]), [p2] "v"(p[2]), [p3] "v"(p[3]), [p4] "v"(p[4]),
+          [p5] "v"(p[5]), [p6] "v"(p[6]), [p7] "v"(p[7]));
+    if (I < N) {
+        // slots start on 32-byte boundaries (E, O and the instance stride
+        // are multiples of 4 limbs): 16-byte stores
+        uint64_t* dst = ((I & 1u) ? O : E) + 4 * I;
+        if (kg == 0) {
+            *reinterpret_cast<uint4*>(dst) = make_uint4(g[0], w1, w2, w3);
+            *reinterpret_cast<uint4*>(dst + 2) = make_uint4(w4, w5, w6, w7);
+            dst[4] = w8;
+        } else {
+            // a zero made here: a hoisted constant would hold registers
+            // through the whole MFMA phase
+            uint32_t z;
+            asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+            const uint4 z4 = make_uint4(z, z, z, z);
+            dst[5] = ((uint64_t)z << 32) | z;
+            *reinterpret_cast<uint4*>(dst + 6) = z4;
+            if (I == 0) {
+                *reinterpret_cast<uint4*>(O) = z4;
+                *reinterpret_cast<uint4*>(O + 2) = z4;
+            }
+        }
     }
 }
 
-template <bool FULL, int R>
-__global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t NT,
-                                                        uint32_t wpi, uint32_t ipb, uint32_t PL, uint32_t mg_2m,
-                                                        uint32_t mg_p1, uint32_t mg_nbr,
+// Per-wave operand addressing of one instance in the MFMA phase.
+struct MfLane {
+    const uint8_t* lds8;
+    const uint32_t* Abase;  // dword y0 >> 2 of the lane's A row at d = 0
+    uint32_t sh;            // bit misalignment of that row
+    uint32_t b_base;        // LDS byte address of the lane's B operand, block 0
+    uint32_t b_lo, b_hi;    // the zero blocks at -1 and P
+    uint32_t n, kg, lane, paddr;
+    uint64_t *E, *O;
+};
+
+// One pass of NTL tiles c0, c0 + wpi, ... over the union of their d ranges.
+template <int NTL>
+__device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t wpi, uint32_t P, uint32_t dmax,
+                                        uint32_t N, const MfPow& pw) {
+    int32_t dlo = (int32_t)dmax, dhi = 0;
+    uint32_t bb[NTL];  // LDS byte address of the lane's B operand at d = 0
+#pragma unroll
+    for (int k = 0; k < NTL; k++) {
+        const int32_t I0 = 32 * (int32_t)(c0 + k * wpi);
+        dlo = min(dlo, max(0, I0 - (int32_t)P + 1));
+        dhi = max(dhi, min((int32_t)dmax, I0 + 31));
+        bb[k] = L.b_base + (uint32_t)(32 * (I0 + (int32_t)L.n));
+    }
+    dlo = __builtin_amdgcn_readfirstlane(dlo);  // wave-uniform: keep d scalar
+    dhi = __builtin_amdgcn_readfirstlane(dhi);
+    mf_v16i acc[NTL];
+#pragma unroll
+    for (int k = 0; k < NTL; k++) {
+#pragma unroll
+        for (int v = 0; v < 16; v++) acc[k][v] = (int)(255u * kMfBias);
+    }
+    // lowest column of block 0: 256 B (the bias correction)
+    if (c0 == 0 && L.lane == 0) acc[0][0] = (int)(256u * kMfBias);
+
+    // two operand sets in ping-pong: the loads of step d-1 go to one set
+    // while the MFMAs of step d read the other (no copies)
+    uint32_t qa0[5], qa1[5];
+    mf_v4i bq0[NTL], bq1[NTL];
+    auto issue = [&](int32_t d, uint32_t (&qa)[5], mf_v4i (&bq)[NTL]) {
+        const uint32_t* pa = L.Abase - 8 * d;
+#pragma unroll
+        for (int i = 0; i < 5; i++) qa[i] = pa[i];
+#pragma unroll
+        for (int k = 0; k < NTL; k++) {
+            // block J = J0 - d, clamped to the zero blocks at -1 and P
+            const uint32_t addr = mf_med3(bb[k] - 32u * (uint32_t)d, L.b_lo, L.b_hi);
+            bq[k] = *reinterpret_cast<const mf_v4i*>(L.lds8 + addr);
+        }
+    };
+    auto mma = [&](const uint32_t (&qc)[5], const mf_v4i (&bc)[NTL]) {
+        const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qc[1], qc[0], L.sh),
+                           (int)__builtin_amdgcn_alignbit(qc[2], qc[1], L.sh),
+                           (int)__builtin_amdgcn_alignbit(qc[3], qc[2], L.sh),
+                           (int)__builtin_amdgcn_alignbit(qc[4], qc[3], L.sh)};
+#pragma unroll
+        for (int k = 0; k < NTL; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
+    };
+    // steps dhi, dhi-1, ..., dlo: pairs, then a last single step
+    const int32_t steps = dhi - dlo + 1;
+    if (steps > 0) {
+        issue(dhi, qa0, bq0);
+        int32_t d = dhi;
+        for (int32_t i = 0; i + 1 < steps; i += 2, d -= 2) {
+            issue(d - 1, qa1, bq1);
+            mma(qa0, bq0);
+            issue(max(d - 2, dlo), qa0, bq0);
+            mma(qa1, bq1);
+        }
+        if (steps & 1) mma(qa0, bq0);
+    }
+#pragma unroll
+    for (int k = 0; k < NTL; k++) mf_store_tile(acc[k], 32 * (c0 + k * wpi) + L.n, L.kg, N, L.E, L.O, L.paddr, pw);
+}
+
+template <int R>
+__global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t dmax,
+                                                        uint32_t N, uint32_t NT, uint32_t wpi, uint32_t ipb,
+                                                        uint32_t PL, uint32_t PLp, uint32_t mg_pl, uint32_t pow8,
                                                         const uint64_t* __restrict__ a,
                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
                                                         uint64_t sa, uint64_t sb, uint64_t sr) {
@@ -160,191 +241,144 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
     __shared__ uint32_t lds_gp[16];
 
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63;
     const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
     const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
-    const uint32_t W32 = 2 * m;
     const uint32_t X0 = 32 * P + 31;
     const uint32_t A_BYTES = mf_a_bytes(P, ipb), BR_BYTES = mf_br_bytes(P);
+    const uint32_t SA = 4 * (P + 1), SB = 4 * (P + 3);  // a / br words per instance
     uint64_t* Eb = lds64;
-    uint64_t* Ob = lds64 + (size_t)ipb * PL;
-    uint8_t* Areg = lds8 + (size_t)16 * ipb * PL;  // a blocks of all instances
-    uint8_t* BRreg = Areg + A_BYTES;                // br blocks, P + 3 per instance
+    uint64_t* Ob = lds64 + (size_t)ipb * PLp;
+    uint8_t* Areg = lds8 + (size_t)16 * ipb * PLp;  // a blocks of all instances
+    uint8_t* BRreg = Areg + A_BYTES;                 // br blocks, P + 3 per instance
+    uint64_t* A64 = reinterpret_cast<uint64_t*>(Areg);
+    uint64_t* BR64 = reinterpret_cast<uint64_t*>(BRreg);
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane;
 
-    // ---- stage digits, in two steps: (1) the operands' limbs, coalesced,
-    // into the E / O area (free until the MFMA phase: 2m limbs per instance
-    // fit in its 2 PL); (2) per (instance, 32-digit block) item, the block's
-    // 7 words from there to packed digits: a (zero-padded blocks) and b
-    // reversed.
+    // ---- stage digits: one wave per operand (operand o < ipb: a of instance
+    // o, else b of instance o - ipb), rows of 128 limbs in order with the
+    // carry between rows in a scalar: x = limbs + 0x8080...80, digits =
+    // x ^ 0x80 per byte below limb m, the carry at limb m and zeros above, up
+    // to limb SA (the instance's a blocks and its trailing zero block). a's
+    // words go to the a region in order, b's to br in reverse order (word
+    // SB - 9 - j for limb j, after br's 8 top zero words). The next row's
+    // loads are issued before the current row is processed.
     {
-        // kMfRawUnroll limbs per thread per round, all loads issued before
-        // the stores
-        uint64_t* raw = lds64;
-        const uint32_t nl = ipb * 2 * m;
-        for (uint32_t t0 = tid; t0 < nl; t0 += kMfRawUnroll * blockDim.x) {
-            uint64_t v[kMfRawUnroll];
-#pragma unroll
-            for (int u = 0; u < kMfRawUnroll; u++) {
-                const uint32_t t = t0 + u * blockDim.x;
-                const uint32_t si = mf_div(t, mg_2m), j = t - si * 2 * m;
-                v[u] = 0;
-                if (t < nl && si < ninst) v[u] = j < m ? a[(inst_base + si) * sa + j] : b[(inst_base + si) * sb + (j - m)];
+        const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), waves = blockDim.x >> 6;
+        const uint32_t nops = 2 * ipb, nrow = (m + kRowLimbs - 1) / kRowLimbs;
+        if (tid < 4) A64[tid] = 0;  // the zero block before instance 0
+        auto fetch = [&](uint32_t o, uint32_t row, uint64_t& x0, uint64_t& x1) {
+            const bool isb = o >= ipb;
+            const uint32_t s = isb ? o - ipb : o;
+            const uint64_t* __restrict__ src = (isb ? b : a) + (inst_base + s) * (isb ? sb : sa);
+            const uint32_t j0 = row * kRowLimbs + 2 * lane;
+            x0 = s < ninst && j0 < m ? src[j0] : 0ull;
+            x1 = s < ninst && j0 + 1 < m ? src[j0 + 1] : 0ull;
+        };
+        uint32_t o = wv, row = 0, cin = 0;
+        uint64_t nx0 = 0, nx1 = 0;
+        if (o < nops) fetch(o, 0, nx0, nx1);
+        while (o < nops) {
+            const uint64_t x0 = nx0, x1 = nx1;
+            uint32_t on = o, rn = row + 1;
+            if (rn == nrow) {
+                rn = 0;
+                on = o + waves;
             }
-#pragma unroll
-            for (int u = 0; u < kMfRawUnroll; u++) {
-                const uint32_t t = t0 + u * blockDim.x;
-                if (t < nl) raw[t] = v[u];
+            if (on < nops) fetch(on, rn, nx0, nx1);
+            const bool isb = o >= ipb;
+            const uint32_t s = isb ? o - ipb : o;
+            const uint32_t j0 = row * kRowLimbs + 2 * lane;
+            const RowState st = row_build(x0, j0 < m ? kMfHalf : 0ull, x1, j0 + 1 < m ? kMfHalf : 0ull, false, false);
+            const uint64_t C = row_chain(st, &cin);
+            uint64_t r0, r1;
+            row_apply(st, C, &r0, &r1);
+            const uint64_t w0 = j0 < m ? r0 ^ kMfHalf : r0;
+            const uint64_t w1 = j0 + 1 < m ? r1 ^ kMfHalf : r1;
+            if (j0 < SA) {  // SA and j0 even: both limbs
+                if (!isb)
+                    *reinterpret_cast<ulonglong2*>(A64 + 4 + s * SA + j0) = make_ulonglong2(w0, w1);
+                else
+                    *reinterpret_cast<ulonglong2*>(BR64 + s * SB + SB - 10 - j0) =
+                        make_ulonglong2(__builtin_bswap64(w1), __builtin_bswap64(w0));
             }
-        }
-    }
-    __syncthreads();
-    {
-        const uint32_t NA = A_BYTES / 32, NBR = P + 3, total = NA + ipb * NBR;
-        for (uint32_t t = tid; t < total; t += blockDim.x) {
-            int32_t J;
-            uint32_t si;
-            uint8_t* dst8;
-            const bool br = t >= NA;
-            const uint32_t* g;
-            if (!br) {  // a-block t: instance (t-1) / (P+1), block (t-1) % (P+1)
-                const int32_t q = (int32_t)t - 1;
-                si = q < 0 ? 0u : mf_div((uint32_t)q, mg_p1);
-                J = q < 0 ? -1 : q - (int32_t)(si * (P + 1));
-                g = reinterpret_cast<const uint32_t*>(lds64 + (size_t)si * 2 * m);
-                dst8 = Areg + t * 32;
-            } else {  // br block Y of instance si holds b-block P - Y, byte-reversed
-                const uint32_t u = t - NA;
-                si = mf_div(u, mg_nbr);
-                const uint32_t Y = u - si * NBR;
-                J = (int32_t)P - (int32_t)Y;
-                g = reinterpret_cast<const uint32_t*>(lds64 + (size_t)si * 2 * m) + W32;
-                dst8 = BRreg + (si * NBR + Y) * 32;
+            if (rn == 0) {
+                // limbs past the rows (at most 8): the carry when the rows
+                // end at limb m, zeros after; and br's top zero words
+                const uint32_t jt = nrow * kRowLimbs + lane;
+                if (jt < SA) {
+                    const uint64_t w = jt == m ? (uint64_t)cin : 0ull;
+                    if (!isb)
+                        A64[4 + s * SA + jt] = w;
+                    else
+                        BR64[s * SB + SB - 9 - jt] = __builtin_bswap64(w);
+                }
+                if (isb && lane < 8) BR64[s * SB + SB - 8 + lane] = 0;
+                cin = 0;
             }
-            if (si >= ipb) J = -1;
-            uint32_t wd[8];
-            const bool have = J >= 0 && J < (int32_t)P;
-#pragma unroll
-            for (int i = 0; i < 7; i++) wd[i] = have ? mf_ld(g, 7 * (uint32_t)J + i, W32) : 0u;
-            wd[7] = 0;
-            uint32_t o[8];
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const int bit = 28 * k;
-                const uint32_t v = __builtin_amdgcn_alignbit(wd[(bit >> 5) + 1], wd[bit >> 5], bit & 31);
-                o[k] = (v & 0x7Fu) | ((v << 1) & 0x7F00u) | ((v << 2) & 0x7F0000u) | ((v << 3) & 0x7F000000u);
-            }
-            uint4* dst = reinterpret_cast<uint4*>(dst8);
-            if (br) {
-                dst[0] = make_uint4(__builtin_bswap32(o[7]), __builtin_bswap32(o[6]), __builtin_bswap32(o[5]),
-                                    __builtin_bswap32(o[4]));
-                dst[1] = make_uint4(__builtin_bswap32(o[3]), __builtin_bswap32(o[2]), __builtin_bswap32(o[1]),
-                                    __builtin_bswap32(o[0]));
-            } else {
-                dst[0] = make_uint4(o[0], o[1], o[2], o[3]);
-                dst[1] = make_uint4(o[4], o[5], o[6], o[7]);
-            }
+            o = on;
+            row = rn;
         }
     }
     __syncthreads();
 
     // ---- MFMA phase: wave wi of instance s takes tiles wi, wi + wpi, ...
     // in passes of up to kMfTiles tiles over the union of their d ranges.
-    // Wave and d indices are scalar; each d step loads the A operand (five
-    // dwords of br) and every tile's B operand (block clamped into
-    // [-1, P]: zeros outside the instance), and the loads of step d-1 are
-    // issued before the MFMAs of step d.
     {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-        const uint32_t lane = tid & 63;
-        const uint32_t s = wv / wpi, wi = wv - s * wpi;
-        const uint32_t n = lane & 31, kg = lane >> 5;
+        // instance and first tile of the wave, kept scalar (the division
+        // would otherwise leave every loop bound in vector registers)
+        const uint32_t s = __builtin_amdgcn_readfirstlane(wv / wpi);
+        const uint32_t wi = __builtin_amdgcn_readfirstlane(wv - s * wpi);
+        MfLane L;
+        L.lds8 = lds8;
+        L.lane = lane;
+        L.n = lane & 31;
+        L.kg = lane >> 5;
+        L.paddr = (lane ^ 32u) << 2;
         const uint8_t* Ab = Areg + (size_t)(1 + s * (P + 1)) * 32;  // block 0 of instance s
-        // B-operand byte addresses (from the start of LDS) of block 0 and of
-        // the zero blocks at -1 and P for this lane's K half
-        const uint32_t b_base = (uint32_t)(Ab - lds8) + 16 * kg;
-        const uint32_t b_lo = b_base - 32, b_hi = b_base + 32 * P;
+        L.b_base = (uint32_t)(Ab - lds8) + 16 * L.kg;
+        L.b_lo = L.b_base - 32;
+        L.b_hi = L.b_base + 32 * P;
         const uint32_t* BR = reinterpret_cast<const uint32_t*>(BRreg + (size_t)s * BR_BYTES);
-        const uint32_t ybase = X0 - n + 16 * kg;  // y0 at d = 0
-        const uint32_t sh = (ybase & 3) * 8;
-        const uint32_t* Abase = BR + (ybase >> 2);  // dword y0 >> 2 at d = 0
-        uint64_t* E = Eb + (size_t)s * PL;
-        uint64_t* O = Ob + (size_t)s * PL;
+        const uint32_t ybase = X0 - L.n + 16 * L.kg;  // y0 at d = 0
+        L.sh = (ybase & 3) * 8;
+        L.Abase = BR + (ybase >> 2);
+        L.E = Eb + (size_t)s * PLp;
+        L.O = Ob + (size_t)s * PLp;
+        MfPow pw;
+        pw.p8 = pow8;
+        pw.p16 = pow8 * pow8;
+        pw.p24 = pw.p16 * pow8;
         for (uint32_t c0 = wi; c0 < NT; c0 += kMfTiles * wpi) {
-            // d range of the pass: union of its tiles' ranges
-            int32_t dlo = (int32_t)P, dhi = 0;
-            uint32_t bb[kMfTiles];  // LDS byte address of the lane's B operand at d = 0
-#pragma unroll
-            for (int k = 0; k < kMfTiles; k++) {
-                const uint32_t ct = c0 + k * wpi;
-                const int32_t I0 = 32 * (int32_t)ct;
-                if (ct < NT) {
-                    dlo = min(dlo, max(0, I0 - (int32_t)P + 1));
-                    dhi = max(dhi, min((int32_t)P, I0 + 31));
-                }
-                bb[k] = b_base + (uint32_t)(32 * (I0 + (int32_t)n));
-            }
-            mf_v16i acc[kMfTiles];
-#pragma unroll
-            for (int k = 0; k < kMfTiles; k++) acc[k] = mf_v16i{};
-            // two operand sets in ping-pong: the loads of step d-1 go to one
-            // set while the MFMAs of step d read the other (no copies)
-            uint32_t qa0[5], qa1[5];
-            mf_v4i bq0[kMfTiles], bq1[kMfTiles];
-            auto issue = [&](int32_t d, uint32_t (&qa)[5], mf_v4i (&bq)[kMfTiles]) {
-                const uint32_t* pa = Abase - 8 * d;
-#pragma unroll
-                for (int i = 0; i < 5; i++) qa[i] = pa[i];
-#pragma unroll
-                for (int k = 0; k < kMfTiles; k++) {
-                    // block J = J0 - d, clamped to the zero blocks at -1 and P
-                    const uint32_t addr = mf_med3(bb[k] - 32u * (uint32_t)d, b_lo, b_hi);
-                    bq[k] = *reinterpret_cast<const mf_v4i*>(lds8 + addr);
-                }
-            };
-            auto mma = [&](const uint32_t (&qc)[5], const mf_v4i (&bc)[kMfTiles]) {
-                const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qc[1], qc[0], sh),
-                                   (int)__builtin_amdgcn_alignbit(qc[2], qc[1], sh),
-                                   (int)__builtin_amdgcn_alignbit(qc[3], qc[2], sh),
-                                   (int)__builtin_amdgcn_alignbit(qc[4], qc[3], sh)};
-#pragma unroll
-                for (int k = 0; k < kMfTiles; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
-            };
-            // steps dhi, dhi-1, ..., dlo: pairs, then a last single step
-            const int32_t steps = dhi - dlo + 1;
-            issue(dhi, qa0, bq0);
-            int32_t d = dhi;
-            for (int32_t i = 0; i + 1 < steps; i += 2, d -= 2) {
-                issue(d - 1, qa1, bq1);
-                mma(qa0, bq0);
-                issue(max(d - 2, dlo), qa0, bq0);
-                mma(qa1, bq1);
-            }
-            if (steps & 1) mma(qa0, bq0);
-#pragma unroll
-            for (int k = 0; k < kMfTiles; k++) {
-                const uint32_t ct = c0 + k * wpi;
-                if (ct < NT) mf_store_tile(acc[k], 32 * ct + n, kg, E, O, PL);
-            }
+            if (c0 + wpi < NT)
+                mf_pass<2>(L, c0, wpi, P, dmax, N, pw);
+            else
+                mf_pass<1>(L, c0, wpi, P, dmax, N, pw);
         }
     }
     __syncthreads();
 
-    // ---- normalisation: E + O over ipb segments of PL limbs (R rows per wave)
+    // ---- normalisation: E + O over ipb segments of PL limbs (instance
+    // stride PLp in LDS), R rows per wave
     const uint32_t Lw = ipb * PL;
-    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
     Rows<R> rows;
     uint32_t off0[R], instv[R];
     {
-        SegPos sp = segpos_init(k_first, PL);
-        const uint32_t sq = kRowLimbs / PL, srr = kRowLimbs % PL;
+        SegPos sp = mf_segpos(k_first, PL, mg_pl);
+        const uint32_t sq = mf_div(kRowLimbs, mg_pl), srr = kRowLimbs - sq * PL;
 #pragma unroll
         for (int v = 0; v < R; v++) {
             const uint32_t k = k_first + v * kRowLimbs;
-            rows.x0[v] = k < Lw ? Eb[k] : 0ull;
-            rows.x1[v] = k + 1 < Lw ? Eb[k + 1] : 0ull;
-            rows.y0[v] = k < Lw ? Ob[k] : 0ull;
-            rows.y1[v] = k + 1 < Lw ? Ob[k + 1] : 0ull;
+            const bool wrap = sp.off + 1 == PL;
+            const uint32_t e0 = sp.inst * PLp + sp.off;
+            const uint32_t e1 = e0 + 1 + (wrap ? PLp - PL : 0u);
+            rows.x0[v] = k < Lw ? Eb[e0] : 0ull;
+            rows.x1[v] = k + 1 < Lw ? Eb[e1] : 0ull;
+            rows.y0[v] = k < Lw ? Ob[e0] : 0ull;
+            rows.y1[v] = k + 1 < Lw ? Ob[e1] : 0ull;
             rows.s0[v] = (sp.off == 0) && (k < Lw);
-            rows.s1[v] = (sp.off + 1 == PL) && (k + 1 < Lw);
+            rows.s1[v] = wrap && (k + 1 < Lw);
             off0[v] = sp.off;
             instv[v] = sp.inst;
             segpos_next(&sp, PL, sq, srr);
@@ -367,61 +401,47 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
 namespace {
 
 struct MfmaLayout {
-    uint32_t D, P, NT, wpi, ipb, waves, R, PL;
+    uint32_t P, dmax, N, NT, wpi, ipb, waves, R, PL, PLp;
     size_t lds;
 };
 
 bool mfma_layout(uint32_t m, bool full, MfmaLayout* L) {
     if (m == 0 || m > kMulMaxLimbs) return false;
-    L->D = (64 * m + 6) / 7;
-    L->P = (L->D + 31) / 32;
+    L->P = m / 4 + 1;
+    L->dmax = (8 * m + 31) / 32;  // (D + 30) / 32, D = 8m + 1
     L->PL = full ? 2 * m : m;
-    const uint32_t blocks_needed = (64 * L->PL + 223) / 224;  // output blocks below limb PL
-    L->NT = (blocks_needed + 31) / 32;
+    L->N = (L->PL + 3) / 4;
+    L->NT = (L->N + 31) / 32;
+    L->PLp = 4 * L->N + 4;  // the last block's 8-limb slot fits
     int64_t w = tune_get_class("mul.mfma_wpi", m, 0);
-    if (w <= 0) w = (L->NT + 2) / 4;  // measured best: about a quarter of the column tiles
+    if (w <= 0) w = (L->NT + 2) / 4;  // about a quarter of the column tiles
     L->wpi = (uint32_t)std::max<int64_t>(1, std::min<int64_t>({w, 16, (int64_t)L->NT}));
     L->ipb = std::max(1u, 4 / L->wpi);
     L->waves = L->ipb * L->wpi;
-    const uint64_t limbs = (uint64_t)L->ipb * L->PL;
+    const uint64_t limbs = (uint64_t)L->ipb * L->PL;  // the normalisation's row window
     uint32_t R = 1;
-    while ((uint64_t)L->waves * R * 128 < limbs) R *= 2;
+    while ((uint64_t)L->waves * R * kRowLimbs < limbs) R *= 2;
     if (R > 8) return false;
     L->R = R;
-    L->lds = (size_t)16 * L->ipb * L->PL + mf_a_bytes(L->P, L->ipb) + (size_t)L->ipb * mf_br_bytes(L->P);
-    // mf_div bounds: limb index x 2m, staging item x (P + 1) and x (P + 3)
-    const uint64_t limbs2 = (uint64_t)L->ipb * 2 * m;
-    const uint64_t items = 1 + (uint64_t)L->ipb * (2 * L->P + 4);
-    if (limbs2 * 2 * m >= (1ull << 32) || items * (L->P + 3) >= (1ull << 32)) return false;
+    L->lds = (size_t)16 * L->ipb * L->PLp + mf_a_bytes(L->P, L->ipb) + (size_t)L->ipb * mf_br_bytes(L->P);
+    // mf_div bound: window limb index (below waves R 128) x segment length
+    const uint64_t kmax = (uint64_t)L->waves * R * kRowLimbs + 2;
+    if (kmax * L->PL >= (1ull << 32)) return false;
     return L->lds <= 160 * 1024 - 256;
 }
 
-template <bool FULL, int R>
+template <int R>
 hipError_t launch_mfma_t(const MfmaLayout& L, uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b,
                          uint64_t* r, const Strides& st, hipStream_t s) {
-    const void* fn = (const void*)mul_mfma_kernel<FULL, R>;
+    const void* fn = (const void*)mul_mfma_kernel<R>;
     if (L.lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds);
         if (e != hipSuccess) return e;
     }
     const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
-    mul_mfma_kernel<FULL, R><<<blocks, 64 * L.waves, L.lds, s>>>(n, m, L.P, L.NT, L.wpi, L.ipb, L.PL,
-                                                                 mf_magic(2 * m), mf_magic(L.P + 1),
-                                                                 mf_magic(L.P + 3), a, b, r,
-                                                                 st.a ? st.a : m, st.b ? st.b : m,
-                                                                 st.r ? st.r : L.PL);
+    mul_mfma_kernel<R><<<blocks, 64 * L.waves, L.lds, s>>>(
+        n, m, L.P, L.dmax, L.N, L.NT, L.wpi, L.ipb, L.PL, L.PLp, mf_magic(L.PL), 256u, a, b, r, st.a ? st.a : m, st.b ? st.b : m, st.r ? st.r : L.PL);
     return hipGetLastError();
-}
-
-template <bool FULL>
-hipError_t launch_mfma_full_or_lo(const MfmaLayout& L, uint32_t n, uint32_t m, const uint64_t* a,
-                                  const uint64_t* b, uint64_t* r, const Strides& st, hipStream_t s) {
-    switch (L.R) {
-        case 1: return launch_mfma_t<FULL, 1>(L, n, m, a, b, r, st, s);
-        case 2: return launch_mfma_t<FULL, 2>(L, n, m, a, b, r, st, s);
-        case 4: return launch_mfma_t<FULL, 4>(L, n, m, a, b, r, st, s);
-        default: return launch_mfma_t<FULL, 8>(L, n, m, a, b, r, st, s);
-    }
 }
 
 }  // namespace
@@ -436,8 +456,12 @@ hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint
     MfmaLayout L;
     if (!mfma_layout(m, full, &L)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
-    return full ? launch_mfma_full_or_lo<true>(L, n, m, a, b, r, st, s)
-                : launch_mfma_full_or_lo<false>(L, n, m, a, b, r, st, s);
+    switch (L.R) {
+        case 1: return launch_mfma_t<1>(L, n, m, a, b, r, st, s);
+        case 2: return launch_mfma_t<2>(L, n, m, a, b, r, st, s);
+        case 4: return launch_mfma_t<4>(L, n, m, a, b, r, st, s);
+        default: return launch_mfma_t<8>(L, n, m, a, b, r, st, s);
+    }
 }
 
 }  // namespace midint

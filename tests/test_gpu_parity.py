@@ -185,6 +185,45 @@ def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, cfg):
         _check_mul(ctx, torch_dev, orc, m)
 
 
+MFMA_PATTERN_SIZES = [1, 3, 4, 5, 64, 100, 128, 129, 256, 777]
+
+
+@pytest.mark.parametrize("wpi", [0, 1, 2])
+@pytest.mark.parametrize("m", MFMA_PATTERN_SIZES)
+def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
+    """The matrix-core kernel writes operands in balanced base-256 digits
+    (byte + 128 with carries): bytes 0x7F make the carry run through the
+    whole operand, 0x80 and 0xFF bytes sit at the digit range's ends, and
+    mixed limbs break chains at random places."""
+    torch = torch_dev
+    rng = np.random.default_rng(m * 7 + wpi)
+    k7f = np.uint64(0x7F7F7F7F7F7F7F7F)
+    k80 = np.uint64(0x8080808080808080)
+    pool = np.array([k7f, k80, MASK, 0, np.uint64(0x7F7F7F7F7F7F7F80)], dtype=np.uint64)
+    rows = [
+        (np.full(m, k7f), np.full(m, k7f)),
+        (np.full(m, k80), np.full(m, MASK)),
+        (np.full(m, MASK), np.full(m, k7f)),
+        (np.full(m, k7f), rng.integers(0, 2**64, size=m, dtype=np.uint64)),
+        (pool[rng.integers(0, 5, size=m)], pool[rng.integers(0, 5, size=m)]),
+        (np.zeros(m, dtype=np.uint64), np.full(m, k80)),
+    ]
+    a = np.concatenate([r[0] for r in rows]).astype(np.uint64)
+    b = np.concatenate([r[1] for r in rows]).astype(np.uint64)
+    n = len(rows)
+    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi}):
+        da, db = dev(torch, a), dev(torch, b)
+        rf = empty(torch, 2 * n * m)
+        rl = empty(torch, n * m)
+        ctx.mul_full(da, db, rf, m)
+        ctx.mul_lo(da, db, rl, m)
+        torch.cuda.synchronize()
+    got = host(rf)
+    want = orc.mul_full(a, b, m)
+    assert np.array_equal(got, want), (m, int(np.flatnonzero(got != want)[0]))
+    assert np.array_equal(host(rl), orc.mul_lo(a, b, m))
+
+
 def _check_mul(ctx, torch_dev, orc, m):
     torch = torch_dev
     n = 6 if m >= 1024 else 11
