@@ -115,7 +115,9 @@ def tune_op(mi, torch, ctx, op, cls, cands, defaults, n, rng, log, sets=1, key_c
 
 
 def tune_ragged(mi, torch, ctx, log):
-    """Block size of the persistent ragged kernel on the 50000-op batch."""
+    """Ragged batch launch parameters on the 50000-op batch: block size of the
+    persistent kernel, the matrix-core threshold (shorter operand limbs) and
+    whether the two graph branches overlap. Returns the best as a dict."""
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
     from workloads import make_ragged
 
@@ -123,23 +125,32 @@ def tune_ragged(mi, torch, ctx, log):
     da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
     db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
     ref = None
-    best, best_ms = 256, None
+    best, best_ms = None, None
     for th in (256, 512):
-        dr = torch.zeros(rl, dtype=torch.int64, device="cuda:0")
-        plan = mi.RaggedPlan(ctx, ops, da, db, dr, threads=th)
-        plan.launch()
-        torch.cuda.synchronize()
-        if ref is None:
-            ref = dr.clone()
-        elif not torch.equal(dr, ref):
-            log(f"  ragged threads={th}: output differs -- rejected")
-            plan.close()
-            continue
-        ms = _timed(torch, plan.launch, 10)
-        plan.close()
-        log(f"  ragged threads={th}: {ms:.3f} ms")
-        if best_ms is None or ms < best_ms:
-            best, best_ms = th, ms
+        for mf_min in (0, 32, 64, 128):
+            for ov in (0, 1):
+                if mf_min == 0 and ov == 1:
+                    continue
+                cand = {"ragged.threads": th, "ragged.mfma_min": mf_min, "ragged.overlap": ov}
+                for k, v in cand.items():
+                    mi.tune_set(k, v)
+                dr = torch.zeros(rl, dtype=torch.int64, device="cuda:0")
+                plan = mi.RaggedPlan(ctx, ops, da, db, dr, threads=th)
+                plan.launch()
+                torch.cuda.synchronize()
+                if ref is None:
+                    ref = dr.clone()
+                elif not torch.equal(dr, ref):
+                    log(f"  ragged {cand}: output differs -- rejected")
+                    plan.close()
+                    continue
+                ms = _timed(torch, plan.launch, 10)
+                plan.close()
+                log(f"  ragged {cand}: {ms:.3f} ms")
+                if best_ms is None or ms < best_ms:
+                    best, best_ms = cand, ms
+    for k, v in best.items():
+        mi.tune_set(k, v)
     return best
 
 
@@ -203,7 +214,7 @@ def main(argv=None):
                           key_cls=L)
         lines += [f"{k}@{L} {v}" for k, v in best.items()]
 
-    lines.append(f"ragged.threads {tune_ragged(mi, torch, ctx, log)}")
+    lines += [f"{k} {v}" for k, v in tune_ragged(mi, torch, ctx, log).items()]
 
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:

@@ -39,6 +39,28 @@ hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t*
 bool mul_mfma_supported(uint32_t m, bool full);
 hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                            hipStream_t s, Strides st = {});
+// Ragged multiplications on the matrix cores: one product (n_a >= n_b limbs,
+// full n_a + n_b limb result) per workgroup of kMfRgWaves waves.
+constexpr uint32_t kMfRgWaves = 4;
+struct MfRgItem {
+    const uint64_t* a;  // n_a limbs (the longer operand)
+    const uint64_t* b;  // n_b limbs
+    uint64_t* r;        // n_a + n_b limbs
+    uint32_t n_a, n_b;
+    uint32_t mg_pl;  // reciprocal of n_a + n_b for the kernel's divisions
+    uint32_t pad;
+};
+MfRgItem mfma_rg_item(const uint64_t* a, const uint64_t* b, uint64_t* r, uint32_t n_a, uint32_t n_b);
+// Launch configuration for items with n_a <= max_na, n_b <= max_nb and
+// n_a + n_b <= max_pl; the kernel's arguments are (const MfRgItem* items,
+// uint32_t pow8), one workgroup per item. False when the sizes do not fit.
+struct MfRgConfig {
+    const void* func;
+    uint32_t threads;
+    size_t lds;
+    uint32_t pow8;
+};
+bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out);
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ).
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                        uint64_t* scratch, hipStream_t s, Strides st = {});

@@ -27,16 +27,19 @@
 // (4N >= output limbs), so the lowest column of block 0 starts at 256 B
 // instead and the blocks sum to the product exactly.
 //
-// LDS: a's digits as 32-byte blocks, the instances of a workgroup one after
-// another with a zero block before, between and after them (B operand: one
-// aligned 16-byte read per lane of block med3(J, -1, P), so a block index
-// outside the instance reads zeros and its MFMA adds nothing; the loop needs
-// no per-tile range tests), and per instance b's digits reversed, br[y] =
+// LDS: a's digits as blocks, the instances of a workgroup one after another
+// with a zero block before, between and after them, each block's two 16-byte
+// K halves in two separate arrays (B operand: one aligned 16-byte read per
+// lane of block med3(J, -1, P), so a block index outside the instance reads
+// zeros and its MFMA adds nothing and the loop needs no per-tile range
+// tests; with the halves apart, 16 lanes read 256 contiguous bytes, free of
+// bank conflicts), and per instance b's digits reversed, br[y] =
 // b_digit[X0 - y] with X0 = 32P + 31 (A operand: row r of T_d is
 // br[y0 .. y0+15], y0 = X0 - 32d - r + 16kg; the misalignment of y0 is fixed
 // per lane, so each row is five aligned dword reads and four funnel shifts).
 // Limb j of an operand is digits 8j .. 8j+7: one 8-byte word of the a region
-// and one byte-swapped 8-byte word of br, so staging needs no bit shuffling.
+// (half (j / 2) mod 2 of block j / 4) and one byte-swapped 8-byte word of br,
+// so staging needs no bit shuffling.
 //
 // Column sums to limbs: lane (n, kg) of a tile holds the 16 sums of output
 // block I = I0 + n at rows 8q + 4kg + t (q, t < 4), i.e. at bits
@@ -170,13 +173,13 @@ template <int NTL>
 __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t wpi, uint32_t P, uint32_t dmax,
                                         uint32_t N, const MfPow& pw) {
     int32_t dlo = (int32_t)dmax, dhi = 0;
-    uint32_t bb[NTL];  // LDS byte address of the lane's B operand at d = 0
+    uint32_t bb[NTL];  // LDS byte address of the lane's B operand at d = 0 (16 bytes per block)
 #pragma unroll
     for (int k = 0; k < NTL; k++) {
         const int32_t I0 = 32 * (int32_t)(c0 + k * wpi);
         dlo = min(dlo, max(0, I0 - (int32_t)P + 1));
         dhi = max(dhi, min((int32_t)dmax, I0 + 31));
-        bb[k] = L.b_base + (uint32_t)(32 * (I0 + (int32_t)L.n));
+        bb[k] = L.b_base + (uint32_t)(16 * (I0 + (int32_t)L.n));
     }
     dlo = __builtin_amdgcn_readfirstlane(dlo);  // wave-uniform: keep d scalar
     dhi = __builtin_amdgcn_readfirstlane(dhi);
@@ -200,7 +203,7 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
 #pragma unroll
         for (int k = 0; k < NTL; k++) {
             // block J = J0 - d, clamped to the zero blocks at -1 and P
-            const uint32_t addr = mf_med3(bb[k] - 32u * (uint32_t)d, L.b_lo, L.b_hi);
+            const uint32_t addr = mf_med3(bb[k] - 16u * (uint32_t)d, L.b_lo, L.b_hi);
             bq[k] = *reinterpret_cast<const mf_v4i*>(L.lds8 + addr);
         }
     };
@@ -229,28 +232,38 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
     for (int k = 0; k < NTL; k++) mf_store_tile(acc[k], 32 * (c0 + k * wpi) + L.n, L.kg, N, L.E, L.O, L.paddr, pw);
 }
 
-template <int R>
-__global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t dmax,
-                                                        uint32_t N, uint32_t NT, uint32_t wpi, uint32_t ipb,
-                                                        uint32_t PL, uint32_t PLp, uint32_t mg_pl, uint32_t pow8,
-                                                        const uint64_t* __restrict__ a,
-                                                        const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
-                                                        uint64_t sa, uint64_t sb, uint64_t sr) {
-    extern __shared__ __align__(16) uint64_t lds64[];
-    uint8_t* lds8 = reinterpret_cast<uint8_t*>(lds64);
-    __shared__ uint32_t lds_gp[16];
+// Sizes of the products of one workgroup (wave-uniform). a is the block
+// operand, b the Toeplitz operand (for unbalanced products the shorter one:
+// a tile's d range is at most dmax + 1 steps).
+struct MfGeom {
+    uint32_t na, nb;   // operand limbs
+    uint32_t Pa, Pb;   // digit blocks, floor(n / 4) + 1
+    uint32_t dmax;     // last d with a nonzero T_d, floor((8 nb + 31) / 32)
+    uint32_t N, NT;    // output blocks below limb PL, column tiles
+    uint32_t wpi, ipb; // waves per instance, instances per workgroup
+    uint32_t PL, PLp;  // output limbs, LDS stride of E and O per instance
+    uint32_t mg_pl;    // mf_magic(PL)
+};
 
+// One workgroup: ninst (<= ipb) products, instance s of a at a + s sa, of b
+// at b + s sb, of the result at r + s sr.
+template <int R>
+__device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const uint64_t* __restrict__ a,
+                                         uint64_t sa, const uint64_t* __restrict__ b, uint64_t sb,
+                                         uint64_t* __restrict__ r, uint64_t sr, uint32_t pow8, uint64_t* lds64,
+                                         uint32_t* lds_gp) {
+    uint8_t* lds8 = reinterpret_cast<uint8_t*>(lds64);
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63;
-    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
-    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
-    const uint32_t X0 = 32 * P + 31;
-    const uint32_t A_BYTES = mf_a_bytes(P, ipb), BR_BYTES = mf_br_bytes(P);
-    const uint32_t SA = 4 * (P + 1), SB = 4 * (P + 3);  // a / br words per instance
+    const uint32_t ipb = g.ipb;
+    const uint32_t X0 = 32 * g.Pb + 31;
+    const uint32_t A_BYTES = mf_a_bytes(g.Pa, ipb), BR_BYTES = mf_br_bytes(g.Pb);
+    const uint32_t SB = 4 * (g.Pb + 3);  // br words per instance
+    const uint32_t AH = A_BYTES / 16;     // u64 words of each half array of the a region
     uint64_t* Eb = lds64;
-    uint64_t* Ob = lds64 + (size_t)ipb * PLp;
-    uint8_t* Areg = lds8 + (size_t)16 * ipb * PLp;  // a blocks of all instances
-    uint8_t* BRreg = Areg + A_BYTES;                 // br blocks, P + 3 per instance
+    uint64_t* Ob = lds64 + (size_t)ipb * g.PLp;
+    uint8_t* Areg = lds8 + (size_t)16 * ipb * g.PLp;  // a blocks of all instances
+    uint8_t* BRreg = Areg + A_BYTES;                   // br blocks, Pb + 3 per instance
     uint64_t* A64 = reinterpret_cast<uint64_t*>(Areg);
     uint64_t* BR64 = reinterpret_cast<uint64_t*>(BRreg);
     const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane;
@@ -258,58 +271,68 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
     // ---- stage digits: one wave per operand (operand o < ipb: a of instance
     // o, else b of instance o - ipb), rows of 128 limbs in order with the
     // carry between rows in a scalar: x = limbs + 0x8080...80, digits =
-    // x ^ 0x80 per byte below limb m, the carry at limb m and zeros above, up
-    // to limb SA (the instance's a blocks and its trailing zero block). a's
-    // words go to the a region in order, b's to br in reverse order (word
+    // x ^ 0x80 per byte below limb n, the carry at limb n and zeros above, up
+    // to limb 4 (P + 1) (the blocks and, for a, the trailing zero block). a's
+    // words go to the a region (a_word), b's to br in reverse order (word
     // SB - 9 - j for limb j, after br's 8 top zero words). The next row's
     // loads are issued before the current row is processed.
     {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6), waves = blockDim.x >> 6;
-        const uint32_t nops = 2 * ipb, nrow = (m + kRowLimbs - 1) / kRowLimbs;
-        if (tid < 4) A64[tid] = 0;  // the zero block before instance 0
+        const uint32_t nops = 2 * ipb;
+        const uint32_t nrow_a = (g.na + kRowLimbs - 1) / kRowLimbs, nrow_b = (g.nb + kRowLimbs - 1) / kRowLimbs;
+        // u64 word of limb j of instance s in the a region (block 1 + s (Pa + 1)
+        // + j / 4, half (j / 2) mod 2)
+        auto a_word = [&](uint32_t s, uint32_t j) {
+            return ((j >> 1) & 1u) * AH + 2 * (1 + s * (g.Pa + 1) + (j >> 2)) + (j & 1u);
+        };
+        if (tid < 4) A64[(tid >> 1) * AH + (tid & 1u)] = 0;  // the zero block before instance 0
         auto fetch = [&](uint32_t o, uint32_t row, uint64_t& x0, uint64_t& x1) {
             const bool isb = o >= ipb;
             const uint32_t s = isb ? o - ipb : o;
-            const uint64_t* __restrict__ src = (isb ? b : a) + (inst_base + s) * (isb ? sb : sa);
+            const uint32_t n = isb ? g.nb : g.na;
+            const uint64_t* __restrict__ src = isb ? b + s * sb : a + s * sa;
             const uint32_t j0 = row * kRowLimbs + 2 * lane;
-            x0 = s < ninst && j0 < m ? src[j0] : 0ull;
-            x1 = s < ninst && j0 + 1 < m ? src[j0 + 1] : 0ull;
+            x0 = s < ninst && j0 < n ? src[j0] : 0ull;
+            x1 = s < ninst && j0 + 1 < n ? src[j0 + 1] : 0ull;
         };
         uint32_t o = wv, row = 0, cin = 0;
         uint64_t nx0 = 0, nx1 = 0;
         if (o < nops) fetch(o, 0, nx0, nx1);
         while (o < nops) {
             const uint64_t x0 = nx0, x1 = nx1;
+            const bool isb = o >= ipb;
+            const uint32_t nrow = isb ? nrow_b : nrow_a;
             uint32_t on = o, rn = row + 1;
             if (rn == nrow) {
                 rn = 0;
                 on = o + waves;
             }
             if (on < nops) fetch(on, rn, nx0, nx1);
-            const bool isb = o >= ipb;
             const uint32_t s = isb ? o - ipb : o;
+            const uint32_t n = isb ? g.nb : g.na;
+            const uint32_t S4 = 4 * ((isb ? g.Pb : g.Pa) + 1);  // limbs written
             const uint32_t j0 = row * kRowLimbs + 2 * lane;
-            const RowState st = row_build(x0, j0 < m ? kMfHalf : 0ull, x1, j0 + 1 < m ? kMfHalf : 0ull, false, false);
+            const RowState st = row_build(x0, j0 < n ? kMfHalf : 0ull, x1, j0 + 1 < n ? kMfHalf : 0ull, false, false);
             const uint64_t C = row_chain(st, &cin);
             uint64_t r0, r1;
             row_apply(st, C, &r0, &r1);
-            const uint64_t w0 = j0 < m ? r0 ^ kMfHalf : r0;
-            const uint64_t w1 = j0 + 1 < m ? r1 ^ kMfHalf : r1;
-            if (j0 < SA) {  // SA and j0 even: both limbs
+            const uint64_t w0 = j0 < n ? r0 ^ kMfHalf : r0;
+            const uint64_t w1 = j0 + 1 < n ? r1 ^ kMfHalf : r1;
+            if (j0 < S4) {  // S4 and j0 even: both limbs
                 if (!isb)
-                    *reinterpret_cast<ulonglong2*>(A64 + 4 + s * SA + j0) = make_ulonglong2(w0, w1);
+                    *reinterpret_cast<ulonglong2*>(A64 + a_word(s, j0)) = make_ulonglong2(w0, w1);
                 else
                     *reinterpret_cast<ulonglong2*>(BR64 + s * SB + SB - 10 - j0) =
                         make_ulonglong2(__builtin_bswap64(w1), __builtin_bswap64(w0));
             }
             if (rn == 0) {
                 // limbs past the rows (at most 8): the carry when the rows
-                // end at limb m, zeros after; and br's top zero words
+                // end at limb n, zeros after; and br's top zero words
                 const uint32_t jt = nrow * kRowLimbs + lane;
-                if (jt < SA) {
-                    const uint64_t w = jt == m ? (uint64_t)cin : 0ull;
+                if (jt < S4) {
+                    const uint64_t w = jt == n ? (uint64_t)cin : 0ull;
                     if (!isb)
-                        A64[4 + s * SA + jt] = w;
+                        A64[a_word(s, jt)] = w;
                     else
                         BR64[s * SB + SB - 9 - jt] = __builtin_bswap64(w);
                 }
@@ -328,45 +351,46 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
         const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
         // instance and first tile of the wave, kept scalar (the division
         // would otherwise leave every loop bound in vector registers)
-        const uint32_t s = __builtin_amdgcn_readfirstlane(wv / wpi);
-        const uint32_t wi = __builtin_amdgcn_readfirstlane(wv - s * wpi);
+        const uint32_t s = __builtin_amdgcn_readfirstlane(wv / g.wpi);
+        const uint32_t wi = __builtin_amdgcn_readfirstlane(wv - s * g.wpi);
         MfLane L;
         L.lds8 = lds8;
         L.lane = lane;
         L.n = lane & 31;
         L.kg = lane >> 5;
         L.paddr = (lane ^ 32u) << 2;
-        const uint8_t* Ab = Areg + (size_t)(1 + s * (P + 1)) * 32;  // block 0 of instance s
-        L.b_base = (uint32_t)(Ab - lds8) + 16 * L.kg;
-        L.b_lo = L.b_base - 32;
-        L.b_hi = L.b_base + 32 * P;
+        // block 0 of instance s in the lane's half array
+        L.b_base = (uint32_t)(Areg - lds8) + L.kg * (A_BYTES / 2) + 16 * (1 + s * (g.Pa + 1));
+        L.b_lo = L.b_base - 16;
+        L.b_hi = L.b_base + 16 * g.Pa;
         const uint32_t* BR = reinterpret_cast<const uint32_t*>(BRreg + (size_t)s * BR_BYTES);
         const uint32_t ybase = X0 - L.n + 16 * L.kg;  // y0 at d = 0
         L.sh = (ybase & 3) * 8;
         L.Abase = BR + (ybase >> 2);
-        L.E = Eb + (size_t)s * PLp;
-        L.O = Ob + (size_t)s * PLp;
+        L.E = Eb + (size_t)s * g.PLp;
+        L.O = Ob + (size_t)s * g.PLp;
         MfPow pw;
         pw.p8 = pow8;
         pw.p16 = pow8 * pow8;
         pw.p24 = pw.p16 * pow8;
-        for (uint32_t c0 = wi; c0 < NT; c0 += kMfTiles * wpi) {
-            if (c0 + wpi < NT)
-                mf_pass<2>(L, c0, wpi, P, dmax, N, pw);
+        for (uint32_t c0 = wi; c0 < g.NT; c0 += kMfTiles * g.wpi) {
+            if (c0 + g.wpi < g.NT)
+                mf_pass<2>(L, c0, g.wpi, g.Pa, g.dmax, g.N, pw);
             else
-                mf_pass<1>(L, c0, wpi, P, dmax, N, pw);
+                mf_pass<1>(L, c0, g.wpi, g.Pa, g.dmax, g.N, pw);
         }
     }
     __syncthreads();
 
     // ---- normalisation: E + O over ipb segments of PL limbs (instance
     // stride PLp in LDS), R rows per wave
+    const uint32_t PL = g.PL, PLp = g.PLp;
     const uint32_t Lw = ipb * PL;
     Rows<R> rows;
     uint32_t off0[R], instv[R];
     {
-        SegPos sp = mf_segpos(k_first, PL, mg_pl);
-        const uint32_t sq = mf_div(kRowLimbs, mg_pl), srr = kRowLimbs - sq * PL;
+        SegPos sp = mf_segpos(k_first, PL, g.mg_pl);
+        const uint32_t sq = mf_div(kRowLimbs, g.mg_pl), srr = kRowLimbs - sq * PL;
 #pragma unroll
         for (int v = 0; v < R; v++) {
             const uint32_t k = k_first + v * kRowLimbs;
@@ -390,12 +414,53 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
         const uint32_t k = k_first + v * kRowLimbs;
         const uint32_t o0 = off0[v];
         const uint32_t i0 = instv[v];
-        if (k < Lw && i0 < ninst) r[(inst_base + i0) * sr + o0] = rows.x0[v];
+        if (k < Lw && i0 < ninst) r[i0 * sr + o0] = rows.x0[v];
         const bool wrap = (o0 + 1 == PL);
         const uint32_t o1 = wrap ? 0u : o0 + 1;
         const uint32_t i1 = i0 + (wrap ? 1u : 0u);
-        if (k + 1 < Lw && i1 < ninst) r[(inst_base + i1) * sr + o1] = rows.x1[v];
+        if (k + 1 < Lw && i1 < ninst) r[i1 * sr + o1] = rows.x1[v];
     }
+}
+
+// Batches of equal-size products: ipb instances per workgroup.
+template <int R>
+__global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t dmax,
+                                                        uint32_t N, uint32_t NT, uint32_t wpi, uint32_t ipb,
+                                                        uint32_t PL, uint32_t PLp, uint32_t mg_pl, uint32_t pow8,
+                                                        const uint64_t* __restrict__ a,
+                                                        const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                        uint64_t sa, uint64_t sb, uint64_t sr) {
+    extern __shared__ __align__(16) uint64_t lds64[];
+    __shared__ uint32_t lds_gp[16];
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+    const MfGeom g = {m, m, P, P, dmax, N, NT, wpi, ipb, PL, PLp, mg_pl};
+    mf_block<R>(g, ninst, a + inst_base * sa, sa, b + inst_base * sb, sb, r + inst_base * sr, sr, pow8, lds64,
+                lds_gp);
+}
+
+// Ragged batches: one product per workgroup (items[blockIdx.x], n_a >= n_b),
+// all waves on its column tiles.
+template <int R>
+__global__ __launch_bounds__(64 * kMfRgWaves) void mul_mfma_rg_kernel(const MfRgItem* __restrict__ items,
+                                                                      uint32_t pow8) {
+    extern __shared__ __align__(16) uint64_t lds64[];
+    __shared__ uint32_t lds_gp[16];
+    const MfRgItem it = items[blockIdx.x];
+    MfGeom g;
+    g.na = it.n_a;
+    g.nb = it.n_b;
+    g.Pa = g.na / 4 + 1;
+    g.Pb = g.nb / 4 + 1;
+    g.dmax = (8 * g.nb + 31) / 32;
+    g.PL = g.na + g.nb;
+    g.N = (g.PL + 3) / 4;
+    g.NT = (g.N + 31) / 32;
+    g.PLp = 4 * g.N + 4;
+    g.wpi = kMfRgWaves;
+    g.ipb = 1;
+    g.mg_pl = it.mg_pl;
+    mf_block<R>(g, 1, it.a, 0, it.b, 0, it.r, 0, pow8, lds64, lds_gp);
 }
 
 namespace {
@@ -439,8 +504,9 @@ hipError_t launch_mfma_t(const MfmaLayout& L, uint32_t n, uint32_t m, const uint
         if (e != hipSuccess) return e;
     }
     const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
-    mul_mfma_kernel<R><<<blocks, 64 * L.waves, L.lds, s>>>(
-        n, m, L.P, L.dmax, L.N, L.NT, L.wpi, L.ipb, L.PL, L.PLp, mf_magic(L.PL), 256u, a, b, r, st.a ? st.a : m, st.b ? st.b : m, st.r ? st.r : L.PL);
+    mul_mfma_kernel<R><<<blocks, 64 * L.waves, L.lds, s>>>(n, m, L.P, L.dmax, L.N, L.NT, L.wpi, L.ipb, L.PL, L.PLp,
+                                                           mf_magic(L.PL), 256u, a, b, r, st.a ? st.a : m,
+                                                           st.b ? st.b : m, st.r ? st.r : L.PL);
     return hipGetLastError();
 }
 
@@ -462,6 +528,37 @@ hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint
         case 4: return launch_mfma_t<4>(L, n, m, a, b, r, st, s);
         default: return launch_mfma_t<8>(L, n, m, a, b, r, st, s);
     }
+}
+
+MfRgItem mfma_rg_item(const uint64_t* a, const uint64_t* b, uint64_t* r, uint32_t n_a, uint32_t n_b) {
+    MfRgItem it = {};
+    it.a = a;
+    it.b = b;
+    it.r = r;
+    it.n_a = n_a;
+    it.n_b = n_b;
+    it.mg_pl = mf_magic(n_a + n_b);
+    return it;
+}
+
+bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out) {
+    if (max_na == 0 || max_nb == 0 || max_na > kMulMaxLimbs || max_nb > max_na || max_pl > max_na + max_nb)
+        return false;
+    uint32_t R = 1;
+    while (kMfRgWaves * R * kRowLimbs < max_pl) R *= 2;
+    if (R > 8) return false;
+    const uint32_t N = (max_pl + 3) / 4, Pa = max_na / 4 + 1, Pb = max_nb / 4 + 1;
+    out->lds = (size_t)16 * (4 * N + 4) + mf_a_bytes(Pa, 1) + mf_br_bytes(Pb);
+    if (out->lds > 160 * 1024 - 256) return false;
+    out->threads = 64 * kMfRgWaves;
+    out->pow8 = 256u;
+    switch (R) {
+        case 1: out->func = (const void*)mul_mfma_rg_kernel<1>; break;
+        case 2: out->func = (const void*)mul_mfma_rg_kernel<2>; break;
+        case 4: out->func = (const void*)mul_mfma_rg_kernel<4>; break;
+        default: out->func = (const void*)mul_mfma_rg_kernel<8>; break;
+    }
+    return true;
 }
 
 }  // namespace midint

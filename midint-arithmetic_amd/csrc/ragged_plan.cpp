@@ -1,6 +1,8 @@
 // Host side of ragged batches: packing ops into work items, upload, and the
-// hipGraph (counter reset node -> persistent kernel node) replayed by
-// midint_ragged_launch.
+// hipGraph replayed by midint_ragged_launch: counter reset node -> matrix-core
+// node (multiplications whose shorter operand has at least ragged.mfma_min
+// limbs, one product per workgroup, largest first) -> persistent kernel node
+// (every other op; with ragged.overlap = 1 it depends on the reset only).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,7 +20,10 @@ struct midint_ragged_plan {
     RgItem* d_items = nullptr;
     RgOp* d_ops = nullptr;
     uint32_t* d_counter = nullptr;
+    MfRgItem* d_mf_items = nullptr;
     uint32_t n_items = 0;
+    uint32_t n_mf_items = 0;
+    MfRgConfig mf = {};
     uint32_t threads = 256;
     uint32_t conv_cap = 0;
     uint32_t grid = 0;
@@ -43,6 +48,7 @@ void free_plan(midint_ragged_plan* p) {
     if (p->d_items) (void)hipFree(p->d_items);
     if (p->d_ops) (void)hipFree(p->d_ops);
     if (p->d_counter) (void)hipFree(p->d_counter);
+    if (p->d_mf_items) (void)hipFree(p->d_mf_items);
     delete p;
 }
 
@@ -166,10 +172,38 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
                                         midint_ragged_plan** out) {
     if (!ctx || !out || (n_ops > 0 && (!ops || !d_a || !d_b || !d_r))) return MIDINT_EINVAL;
     *out = nullptr;
+    // multiplications for the matrix-core node: shorter operand >= mfma_min
+    // limbs (0: none); the rest go to the persistent kernel
+    const int64_t mf_min = tune_get("ragged.mfma_min", 32);
+    std::vector<midint_op> rest;
+    std::vector<MfRgItem> h_mf;
+    uint32_t mf_na = 0, mf_nb = 0, mf_pl = 0;
+    rest.reserve(n_ops);
+    for (size_t i = 0; i < n_ops; i++) {
+        const midint_op& o = ops[i];
+        const bool ok = o.reserved == 0 && o.kind == MIDINT_OP_MUL && o.n_a <= kRgMaxLimbs && o.n_b <= kRgMaxLimbs;
+        if (ok && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
+            const bool sw = o.n_a < o.n_b;  // the longer operand is the kernel's a
+            const uint64_t* x = sw ? d_b + o.off_b : d_a + o.off_a;
+            const uint64_t* y = sw ? d_a + o.off_a : d_b + o.off_b;
+            const uint32_t nx = (uint32_t)(sw ? o.n_b : o.n_a), ny = (uint32_t)(sw ? o.n_a : o.n_b);
+            h_mf.push_back(mfma_rg_item(x, y, d_r + o.off_r, nx, ny));
+            mf_na = std::max(mf_na, nx);
+            mf_nb = std::max(mf_nb, ny);
+            mf_pl = std::max(mf_pl, nx + ny);
+        } else {
+            rest.push_back(o);
+        }
+    }
+    std::stable_sort(h_mf.begin(), h_mf.end(), [](const MfRgItem& x, const MfRgItem& y) {
+        return (uint64_t)x.n_a * x.n_b > (uint64_t)y.n_a * y.n_b;
+    });
+    MfRgConfig mf = {};
+    if (!h_mf.empty() && !mfma_rg_config(mf_na, mf_nb, mf_pl, &mf)) return MIDINT_EUNSUPPORTED;
     std::vector<RgItem> h_items;
     std::vector<RgOp> h_ops;
     uint32_t conv_cap = 0;
-    midint_status st = ragged_pack(ops, n_ops, 0, 0, 0, h_items, h_ops, &conv_cap);
+    midint_status st = ragged_pack(rest.data(), rest.size(), 0, 0, 0, h_items, h_ops, &conv_cap);
     if (st != MIDINT_OK) return st;
 
     midint_ragged_plan* p = new (std::nothrow) midint_ragged_plan;
@@ -182,6 +216,8 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     const int64_t th = (tuning && tuning->threads) ? tuning->threads : tune_get("ragged.threads", 256);
     p->threads = th == 512 ? 512 : 256;
     p->n_items = (uint32_t)h_items.size();
+    p->n_mf_items = (uint32_t)h_mf.size();
+    p->mf = mf;
     p->conv_cap = conv_cap;
     p->lds = ragged_lds_size(p->threads, p->conv_cap);
     p->A = d_a;
@@ -195,6 +231,11 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         e = hipMemcpy(p->d_items, h_items.data(), h_items.size() * sizeof(RgItem), hipMemcpyHostToDevice);
     if (e == hipSuccess && !h_ops.empty())
         e = hipMemcpy(p->d_ops, h_ops.data(), h_ops.size() * sizeof(RgOp), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !h_mf.empty()) e = hipMalloc((void**)&p->d_mf_items, h_mf.size() * sizeof(MfRgItem));
+    if (e == hipSuccess && !h_mf.empty())
+        e = hipMemcpy(p->d_mf_items, h_mf.data(), h_mf.size() * sizeof(MfRgItem), hipMemcpyHostToDevice);
+    if (e == hipSuccess && !h_mf.empty() && mf.lds > 64 * 1024)
+        e = hipFuncSetAttribute(mf.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mf.lds);
     const void* kfn = ragged_kernel_ptr(p->threads);
     if (e == hipSuccess && p->lds > 64 * 1024)
         e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
@@ -210,9 +251,10 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     if (tuning && tuning->blocks_per_cu > 0 && (int)tuning->blocks_per_cu < per_cu) per_cu = (int)tuning->blocks_per_cu;
     p->grid = (uint32_t)std::max(1, std::min<int>(per_cu * cus, (int)std::max<uint32_t>(1, p->n_items)));
 
-    // ---- graph: reset the work counter, then the persistent kernel
+    // ---- graph: reset the work counter, the matrix-core products, then the
+    // persistent kernel
     e = hipGraphCreate(&p->graph, 0);
-    hipGraphNode_t n_reset = nullptr, n_main = nullptr;
+    hipGraphNode_t n_reset = nullptr, n_mf = nullptr, n_main = nullptr;
     if (e == hipSuccess) {
         void* rargs[] = {(void*)&p->d_counter};
         hipKernelNodeParams kp = {};
@@ -224,7 +266,22 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         kp.extra = nullptr;
         e = hipGraphAddKernelNode(&n_reset, p->graph, nullptr, 0, &kp);
     }
-    if (e == hipSuccess) {
+    hipGraphNode_t n_prev = n_reset;
+    if (e == hipSuccess && p->n_mf_items > 0) {
+        void* margs[] = {(void*)&p->d_mf_items, (void*)&p->mf.pow8};
+        hipKernelNodeParams kp = {};
+        kp.func = const_cast<void*>(p->mf.func);
+        kp.gridDim = dim3(p->n_mf_items);
+        kp.blockDim = dim3(p->mf.threads);
+        kp.sharedMemBytes = (unsigned)p->mf.lds;
+        kp.kernelParams = margs;
+        kp.extra = nullptr;
+        e = hipGraphAddKernelNode(&n_mf, p->graph, &n_reset, 1, &kp);
+        // ragged.overlap = 1: the persistent kernel does not wait for the
+        // matrix-core node (parallel branches after the reset)
+        if (tune_get("ragged.overlap", 1) == 0) n_prev = n_mf;
+    }
+    if (e == hipSuccess && p->n_items > 0) {
         void* kargs[] = {(void*)&p->d_items, (void*)&p->d_ops, (void*)&p->n_items, (void*)&p->d_counter,
                          (void*)&p->A,       (void*)&p->B,     (void*)&p->R,       (void*)&p->conv_cap};
         hipKernelNodeParams kp = {};
@@ -234,7 +291,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         kp.sharedMemBytes = (unsigned)p->lds;
         kp.kernelParams = kargs;
         kp.extra = nullptr;
-        e = hipGraphAddKernelNode(&n_main, p->graph, &n_reset, 1, &kp);
+        e = hipGraphAddKernelNode(&n_main, p->graph, &n_prev, 1, &kp);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
@@ -247,7 +304,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
 
 midint_status midint_ragged_launch(midint_ragged_plan* p, void* stream) {
     if (!p) return MIDINT_EINVAL;
-    if (p->n_items == 0) return MIDINT_OK;
+    if (p->n_items == 0 && p->n_mf_items == 0) return MIDINT_OK;
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess) return MIDINT_EHIP;
     if (cur != p->device && hipSetDevice(p->device) != hipSuccess) return MIDINT_EHIP;
@@ -257,7 +314,7 @@ midint_status midint_ragged_launch(midint_ragged_plan* p, void* stream) {
 midint_status midint_ragged_plan_stats(const midint_ragged_plan* p, uint64_t* n_items, uint64_t* grid,
                                        uint64_t* lds_bytes) {
     if (!p) return MIDINT_EINVAL;
-    if (n_items) *n_items = p->n_items;
+    if (n_items) *n_items = (uint64_t)p->n_items + p->n_mf_items;
     if (grid) *grid = p->grid;
     if (lds_bytes) *lds_bytes = p->lds;
     return MIDINT_OK;

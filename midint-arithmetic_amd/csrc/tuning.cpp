@@ -3,8 +3,8 @@
 //
 //   mul.q         columns per lane group of the classical kernel (4, 8 or 16)
 //   mul.rpark     1: park the first column group in registers, not LDS (Q >= 8)
-//   mul.mfma      1: int8 matrix-core classical kernel (7-bit digits)
-//   mul.mfma_wpi  waves per instance of that kernel (0: half the column tiles)
+//   mul.mfma      1: int8 matrix-core classical kernel (balanced 8-bit digits)
+//   mul.mfma_wpi  waves per instance of that kernel (0: a quarter of the column tiles)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)
@@ -16,6 +16,9 @@
 //   ntt.vals      values per thread of that kernel, per transform length
 //                 ("ntt.vals@L"): 16 or 32, 0 for the built-in choice
 //   ragged.threads  256 or 512
+//   ragged.mfma_min ragged multiplications whose shorter operand has at least
+//                 this many limbs run on the matrix-core kernel (0: none)
+//   ragged.overlap  1: the persistent kernel runs beside the matrix-core node
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -34,7 +37,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"ntt.threads", 0}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256},
+        {"ntt.threads", 0}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
     };
     return t;
 }

@@ -121,3 +121,51 @@ def test_ragged_full_workload(mi, ctx, torch_dev, orc):
                 assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])
             else:
                 assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])
+
+
+@pytest.mark.parametrize("mfma_min", [0, 1, 16])
+def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min):
+    """Multiplications whose shorter operand has at least ragged.mfma_min
+    limbs run on the matrix-core graph node (one product per workgroup, the
+    longer operand as the block operand), the rest on the persistent
+    kernel: 0 sends none there, 1 sends every one (1 x 1 and 1 x 1024
+    products included)."""
+    ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=400, seed=31, lo_bits=6, hi_bits=16)
+    old = mi.tune_get("ragged.mfma_min")
+    mi.tune_set("ragged.mfma_min", mfma_min)
+    try:
+        got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
+    finally:
+        mi.tune_set("ragged.mfma_min", old)
+    want = ragged_reference(orc, ops, a, b, rl)
+    assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
+
+
+def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc):
+    """Every multiplication on the matrix-core node: unbalanced and tiny
+    shapes, all-ones and 0x7F-byte operands (the longest digit carry chain)."""
+    K7F = np.uint64(0x7F7F7F7F7F7F7F7F)
+    MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+    shapes = [(1, 1), (1, 1024), (1024, 1), (1024, 1024), (3, 5), (1023, 1021), (4, 4), (511, 513), (128, 129),
+              (1000, 7), (64, 900)]
+    ops, a_parts, b_parts = [], [], []
+    oa = ob = orr = 0
+    rng = np.random.default_rng(5)
+    for i, (na, nb) in enumerate(shapes * 3):
+        fill = (MASK, K7F, None)[i % 3]
+        x = np.full(na, fill, dtype=np.uint64) if fill is not None else rng.integers(0, 2**64, na, dtype=np.uint64)
+        y = np.full(nb, fill, dtype=np.uint64) if fill is not None else rng.integers(0, 2**64, nb, dtype=np.uint64)
+        ops.append((1, na, nb, oa, ob, orr))
+        a_parts.append(x)
+        b_parts.append(y)
+        oa += na
+        ob += nb
+        orr += na + nb
+    a, b = np.concatenate(a_parts), np.concatenate(b_parts)
+    old = mi.tune_get("ragged.mfma_min")
+    mi.tune_set("ragged.mfma_min", 1)
+    try:
+        got = run_plan(mi, ctx, torch_dev, ops, a, b, orr)
+    finally:
+        mi.tune_set("ragged.mfma_min", old)
+    assert np.array_equal(got, ragged_reference(orc, ops, a, b, orr))
