@@ -100,11 +100,9 @@ struct MfPow {
     uint32_t p8, p16, p24;
 };
 
-// Folds a finished tile into the E / O slots of its 32 output blocks (only
-// blocks below N are written).
-__device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint32_t kg, uint32_t N, uint64_t* E,
-                                              uint64_t* O, uint32_t paddr, const MfPow& pw) {
-    uint32_t g[8];
+// A finished tile's 16 column sums of the lane as the words of
+// sum_q G_q 2^(64q), G_q = sum_t c_{4q+t} 2^(8t) < 2^55.
+__device__ __forceinline__ void mf_fold(const mf_v16i& c, const MfPow& pw, uint32_t (&g)[8]) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
         uint64_t G = (uint64_t)(uint32_t)c[4 * q];
@@ -114,10 +112,12 @@ __device__ __forceinline__ void mf_store_tile(const mf_v16i& c, uint32_t I, uint
         g[2 * q] = (uint32_t)G;
         g[2 * q + 1] = (uint32_t)(G >> 32);
     }
-    // the partner lane's words (kg = 1 rows, 32 bits up)
-    uint32_t p[8];
-#pragma unroll
-    for (int i = 0; i < 8; i++) p[i] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)paddr, (int)g[i]);
+}
+
+// Writes block I's E / O slot (only blocks below N) from the lane's words g
+// and its partner's words p (the kg = 1 rows, 32 bits up).
+__device__ __forceinline__ void mf_store_block(const uint32_t (&g)[8], const uint32_t (&p)[8], uint32_t I,
+                                               uint32_t kg, uint32_t N, uint64_t* E, uint64_t* O) {
     uint32_t w1, w2, w3, w4, w5, w6, w7, w8;
     uint64_t cy;
     asm("v_add_co_u32_e64 %[w1], %[c], %[g1], %[p0]\n\t"
@@ -228,8 +228,32 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
         }
         if (steps & 1) mma(qa0, bq0);
     }
+    // column sums to E / O: both tiles' partner exchanges in flight together
+    uint32_t gw[NTL][8], pw8[NTL][8];
 #pragma unroll
-    for (int k = 0; k < NTL; k++) mf_store_tile(acc[k], 32 * (c0 + k * wpi) + L.n, L.kg, N, L.E, L.O, L.paddr, pw);
+    for (int k = 0; k < NTL; k++) mf_fold(acc[k], pw, gw[k]);
+#pragma unroll
+    for (int k = 0; k < NTL; k++) {
+#pragma unroll
+        for (int i = 0; i < 8; i++) pw8[k][i] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)L.paddr, (int)gw[k][i]);
+    }
+#pragma unroll
+    for (int k = 0; k < NTL; k++) mf_store_block(gw[k], pw8[k], 32 * (c0 + k * wpi) + L.n, L.kg, N, L.E, L.O);
+}
+
+// x <- x + y per segment with carries inside the wave only: for windows
+// where every wave's R rows hold whole segments (no barrier, no exchange).
+template <int R>
+__device__ __forceinline__ void wave_add_rows(Rows<R>& t) {
+    RowState st[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) st[v] = row_build(t.x0[v], t.y0[v], t.x1[v], t.y1[v], t.s0[v], t.s1[v]);
+    uint32_t c = 0;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint64_t C = row_chain(st[v], &c);
+        row_apply(st[v], C, &t.x0[v], &t.x1[v]);
+    }
 }
 
 // Sizes of the products of one workgroup (wave-uniform). a is the block
@@ -408,7 +432,11 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
             segpos_next(&sp, PL, sq, srr);
         }
     }
-    block_add_rows<R>(rows, lds_gp);
+    // R 128-limb rows a multiple of PL: each wave holds whole products
+    if ((R * kRowLimbs) % PL == 0)
+        wave_add_rows<R>(rows);
+    else
+        block_add_rows<R>(rows, lds_gp);
 #pragma unroll
     for (int v = 0; v < R; v++) {
         const uint32_t k = k_first + v * kRowLimbs;
@@ -481,7 +509,9 @@ bool mfma_layout(uint32_t m, bool full, MfmaLayout* L) {
     int64_t w = tune_get_class("mul.mfma_wpi", m, 0);
     if (w <= 0) w = (L->NT + 2) / 4;  // about a quarter of the column tiles
     L->wpi = (uint32_t)std::max<int64_t>(1, std::min<int64_t>({w, 16, (int64_t)L->NT}));
-    L->ipb = std::max(1u, 4 / L->wpi);
+    const int64_t ipb = tune_get_class("mul.mfma_ipb", m, 0);  // instances per workgroup, 0: 4 / wpi
+    L->ipb = ipb > 0 ? (uint32_t)std::min<int64_t>(ipb, 16) : std::max(1u, 4 / L->wpi);
+    if (L->ipb * L->wpi > 16) return false;
     L->waves = L->ipb * L->wpi;
     const uint64_t limbs = (uint64_t)L->ipb * L->PL;  // the normalisation's row window
     uint32_t R = 1;
