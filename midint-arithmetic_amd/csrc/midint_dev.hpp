@@ -222,6 +222,18 @@ __device__ __forceinline__ uint64_t gl_reduce(uint32_t l0, uint32_t l1, uint32_t
     return join32(x0, x1);
 }
 
+// x * 2^s for 0 < s < 64: the 128-bit shift, then the reduction. Meant for
+// compile-time s (the shifts and word selects fold away).
+__device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, int s) {
+    const uint32_t x0 = lo32(x), x1 = hi32(x);
+    if (s < 32) {
+        return gl_reduce(x0 << s, __builtin_amdgcn_alignbit(x1, x0, 32 - s), x1 >> (32 - s), 0u);
+    } else if (s == 32) {
+        return gl_reduce(0u, x0, x1, 0u);
+    }
+    return gl_reduce(0u, x0 << (s - 32), __builtin_amdgcn_alignbit(x1, x0, 64 - s), x1 >> (64 - s));
+}
+
 // 64x64 -> 128 from four v_mad_u64_u32 (no partial sum can overflow 64
 // bits), then the reduction.
 __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {

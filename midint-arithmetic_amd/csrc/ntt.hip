@@ -369,7 +369,11 @@ static uint64_t h_pow(uint64_t a, uint64_t e) {
 
 void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_len) {
     const uint64_t L = 1ull << lg;
-    const uint64_t w = h_pow(7, (kGlP - 1) / L);  // 7 generates the multiplicative group
+    // 7 generates the multiplicative group; 7^((p-1)/64) = 8^13 and
+    // 13 * 5 = 1 (mod 64), so w = 7^(5 (p-1) / L) has w^(L/64) = 8 and the
+    // twiddles of spans h <= 32 are powers of two (2^96 = -1 mod p):
+    // w^(j L / 2h) = 2^(96 j / h). The kernels' last passes rely on it.
+    const uint64_t w = h_pow(h_pow(7, (kGlP - 1) / L), 5);
     const uint64_t wi = h_pow(w, kGlP - 2);
     uint64_t f = 1, g = 1;
     for (uint64_t i = 0; i < L / 2; i++) {

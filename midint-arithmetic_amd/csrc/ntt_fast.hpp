@@ -60,6 +60,11 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
                 if (k & (1 << jj)) continue;
                 const uint32_t klo = (uint32_t)(k & ((1 << jj) - 1));
                 const bool unit = lgmin == 0 && klo == 0;  // w^0
+                // last pass (h_min = 1): the twiddle of span h = 2^lgh is
+                // w^(klo L / 2h) = 2^(sh), sh = (96 / h) klo (the root has
+                // w^(L/64) = 8), a shift instead of a table multiply
+                const int lgh = LG - 1 - s;
+                const int sh = lgmin == 0 && lgh <= 5 ? (96 >> lgh) * (int)klo : -1;
                 uint64_t& a = v[j * K + k];
                 uint64_t& b = v[j * K + k + (1 << jj)];
                 if (!INV) {
@@ -67,7 +72,14 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
                     const bool zero_x = ZHI && S0 == 0 && s == 0;
                     a = zero_x ? u : gl_add(u, x);
                     const uint64_t d = zero_x ? u : gl_sub(u, x);
-                    b = unit ? d : gl_mul(d, tw[(g_lo + (klo << lgmin)) << s]);
+                    b = unit           ? d
+                        : sh > 0 && sh < 64 ? gl_mul_pow2(d, sh)
+                                            : gl_mul(d, tw[(g_lo + (klo << lgmin)) << s]);
+                } else if (sh > 32) {
+                    // w^-e = 2^(192 - sh) = -2^(96 - sh): swap the add and the sub
+                    const uint64_t u = a, y = gl_mul_pow2(b, 96 - sh);
+                    a = gl_sub(u, y);
+                    b = gl_add(u, y);
                 } else {
                     const uint64_t u = a, x = unit ? b : gl_mul(b, tw[(g_lo + (klo << lgmin)) << s]);
                     a = gl_add(u, x);
