@@ -74,7 +74,7 @@ __host__ __device__ constexpr uint32_t mf_br_bytes(uint32_t P) { return (P + 3) 
 // floor(t / d) from mg = ceil(2^32 / d) (mf_magic, on the host) whenever
 // t * d < 2^32: t mg / 2^32 = t / d + t e / (d 2^32) with e = mg d - 2^32 < d,
 // and t e < 2^32 keeps the excess below 1 / d, which cannot carry past the
-// next multiple of 1 / d. mfma_layout checks the bound for every division.
+// next multiple of 1 / d. mfma_layout and mfma_rg_config check the bound.
 // d = 1 (2^32 does not fit) is encoded as mg = 0.
 __device__ __forceinline__ uint32_t mf_div(uint32_t t, uint32_t mg) { return mg ? __umulhi(t, mg) : t; }
 __host__ constexpr uint32_t mf_magic(uint32_t d) { return d == 1 ? 0u : (uint32_t)((0xFFFFFFFFull + d) / d); }
@@ -577,6 +577,8 @@ bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfi
     uint32_t R = 1;
     while (kMfRgWaves * R * kRowLimbs < max_pl) R *= 2;
     if (R > 8) return false;
+    // mf_div bound of the normalisation: window limb index x PL < 2^32
+    if (((uint64_t)kMfRgWaves * R * kRowLimbs + 2) * max_pl >= (1ull << 32)) return false;
     const uint32_t N = (max_pl + 3) / 4, Pa = max_na / 4 + 1, Pb = max_nb / 4 + 1;
     out->lds = (size_t)16 * (4 * N + 4) + mf_a_bytes(Pa, 1) + mf_br_bytes(Pb);
     if (out->lds > 160 * 1024 - 256) return false;
