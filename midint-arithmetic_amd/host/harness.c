@@ -15,8 +15,11 @@
  * validateExact :71-81). Output: the reference's printf lines
  * (cuda/main.cu:112-115, 307-310, 513-519) followed by one JSON line each.
  *
- * usage: midint-harness [total_work_u32_words] [device]
- *        default total_work = 2^27 (cuda/Makefile:25)
+ * usage: midint-harness [total_work_u32_words] [device] [tune_file | none]
+ *        default total_work = 2^27 (cuda/Makefile:25); the launch-parameter
+ *        table defaults to ../tune/mi355x.txt next to this binary (the
+ *        autotuned choice of kernel per size class), "none" keeps the
+ *        library's built-in defaults
  */
 #define _GNU_SOURCE
 #include <dlfcn.h>
@@ -24,6 +27,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 #include "midint.h"
 
@@ -137,11 +141,34 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int 
     return ok != 0;
 }
 
+/* Loads the launch-parameter table: `arg`, or the one shipped beside the
+ * executable (../tune/mi355x.txt); "none" keeps the built-in defaults. */
+static void load_tuning(const char* arg) {
+    char path[4096];
+    if (arg && strcmp(arg, "none") == 0) {
+        printf("tuning: built-in defaults\n");
+        return;
+    }
+    if (arg) {
+        snprintf(path, sizeof path, "%s", arg);
+    } else {
+        const ssize_t n = readlink("/proc/self/exe", path, sizeof path - 1);
+        if (n <= 0) return;
+        path[n] = 0;
+        char* slash = strrchr(path, '/');
+        if (!slash) return;
+        snprintf(slash, sizeof path - (size_t)(slash - path), "/../tune/mi355x.txt");
+    }
+    const midint_status st = midint_tune_load(path);
+    printf("tuning: %s (%s)\n", path, st == MIDINT_OK ? "loaded" : midint_strerror(st));
+}
+
 int main(int argc, char** argv) {
     uint64_t total_work = 1ull << 27;
     int device = 0;
     if (argc > 1) total_work = strtoull(argv[1], NULL, 10);
     if (argc > 2) device = atoi(argv[2]);
+    load_tuning(argc > 3 ? argv[3] : NULL);
     const int gmp = load_gmp();
     if (!gmp) printf("libgmp.so.10 not found: results are not validated\n");
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64, 32, 16};
