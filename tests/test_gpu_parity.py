@@ -125,8 +125,8 @@ MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256,
 # Classical-kernel configurations (tuning keys): u32 words with mad + carry
 # at Q = 4, 8, 16 columns per lane group, the first group parked in LDS or
 # in registers, with and without operand prefetch; 28-bit digits with one
-# mad per product; 7-bit digits on the int8 matrix cores (automatic and one
-# wave per instance).
+# mad per product; balanced 8-bit digits on the int8 matrix cores (automatic
+# and one wave per instance).
 MUL_CFGS = [
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 1},
