@@ -24,8 +24,10 @@
 //
 // Several micro-batches are in flight on different streams, so work of
 // requests admitted later overlaps on the device with work admitted earlier.
-// Kernels per micro-batch: the ragged persistent kernel for ops whose
-// operands are <= 1024 limbs; the add kernel for larger additions (padded
+// Kernels per micro-batch: for ops whose operands are <= 1024 limbs, the
+// matrix-core kernel (one product per workgroup) for multiplications whose
+// shorter operand has at least ragged.mfma_min limbs and the ragged
+// persistent kernel for the rest; the add kernel for larger additions (padded
 // to a 2048- or 4096-limb class, carry-out kept); the NTT kernel for larger
 // products (padded to 2048 limbs, or split into four 2048-limb products plus
 // the combine kernel above 2048 limbs). Every path is exact, so a request's
@@ -295,10 +297,36 @@ midint_status launch_batch(midint_server* s, size_t si) {
     out += n_ntt * 2 * kNttM;
     const size_t carry_bytes = align_up(add2k.size() + add4k.size(), 8);
     const size_t items_off = in;  // limbs
-    midint_status ps = ragged_pack(rops.data(), rops.size(), 0, 0, 0, items, rgops, &conv_cap);
+    // ragged multiplications whose shorter operand has at least
+    // ragged.mfma_min limbs go to the matrix-core kernel (one product per
+    // workgroup), the rest to the persistent kernel
+    const int64_t mf_min = tune_get("ragged.mfma_min", 32);
+    std::vector<midint_op> pack_ops;
+    std::vector<size_t> mf_idx;
+    uint32_t mf_na = 0, mf_nb = 0;
+    for (size_t k = 0; k < rops.size(); k++) {
+        const midint_op& o = rops[k];
+        if (o.kind == MIDINT_OP_MUL && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
+            mf_idx.push_back(k);
+            mf_na = std::max(mf_na, (uint32_t)std::max(o.n_a, o.n_b));
+            mf_nb = std::max(mf_nb, (uint32_t)std::min(o.n_a, o.n_b));
+        } else {
+            pack_ops.push_back(o);
+        }
+    }
+    MfRgConfig mf = {};
+    if (!mf_idx.empty() && !mfma_rg_config(mf_na, mf_nb, mf_na + mf_nb, &mf)) {
+        for (size_t k : mf_idx) pack_ops.push_back(rops[k]);  // no fitting configuration
+        mf_idx.clear();
+    }
+    std::stable_sort(mf_idx.begin(), mf_idx.end(), [&](size_t x, size_t y) {
+        return (uint64_t)rops[x].n_a * rops[x].n_b > (uint64_t)rops[y].n_a * rops[y].n_b;
+    });
+    midint_status ps = ragged_pack(pack_ops.data(), pack_ops.size(), 0, 0, 0, items, rgops, &conv_cap);
     if (ps != MIDINT_OK) return ps;
-    const size_t meta_limbs =
-        align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 8) / 8 + 1 /* counter */;
+    // metadata: [RgItem | RgOp | MfRgItem (16-byte aligned) | counter limb]
+    const size_t mf_meta_off = align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 16);
+    const size_t meta_limbs = align_up(mf_meta_off + mf_idx.size() * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
     in += meta_limbs;
     in = align_up(in, 32);
     const size_t in_bytes = in * 8;
@@ -366,14 +394,27 @@ midint_status launch_batch(midint_server* s, size_t si) {
             }
         }
     }
-    uint8_t* meta = reinterpret_cast<uint8_t*>(h + items_off);
-    memcpy(meta, items.data(), items.size() * sizeof(RgItem));
-    memcpy(meta + items.size() * sizeof(RgItem), rgops.data(), rgops.size() * sizeof(RgOp));
-
     // ---- device pointers
     uint8_t* dreg = s->pool.base + off;
     uint64_t* d_in = reinterpret_cast<uint64_t*>(dreg);
     uint64_t* d_out = reinterpret_cast<uint64_t*>(dreg + in_bytes);
+
+    uint8_t* meta = reinterpret_cast<uint8_t*>(h + items_off);
+    memcpy(meta, items.data(), items.size() * sizeof(RgItem));
+    memcpy(meta + items.size() * sizeof(RgItem), rgops.data(), rgops.size() * sizeof(RgOp));
+    {
+        // matrix-core items: device addresses inside this batch's region,
+        // the longer operand first
+        MfRgItem* mfi = reinterpret_cast<MfRgItem*>(meta + mf_meta_off);
+        for (size_t q = 0; q < mf_idx.size(); q++) {
+            const midint_op& o = rops[mf_idx[q]];
+            const bool sw = o.n_a < o.n_b;
+            const uint64_t* x = sw ? d_in + rg_b + o.off_b : d_in + rg_a + o.off_a;
+            const uint64_t* y = sw ? d_in + rg_a + o.off_a : d_in + rg_b + o.off_b;
+            mfi[q] = mfma_rg_item(x, y, d_out + rg_r + o.off_r, (uint32_t)(sw ? o.n_b : o.n_a),
+                                  (uint32_t)(sw ? o.n_a : o.n_b));
+        }
+    }
     uint8_t* d_carry = dreg + in_bytes + out * 8;
     RgItem* d_items = reinterpret_cast<RgItem*>(d_in + items_off);
     RgOp* d_rgops = reinterpret_cast<RgOp*>(reinterpret_cast<uint8_t*>(d_items) + items.size() * sizeof(RgItem));
@@ -395,6 +436,15 @@ midint_status launch_batch(midint_server* s, size_t si) {
         const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(n_items, s->cus * s->ragged_per_cu));
         if (e == hipSuccess)
             e = hipLaunchKernel(ragged_kernel_ptr(256), dim3(grid), dim3(256), args, lds, st);
+    }
+    if (e == hipSuccess && !mf_idx.empty()) {
+        MfRgItem* d_mf = reinterpret_cast<MfRgItem*>(reinterpret_cast<uint8_t*>(d_in + items_off) + mf_meta_off);
+        uint32_t pow8 = mf.pow8;
+        void* args[] = {(void*)&d_mf, (void*)&pow8};
+        if (mf.lds > 64 * 1024)
+            e = hipFuncSetAttribute(mf.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mf.lds);
+        if (e == hipSuccess)
+            e = hipLaunchKernel(mf.func, dim3((uint32_t)mf_idx.size()), dim3(mf.threads), args, mf.lds, st);
     }
     if (e == hipSuccess && !add2k.empty())
         e = launch_add((uint32_t)add2k.size(), 2048, d_in + a2k, d_in + a2k + add2k.size() * 2048, d_out + a2k_r,
@@ -498,8 +548,9 @@ void retire(midint_server* s, Slot& sl, double t, std::vector<uint64_t>& done) {
 // slot (measured: ~6.5 ms stalls of the submit loop). Counters are reset
 // afterwards, so a fresh server reports only live traffic.
 midint_status warm_up(midint_server* s) {
-    const uint32_t shapes[][3] = {{MIDINT_OP_MUL, 8, 8},    {MIDINT_OP_ADD, 8, 8},     {MIDINT_OP_ADD, 2048, 8},
-                                  {MIDINT_OP_ADD, 4096, 8}, {MIDINT_OP_MUL, 2048, 8}, {MIDINT_OP_MUL, 4096, 8}};
+    const uint32_t shapes[][3] = {{MIDINT_OP_MUL, 8, 8},       {MIDINT_OP_MUL, 64, 64},  {MIDINT_OP_MUL, 1024, 1024},
+                                  {MIDINT_OP_ADD, 8, 8},       {MIDINT_OP_ADD, 2048, 8}, {MIDINT_OP_ADD, 4096, 8},
+                                  {MIDINT_OP_MUL, 2048, 8},    {MIDINT_OP_MUL, 4096, 8}};
     std::vector<uint64_t> ones(kServeMaxLimbs, 1);
     std::vector<uint64_t> done;
     for (size_t si = 0; si < s->slots.size(); si++) {

@@ -78,3 +78,21 @@ def test_budget_defers_not_drops(mi, ctx, orc):
     assert st["deferred_admissions"] > 0
     for j, (k, a, b) in enumerate(reqs):
         assert np.array_equal(out["results"][j], request_reference(orc, k, a, b))
+
+
+@pytest.mark.parametrize("mfma_min", [0, 1])
+def test_matrix_core_route_threshold(mi, ctx, orc, mfma_min):
+    """Multiplications up to 1024 limbs whose shorter operand has at least
+    ragged.mfma_min limbs run on the matrix-core kernel inside the micro-batch
+    (0: none, 1: all); results equal the oracle's either way."""
+    reqs = make_requests(mi.gen_limbs, 150, seed=17)
+    old = mi.tune_get("ragged.mfma_min")
+    mi.tune_set("ragged.mfma_min", mfma_min)
+    try:
+        srv = mi.Server(ctx, streams=2)
+        out = mi.serve_poisson(srv, reqs, qps=20000, seed=6)["results"]
+        srv.close()
+    finally:
+        mi.tune_set("ragged.mfma_min", old)
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out[j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
