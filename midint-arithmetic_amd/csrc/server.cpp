@@ -28,10 +28,11 @@
 // matrix-core kernel (one product per workgroup) for multiplications whose
 // shorter operand has at least ragged.mfma_min limbs and the ragged
 // persistent kernel for the rest; the add kernel for larger additions (padded
-// to a 2048- or 4096-limb class, carry-out kept); the NTT kernel for larger
-// products (padded to 2048 limbs, or split into four 2048-limb products plus
-// the combine kernel above 2048 limbs). Every path is exact, so a request's
-// result does not depend on what it was batched with.
+// to a 2048- or 4096-limb class, carry-out kept); the matrix-core kernel for
+// larger products (one workgroup each up to 2048 limbs, or four full
+// 2048-limb quarter products plus the combine kernel above 2048 limbs; the
+// NTT kernel when no launch configuration fits). Every path is exact, so a
+// request's result does not depend on what it was batched with.
 #include <hip/hip_runtime.h>
 #include <string.h>
 #include <time.h>
@@ -324,9 +325,17 @@ midint_status launch_batch(midint_server* s, size_t si) {
     });
     midint_status ps = ragged_pack(pack_ops.data(), pack_ops.size(), 0, 0, 0, items, rgops, &conv_cap);
     if (ps != MIDINT_OK) return ps;
-    // metadata: [RgItem | RgOp | MfRgItem (16-byte aligned) | counter limb]
+    // products above the ragged limit: matrix-core items over the padded
+    // 2048-limb operand slots (one per 2048-class product, four full-size
+    // quarter products per split product), with the NTT kernel as the
+    // fallback when no configuration fits
+    MfRgConfig bigmf = {};
+    const bool big_mf = n_ntt > 0 && mfma_rg_config(kNttM, kNttM, 2 * kNttM, &bigmf);
+    // metadata: [RgItem | RgOp | MfRgItem (16-byte aligned) | big MfRgItem | counter limb]
     const size_t mf_meta_off = align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 16);
-    const size_t meta_limbs = align_up(mf_meta_off + mf_idx.size() * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
+    const size_t big_meta_off = mf_meta_off + mf_idx.size() * sizeof(MfRgItem);
+    const size_t meta_limbs =
+        align_up(big_meta_off + (big_mf ? n_ntt : 0) * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
     in += meta_limbs;
     in = align_up(in, 32);
     const size_t in_bytes = in * 8;
@@ -414,6 +423,25 @@ midint_status launch_batch(midint_server* s, size_t si) {
             mfi[q] = mfma_rg_item(x, y, d_out + rg_r + o.off_r, (uint32_t)(sw ? o.n_b : o.n_a),
                                   (uint32_t)(sw ? o.n_a : o.n_b));
         }
+        if (big_mf) {
+            // instance inst: operands at X / Y + inst kNttM (zero-padded),
+            // product at ntt_r + inst 2 kNttM
+            MfRgItem* bi = reinterpret_cast<MfRgItem*>(meta + big_meta_off);
+            const uint64_t* X = d_in + nttx;
+            const uint64_t* Y = d_in + nttx + n_ntt * kNttM;
+            uint64_t* Z = d_out + ntt_r;
+            size_t inst = 0;
+            for (uint64_t id : ntt2k) {
+                const Request& r = s->reqs[id];
+                const bool sw = r.na < r.nb;
+                const uint64_t* x = (sw ? Y : X) + inst * kNttM;
+                const uint64_t* y = (sw ? X : Y) + inst * kNttM;
+                bi[inst] = mfma_rg_item(x, y, Z + inst * 2 * kNttM, sw ? r.nb : r.na, sw ? r.na : r.nb);
+                inst++;
+            }
+            for (; inst < n_ntt; inst++)  // the split's quarters: all 2 kNttM limbs feed the combine
+                bi[inst] = mfma_rg_item(X + inst * kNttM, Y + inst * kNttM, Z + inst * 2 * kNttM, kNttM, kNttM);
+        }
     }
     uint8_t* d_carry = dreg + in_bytes + out * 8;
     RgItem* d_items = reinterpret_cast<RgItem*>(d_in + items_off);
@@ -452,16 +480,24 @@ midint_status launch_batch(midint_server* s, size_t si) {
     if (e == hipSuccess && !add4k.empty())
         e = launch_add((uint32_t)add4k.size(), 4096, d_in + a4k, d_in + a4k + add4k.size() * 4096, d_out + a4k_r,
                        d_carry + add2k.size(), false, st);
-    if (e == hipSuccess && n_ntt > 0) {
+    if (e == hipSuccess && n_ntt > 0 && big_mf) {
+        MfRgItem* d_big = reinterpret_cast<MfRgItem*>(reinterpret_cast<uint8_t*>(d_in + items_off) + big_meta_off);
+        uint32_t pow8 = bigmf.pow8;
+        void* args[] = {(void*)&d_big, (void*)&pow8};
+        if (bigmf.lds > 64 * 1024)
+            e = hipFuncSetAttribute(bigmf.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bigmf.lds);
+        if (e == hipSuccess)
+            e = hipLaunchKernel(bigmf.func, dim3((uint32_t)n_ntt), dim3(bigmf.threads), args, bigmf.lds, st);
+    } else if (e == hipSuccess && n_ntt > 0) {
         NttTables t;
         midint_status ts = ctx_ntt_tables(s->ctx, ntt_lg_for_limbs(kNttM), &t);
         if (ts != MIDINT_OK) e = hipErrorUnknown;
         if (e == hipSuccess)
             e = launch_mul_ntt((uint32_t)n_ntt, kNttM, d_in + nttx, d_in + nttx + n_ntt * kNttM, d_out + ntt_r, t,
                                st);
-        if (e == hipSuccess && !ntt4k.empty())
-            e = launch_bigmul_combine((uint32_t)ntt4k.size(), d_out + ntt_r + ntt2k.size() * 2 * kNttM, st);
     }
+    if (e == hipSuccess && !ntt4k.empty())
+        e = launch_bigmul_combine((uint32_t)ntt4k.size(), d_out + ntt_r + ntt2k.size() * 2 * kNttM, st);
     if (e == hipSuccess) e = launch_stage_copy(d_out, sl.pinned_dev + in_bytes, out_bytes, st);
     if (e == hipSuccess) e = hipEventRecord(sl.ev, st);
     if (e != hipSuccess) {
