@@ -2,6 +2,7 @@
 include/midint.h declares, reports errors without a GPU call, and its
 host-side generators agree with the oracle's."""
 import ctypes
+import os
 import subprocess
 
 import numpy as np
@@ -62,8 +63,6 @@ def test_generators_match_oracle(mi, orc):
 
 
 def test_harness_binary_built(mi):
-    import os
-
     assert os.access(os.path.join(mi.HERE, "bin", "midint-harness"), os.X_OK)
 
 
@@ -84,6 +83,18 @@ def test_tuning_table(mi, tmp_path):
     assert L.midint_tune_load(str(f).encode()) == 6
     mi.tune_set("add.rows", old)
     mi.tune_set("add.nt", 1)
+
+
+def test_shipped_tuning_table_loads(mi):
+    """Every key of the shipped table is one the library knows (an unknown
+    key fails the load), including the matrix-core and ragged-graph keys."""
+    path = os.path.join(mi.HERE, "tune", "mi355x.txt")
+    keys = [ln.split()[0] for ln in open(path) if ln.strip() and not ln.startswith("#")]
+    assert {"mul.mfma@128", "ragged.mfma_min", "ragged.overlap"} <= set(keys)
+    saved = {k: mi.tune_get(k) for k in ("ragged.mfma_min", "ragged.overlap", "mul.mfma_ipb")}
+    assert mi.tune_load(path)
+    for k, v in saved.items():
+        mi.tune_set(k, v)
 
 
 def test_new_entry_points_reject_bad_arguments(mi):
