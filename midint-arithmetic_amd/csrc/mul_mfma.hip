@@ -316,8 +316,13 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
             const uint32_t n = isb ? g.nb : g.na;
             const uint64_t* __restrict__ src = isb ? b + s * sb : a + s * sa;
             const uint32_t j0 = row * kRowLimbs + 2 * lane;
-            x0 = s < ninst && j0 < n ? src[j0] : 0ull;
-            x1 = s < ninst && j0 + 1 < n ? src[j0 + 1] : 0ull;
+            if (s < ninst && (row + 1) * kRowLimbs <= n) {  // a whole row (wave-uniform)
+                x0 = src[j0];
+                x1 = src[j0 + 1];
+            } else {
+                x0 = s < ninst && j0 < n ? src[j0] : 0ull;
+                x1 = s < ninst && j0 + 1 < n ? src[j0 + 1] : 0ull;
+            }
         };
         uint32_t o = wv, row = 0, cin = 0;
         uint64_t nx0 = 0, nx1 = 0;
@@ -412,9 +417,30 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
     const uint32_t Lw = ipb * PL;
     Rows<R> rows;
     uint32_t off0[R], instv[R];
+    // the rows cover exactly the workgroup's ninst = ipb products of even
+    // length: every limb pair is in range and inside one product, so the
+    // pairs load and store without checks (E and O pairs are 16-byte aligned)
+    const bool full = (blockDim.x >> 6) * R * kRowLimbs == Lw && ninst == ipb && (PL & 1u) == 0;
     {
         SegPos sp = mf_segpos(k_first, PL, g.mg_pl);
         const uint32_t sq = mf_div(kRowLimbs, g.mg_pl), srr = kRowLimbs - sq * PL;
+        if (full) {
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                const uint32_t e0 = sp.inst * PLp + sp.off;
+                const ulonglong2 xe = *reinterpret_cast<const ulonglong2*>(Eb + e0);
+                const ulonglong2 yo = *reinterpret_cast<const ulonglong2*>(Ob + e0);
+                rows.x0[v] = xe.x;
+                rows.x1[v] = xe.y;
+                rows.y0[v] = yo.x;
+                rows.y1[v] = yo.y;
+                rows.s0[v] = sp.off == 0;
+                rows.s1[v] = false;
+                off0[v] = sp.off;
+                instv[v] = sp.inst;
+                segpos_next(&sp, PL, sq, srr);
+            }
+        } else
 #pragma unroll
         for (int v = 0; v < R; v++) {
             const uint32_t k = k_first + v * kRowLimbs;
@@ -437,6 +463,15 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
         wave_add_rows<R>(rows);
     else
         block_add_rows<R>(rows, lds_gp);
+    if (full) {
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            uint64_t* dst = r + instv[v] * sr + off0[v];
+            dst[0] = rows.x0[v];
+            dst[1] = rows.x1[v];
+        }
+        return;
+    }
 #pragma unroll
     for (int v = 0; v < R; v++) {
         const uint32_t k = k_first + v * kRowLimbs;
