@@ -20,12 +20,13 @@
 // sum leaves the matrix core.
 //
 // Bias. A column sum is a sum of at most D digit products in
-// [-16256, 16384]. Every accumulator starts at 255 B (B = 2^21), which makes
-// each column sum non-negative and below 2^30 for m <= 4096 limbs. That
+// [-16256, 16384]. The accumulators start at zero and hold signed sums; when
+// a tile is folded into limbs, 255 B (B = 2^21) is added to every column
+// sum, which makes it non-negative and below 2^30 for m <= 4096 limbs. That
 // adds B (2^256 - 1) to every block value; over the N output blocks the
 // total is B 2^(256 N) - B, and 2^(256 N) is zero modulo the output length
-// (4N >= output limbs), so the lowest column of block 0 starts at 256 B
-// instead and the blocks sum to the product exactly.
+// (4N >= output limbs), so the lowest column of block 0 gets 256 B instead
+// and the blocks sum to the product exactly.
 //
 // LDS: a's digits as blocks, the instances of a workgroup one after another
 // with a zero block before, between and after them, each block's two 16-byte
@@ -43,10 +44,11 @@
 //
 // Column sums to limbs: lane (n, kg) of a tile holds the 16 sums of output
 // block I = I0 + n at rows 8q + 4kg + t (q, t < 4), i.e. at bits
-// 64q + 32kg + 8t of the block. G_q = sum_t c_{8q+4kg+t} 2^(8t) < 2^55
-// (three v_mad_u64_u32), so the lane's part is sum_q G_q 2^(64q) shifted up
-// by 32 kg bits, with no carries inside the lane. The kg = 0 lane takes its
-// partner's G_q (ds_bpermute) and adds them 32 bits up in one 9-word carry
+// 64q + 32kg + 8t of the block. G_q = sum_t (c_{8q+4kg+t} + 255 B) 2^(8t)
+// < 2^55 (four v_mad_i64_i32 onto the bias of the four columns), so the
+// lane's part is sum_q G_q 2^(64q) shifted up by 32 kg bits, with no
+// carries inside the lane. The kg = 0 lane takes its partner's G_q
+// (ds_bpermute) and adds them 32 bits up in one 9-word carry
 // chain: the block value V < 2^279 (5 limbs). Even blocks write vector E,
 // odd blocks vector O, each an 8-limb slot at limb 4I (the kg = 1 lane
 // writes its 3 zero limbs); the slots of each vector tile it and the product
@@ -93,24 +95,30 @@ __device__ __forceinline__ uint32_t mf_med3(uint32_t x, uint32_t lo, uint32_t hi
     return r;
 }
 
-// 2^8, 2^16, 2^24 in SGPRs the compiler cannot see through (built from a
-// kernel argument), so that c * 2^k + G stays one v_mad_u64_u32 instead of
-// a 64-bit shift and add.
+// 1, 2^8, 2^16, 2^24 in SGPRs the compiler cannot see through (built from
+// a kernel argument), so that c * 2^k + G stays one v_mad_i64_i32 instead
+// of a 64-bit shift and add.
 struct MfPow {
-    uint32_t p8, p16, p24;
+    uint32_t p0, p8, p16, p24;
 };
 
-// A finished tile's 16 column sums of the lane as the words of
-// sum_q G_q 2^(64q), G_q = sum_t c_{4q+t} 2^(8t) < 2^55.
-__device__ __forceinline__ void mf_fold(const mf_v16i& c, const MfPow& pw, uint32_t (&g)[8]) {
+// The bias of one G_q: 255 B in each of its four columns.
+constexpr uint64_t kMfFoldBias = (uint64_t)(255u * kMfBias) * 0x01010101ull;
+
+// A finished tile's 16 signed column sums of the lane (the accumulators
+// start at zero) as the words of sum_q G_q 2^(64q), G_q = sum_t (c_{4q+t} +
+// 255 B) 2^(8t) < 2^55; bias0 is G_0's bias (one B more in the lowest
+// column of block 0).
+__device__ __forceinline__ void mf_fold(const mf_v16i& c, const MfPow& pw, uint64_t bias0, uint32_t (&g)[8]) {
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-        uint64_t G = (uint64_t)(uint32_t)c[4 * q];
-        G += (uint64_t)(uint32_t)c[4 * q + 1] * pw.p8;
-        G += (uint64_t)(uint32_t)c[4 * q + 2] * pw.p16;
-        G += (uint64_t)(uint32_t)c[4 * q + 3] * pw.p24;
+        int64_t G = (int64_t)(q == 0 ? bias0 : kMfFoldBias);
+        G += (int64_t)c[4 * q] * (int32_t)pw.p0;
+        G += (int64_t)c[4 * q + 1] * (int32_t)pw.p8;
+        G += (int64_t)c[4 * q + 2] * (int32_t)pw.p16;
+        G += (int64_t)c[4 * q + 3] * (int32_t)pw.p24;
         g[2 * q] = (uint32_t)G;
-        g[2 * q + 1] = (uint32_t)(G >> 32);
+        g[2 * q + 1] = (uint32_t)((uint64_t)G >> 32);
     }
 }
 
@@ -184,13 +192,6 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
     dlo = __builtin_amdgcn_readfirstlane(dlo);  // wave-uniform: keep d scalar
     dhi = __builtin_amdgcn_readfirstlane(dhi);
     mf_v16i acc[NTL];
-#pragma unroll
-    for (int k = 0; k < NTL; k++) {
-#pragma unroll
-        for (int v = 0; v < 16; v++) acc[k][v] = (int)(255u * kMfBias);
-    }
-    // lowest column of block 0: 256 B (the bias correction)
-    if (c0 == 0 && L.lane == 0) acc[0][0] = (int)(256u * kMfBias);
 
     // two operand sets in ping-pong: the loads of step d-1 go to one set
     // while the MFMAs of step d read the other (no copies)
@@ -215,23 +216,40 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
 #pragma unroll
         for (int k = 0; k < NTL; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
     };
-    // steps dhi, dhi-1, ..., dlo: pairs, then a last single step
+    // steps dhi, dhi-1, ..., dlo: the first from zero accumulators, then
+    // pairs, then a last single step
     const int32_t steps = dhi - dlo + 1;
     if (steps > 0) {
         issue(dhi, qa0, bq0);
-        int32_t d = dhi;
-        for (int32_t i = 0; i + 1 < steps; i += 2, d -= 2) {
-            issue(d - 1, qa1, bq1);
-            mma(qa0, bq0);
-            issue(max(d - 2, dlo), qa0, bq0);
-            mma(qa1, bq1);
+        issue(max(dhi - 1, dlo), qa1, bq1);
+        {
+            const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qa0[1], qa0[0], L.sh),
+                               (int)__builtin_amdgcn_alignbit(qa0[2], qa0[1], L.sh),
+                               (int)__builtin_amdgcn_alignbit(qa0[3], qa0[2], L.sh),
+                               (int)__builtin_amdgcn_alignbit(qa0[4], qa0[3], L.sh)};
+#pragma unroll
+            for (int k = 0; k < NTL; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bq0[k], mf_v16i{}, 0, 0, 0);
         }
-        if (steps & 1) mma(qa0, bq0);
+        // set 1 holds step d; rest steps remain, d down to dlo
+        int32_t d = dhi - 1;
+        const int32_t rest = steps - 1;
+        for (int32_t i = 0; i + 1 < rest; i += 2, d -= 2) {
+            issue(d - 1, qa0, bq0);
+            mma(qa1, bq1);
+            issue(max(d - 2, dlo), qa1, bq1);
+            mma(qa0, bq0);
+        }
+        if (rest & 1) mma(qa1, bq1);
+    } else {
+#pragma unroll
+        for (int k = 0; k < NTL; k++) acc[k] = mf_v16i{};
     }
     // column sums to E / O: both tiles' partner exchanges in flight together
     uint32_t gw[NTL][8], pw8[NTL][8];
+    // lowest column of block 0: 256 B (the bias correction)
+    const uint64_t bias0 = kMfFoldBias + (c0 == 0 && L.lane == 0 ? (uint64_t)kMfBias : 0ull);
 #pragma unroll
-    for (int k = 0; k < NTL; k++) mf_fold(acc[k], pw, gw[k]);
+    for (int k = 0; k < NTL; k++) mf_fold(acc[k], pw, k == 0 ? bias0 : kMfFoldBias, gw[k]);
 #pragma unroll
     for (int k = 0; k < NTL; k++) {
 #pragma unroll
@@ -399,6 +417,7 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
         L.E = Eb + (size_t)s * g.PLp;
         L.O = Ob + (size_t)s * g.PLp;
         MfPow pw;
+        pw.p0 = pow8 >> 8;
         pw.p8 = pow8;
         pw.p16 = pow8 * pow8;
         pw.p24 = pw.p16 * pow8;
