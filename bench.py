@@ -208,7 +208,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only: every instance is independent, so N GPUs would each run "
-              f"this same kernel on 1/N of the batch with no exchange (see DESIGN.md); --gpus {args.gpus} "
+              f"this same kernel on 1/N of the batch with no exchange; --gpus {args.gpus} "
               f"is not measured here.", file=sys.stderr)
         sys.exit(2)
 
