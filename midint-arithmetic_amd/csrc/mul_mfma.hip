@@ -359,12 +359,12 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
             const uint32_t n = isb ? g.nb : g.na;
             const uint32_t S4 = 4 * ((isb ? g.Pb : g.Pa) + 1);  // limbs written
             const uint32_t j0 = row * kRowLimbs + 2 * lane;
-            const RowState st = row_build(x0, j0 < n ? kMfHalf : 0ull, x1, j0 + 1 < n ? kMfHalf : 0ull, false, false);
+            const uint64_t h0 = j0 < n ? kMfHalf : 0ull, h1 = j0 + 1 < n ? kMfHalf : 0ull;
+            const RowState st = row_build(x0, h0, x1, h1, false, false);
             const uint64_t C = row_chain(st, &cin);
             uint64_t r0, r1;
             row_apply(st, C, &r0, &r1);
-            const uint64_t w0 = j0 < n ? r0 ^ kMfHalf : r0;
-            const uint64_t w1 = j0 + 1 < n ? r1 ^ kMfHalf : r1;
+            const uint64_t w0 = r0 ^ h0, w1 = r1 ^ h1;
             if (j0 < S4) {  // S4 and j0 even: both limbs
                 if (!isb)
                     *reinterpret_cast<ulonglong2*>(A64 + a_word(s, j0)) = make_ulonglong2(w0, w1);
@@ -396,10 +396,14 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
     // in passes of up to kMfTiles tiles over the union of their d ranges.
     {
         const uint32_t wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-        // instance and first tile of the wave, kept scalar (the division
-        // would otherwise leave every loop bound in vector registers)
-        const uint32_t s = __builtin_amdgcn_readfirstlane(wv / g.wpi);
-        const uint32_t wi = __builtin_amdgcn_readfirstlane(wv - s * g.wpi);
+        // instance and first tile of the wave, by scalar subtraction (wv <
+        // 16 waves): a division would run on the vector unit and leave
+        // every loop bound in vector registers
+        uint32_t s = 0, wi = wv;
+        while (wi >= g.wpi) {
+            wi -= g.wpi;
+            s++;
+        }
         MfLane L;
         L.lds8 = lds8;
         L.lane = lane;
