@@ -224,6 +224,34 @@ def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     assert np.array_equal(host(rl), orc.mul_lo(a, b, m))
 
 
+@pytest.mark.parametrize("seed", range(24))
+def test_mul_mfma_layout_sweep(mi, ctx, torch_dev, orc, seed):
+    """Random sizes, batch counts and launch layouts (instances per
+    workgroup, waves per instance) of the matrix-core kernel. Workgroups
+    whose rows cover exactly their products load and store without range
+    checks; a partial last workgroup or an odd product length takes the
+    checked path. Both agree with the oracle bit for bit. (Layouts beyond
+    16 waves per workgroup run on the word kernel, which must agree too.)"""
+    torch = torch_dev
+    rng = np.random.default_rng(0x5EED + seed)
+    m = int(rng.choice([int(rng.integers(1, 64)), int(rng.integers(64, 300)), int(rng.integers(300, 1100))]))
+    n = int(rng.integers(1, 25))
+    ipb = int(rng.choice([0, 1, 2, 3, 4, 5, 8]))
+    wpi = int(rng.choice([0, 1, 2, 4]))
+    a, b = operands(seed * 31 + m, n, m)
+    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb}):
+        da, db = dev(torch, a), dev(torch, b)
+        rf = empty(torch, 2 * n * m)
+        rl = empty(torch, n * m)
+        ctx.mul_full(da, db, rf, m)
+        ctx.mul_lo(da, db, rl, m)
+        torch.cuda.synchronize()
+    got = host(rf)
+    want = orc.mul_full(a, b, m)
+    assert np.array_equal(got, want), (m, n, ipb, wpi, int(np.flatnonzero(got != want)[0]))
+    assert np.array_equal(host(rl), orc.mul_lo(a, b, m)), (m, n, ipb, wpi)
+
+
 def _check_mul(ctx, torch_dev, orc, m):
     torch = torch_dev
     n = 6 if m >= 1024 else 11
