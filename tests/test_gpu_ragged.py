@@ -141,6 +141,32 @@ def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min):
     assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
+    """Random batches (op count, size range, add/mul mix) under random
+    settings of every ragged tuning key: threads per workgroup of the
+    persistent kernel, the matrix-core threshold, and whether the persistent
+    kernel runs beside the matrix-core node or after it. Two replays of the
+    graph, each equal to the oracle."""
+    rng = np.random.default_rng(0xA11 + seed)
+    lo = int(rng.integers(6, 11))
+    hi = int(rng.integers(lo + 1, 17))
+    ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=int(rng.integers(50, 500)), seed=100 + seed,
+                                mul_frac=float(rng.choice([0.3, 0.7, 1.0])), lo_bits=lo, hi_bits=hi)
+    cfg = {"ragged.mfma_min": int(rng.choice([0, 1, 8, 32, 64])), "ragged.overlap": int(rng.integers(0, 2))}
+    threads = int(rng.choice([256, 512]))
+    saved = {k: mi.tune_get(k) for k in cfg}
+    for k, v in cfg.items():
+        mi.tune_set(k, v)
+    try:
+        got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, threads=threads, reps=2)
+    finally:
+        for k, v in saved.items():
+            mi.tune_set(k, v)
+    want = ragged_reference(orc, ops, a, b, rl)
+    assert np.array_equal(got, want), (cfg, threads, int(np.flatnonzero(got != want)[0]))
+
+
 def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc):
     """Every multiplication on the matrix-core node: unbalanced and tiny
     shapes, all-ones and 0x7F-byte operands (the longest digit carry chain)."""
