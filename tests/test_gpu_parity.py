@@ -303,7 +303,8 @@ def midint_max_mul(ctx):
     return load_package().lib().midint_max_limbs_mul()
 
 
-NTT_SIZES = [1, 2, 3, 8, 16, 64, 100, 128, 256, 1000, 1024, 2048]
+# transform lengths 2^6 .. 2^14 points (L >= 8m): every length at least once
+NTT_SIZES = [1, 2, 3, 8, 16, 24, 64, 100, 128, 256, 300, 512, 1000, 1024, 2048]
 
 
 @pytest.mark.parametrize("m", NTT_SIZES)
@@ -317,6 +318,23 @@ def test_mul_ntt(ctx, torch_dev, orc, m):
     want = orc.mul_full(a, b, m)
     got = host(dr)
     assert np.array_equal(got, want), (m, int(np.flatnonzero(got != want)[0]))
+
+
+@pytest.mark.parametrize("vals", [16, 32])
+@pytest.mark.parametrize("m", [256, 512, 1024, 2048])
+def test_mul_ntt_values_per_thread(mi, ctx, torch_dev, orc, m, vals):
+    """Both register-pass layouts of the 2^11 .. 2^14-point kernels (16 or
+    32 values per thread, ntt.vals), whichever the tuning table picks."""
+    torch = torch_dev
+    n = 3
+    a, b = operands(m * 31 + vals, n, m)
+    with tuned(mi, 8 * m, {"ntt.vals": vals}):
+        dr = empty(torch, 2 * n * m)
+        ctx.mul_ntt(dev(torch, a), dev(torch, b), dr, m)
+        torch.cuda.synchronize()
+    got = host(dr)
+    want = orc.mul_full(a, b, m)
+    assert np.array_equal(got, want), (m, vals, int(np.flatnonzero(got != want)[0]))
 
 
 def test_ntt_conv_kat(ctx, torch_dev, orc):
