@@ -89,6 +89,31 @@ def test_add(ctx, torch_dev, orc, m):
         assert np.array_equal(dc.cpu().numpy(), wc), (m, n)
 
 
+ADD_CFGS = [(r, t, nt) for r in (1, 2, 4, 8) for t in (64, 256, 1024) for nt in (0, 1)]
+
+
+@pytest.mark.parametrize("cfg", ADD_CFGS, ids=lambda c: f"rows{c[0]}-threads{c[1]}-nt{c[2]}")
+def test_add_layouts(mi, ctx, torch_dev, orc, cfg):
+    """Every addition layout the tuning table can pick up to 256 limbs (rows
+    of 128 limbs per wave, threads per block), with and without
+    non-temporal loads. A layout whose rows do not cover an instance falls
+    back to the default one."""
+    rows, threads, nt = cfg
+    torch = torch_dev
+    for m in (1, 7, 64, 129, 256):
+        with tuned(mi, m, {"add.rows": rows, "add.threads": threads, "add.nt": nt}):
+            for n in (3, 41):
+                a, b = operands(m * 17 + n + rows, n, m)
+                dr = empty(torch, n * m)
+                dc = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+                ctx.add(dev(torch, a), dev(torch, b), dr, m, carry=dc)
+                torch.cuda.synchronize()
+                want, wc = orc.add(a, b, m)
+                got = host(dr)[: n * m]
+                assert np.array_equal(got, want), (m, n, int(np.flatnonzero(got != want)[0]))
+                assert np.array_equal(dc.cpu().numpy(), wc), (m, n)
+
+
 @pytest.mark.parametrize("m", [1, 3, 64, 129, 1000, 4096])
 def test_add_in_place_and_unaligned_vectors(ctx, torch_dev, orc, m):
     torch = torch_dev
