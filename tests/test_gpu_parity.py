@@ -362,6 +362,26 @@ def test_mul_ntt_values_per_thread(mi, ctx, torch_dev, orc, m, vals):
     assert np.array_equal(got, want), (m, vals, int(np.flatnonzero(got != want)[0]))
 
 
+@pytest.mark.parametrize("m", [128, 512, 2048])
+def test_mul_ntt_lds_kernel(mi, ctx, torch_dev, orc, m):
+    """ntt.fast = 0: 2^10 .. 2^14-point products on the all-in-LDS kernel
+    (one radix-2 stage per barrier) instead of the register passes."""
+    torch = torch_dev
+    n = 3
+    a, b = operands(m * 37, n, m)
+    old = mi.tune_get("ntt.fast")
+    mi.tune_set("ntt.fast", 0)
+    try:
+        dr = empty(torch, 2 * n * m)
+        ctx.mul_ntt(dev(torch, a), dev(torch, b), dr, m)
+        torch.cuda.synchronize()
+    finally:
+        mi.tune_set("ntt.fast", old)
+    got = host(dr)
+    want = orc.mul_full(a, b, m)
+    assert np.array_equal(got, want), (m, int(np.flatnonzero(got != want)[0]))
+
+
 def test_ntt_conv_kat(ctx, torch_dev, orc):
     """cuda/main.cu:637-652: the reference's FFT known-answer test (raw
     convolution), padded to the smallest supported transform (64 points)."""
