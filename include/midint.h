@@ -311,9 +311,10 @@ midint_status midint_gpu_mul_fft_prime_host(int device, uint32_t prime, uint32_t
 
 /* ------------------------------------------------------------- tuning */
 /* Process-wide launch parameters used by the kernels' launchers, set by the
- * autotuner (midint-arithmetic_amd/autotune.py) or by hand: "mul.q" (4|8),
- * "mul.prefetch" (0|1), "add.rows", "add.threads", "add.nt" (0|1),
- * "ntt.threads", "ragged.threads". "key@C" (C a power of two) sets the value
+ * autotuner (midint-arithmetic_amd/autotune.py) or by hand, for example
+ * "mul.q" (4|8), "mul.mfma" (0|1), "add.rows", "add.threads", "add.nt" (0|1),
+ * "ntt.vals", "ragged.threads" (the full list is in csrc/tuning.cpp).
+ * "key@C" (C a power of two) sets the value
  * for operands of (C/2, C] limbs. Unknown keys: MIDINT_ENOTFOUND. Results
  * never depend on these values, only speed. No reference counterpart (the reference fixes
  * Q and block sizes at compile time, cuda/main.cu:64-78, 243-261). */
