@@ -208,11 +208,15 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
             bq[k] = *reinterpret_cast<const mf_v4i*>(L.lds8 + addr);
         }
     };
+    // the row of T_d: the lane's 16 digits from five aligned dwords
+    auto a_op = [&](const uint32_t (&qc)[5]) {
+        return mf_v4i{(int)__builtin_amdgcn_alignbit(qc[1], qc[0], L.sh),
+                      (int)__builtin_amdgcn_alignbit(qc[2], qc[1], L.sh),
+                      (int)__builtin_amdgcn_alignbit(qc[3], qc[2], L.sh),
+                      (int)__builtin_amdgcn_alignbit(qc[4], qc[3], L.sh)};
+    };
     auto mma = [&](const uint32_t (&qc)[5], const mf_v4i (&bc)[NTL]) {
-        const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qc[1], qc[0], L.sh),
-                           (int)__builtin_amdgcn_alignbit(qc[2], qc[1], L.sh),
-                           (int)__builtin_amdgcn_alignbit(qc[3], qc[2], L.sh),
-                           (int)__builtin_amdgcn_alignbit(qc[4], qc[3], L.sh)};
+        const mf_v4i av = a_op(qc);
 #pragma unroll
         for (int k = 0; k < NTL; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bc[k], acc[k], 0, 0, 0);
     };
@@ -223,10 +227,7 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
         issue(dhi, qa0, bq0);
         issue(max(dhi - 1, dlo), qa1, bq1);
         {
-            const mf_v4i av = {(int)__builtin_amdgcn_alignbit(qa0[1], qa0[0], L.sh),
-                               (int)__builtin_amdgcn_alignbit(qa0[2], qa0[1], L.sh),
-                               (int)__builtin_amdgcn_alignbit(qa0[3], qa0[2], L.sh),
-                               (int)__builtin_amdgcn_alignbit(qa0[4], qa0[3], L.sh)};
+            const mf_v4i av = a_op(qa0);
 #pragma unroll
             for (int k = 0; k < NTL; k++) acc[k] = __builtin_amdgcn_mfma_i32_32x32x32_i8(av, bq0[k], mf_v16i{}, 0, 0, 0);
         }
