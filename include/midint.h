@@ -89,7 +89,10 @@ midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
 
 /* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance.
  * Replaces polyKerQ (cuda/ker-classic-mul.cu.h:500-575). Uses 4*n*m limbs
- * of context scratch. */
+ * of context scratch (allocated on first need, grown when too small). Calls
+ * that use context scratch (poly, poly_ntt, poly_strided and the repacking
+ * u32 path) on different streams of one context are ordered on the device:
+ * each waits for the previous scratch user to finish. */
 midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                           uint64_t* d_r, void* stream);
 
@@ -108,7 +111,8 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
 
 /* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance with NTT products, the
  * composite of midint_poly. Replaces polyFttKer (cuda/ker-fft-mul.cu.h:
- * 456-509). Uses 4*n*m limbs of context scratch; m <= midint_max_limbs_ntt(). */
+ * 456-509). Uses 4*n*m limbs of context scratch (ordered across streams as
+ * for midint_poly); m <= midint_max_limbs_ntt(). */
 midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                               uint64_t* d_r, void* stream);
 
@@ -158,8 +162,8 @@ midint_status midint_cgbn(midint_ctx* ctx, uint32_t op, uint32_t n, uint32_t bit
  * bytes (optional). Pointers need 4-byte alignment. Even m32 with 8-byte
  * aligned pointers is the u64 layout and runs in place; otherwise each
  * instance is zero-extended to whole u64 limbs in context scratch
- * (2..7 * n * ceil(m32/2) limbs), so calls on one context that take that
- * path must not overlap on different streams. */
+ * (2..7 * n * ceil(m32/2) limbs); such calls on different streams of one
+ * context take turns on the device (see midint_poly). */
 midint_status midint_add_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                              uint32_t* d_r, uint8_t* d_carry, void* stream);
 midint_status midint_add6_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,

@@ -20,8 +20,13 @@ struct midint_ctx {
     std::mutex mu;
     std::map<uint32_t, NttTables> ntt;  // keyed by log2 length
     std::map<uint32_t, PrimeTables> prime_ntt;  // keyed by (prime, lg, root kind)
+    // Context scratch (poly / poly_ntt temporaries, u32 repacking). Its last
+    // user recorded scratch_ev on its stream; the next user's stream waits on
+    // it, so calls on different streams take turns on the device.
     uint64_t* scratch = nullptr;
     size_t scratch_limbs = 0;
+    hipEvent_t scratch_ev = nullptr;
+    bool scratch_ev_live = false;
 };
 
 static thread_local int g_last_hip = 0;
@@ -40,6 +45,58 @@ static midint_status bind_device(const midint_ctx* ctx) {
 }
 
 static bool aligned8(const void* p) { return (((uintptr_t)p) & 7u) == 0; }
+
+// Makes at least `limbs` of context scratch available to work enqueued next
+// on `stream`: the stream first waits for the scratch's previous user. Call
+// with ctx->mu held, then enqueue, then release_scratch.
+static midint_status acquire_scratch(midint_ctx* ctx, size_t limbs, hipStream_t stream) {
+    hipError_t e;
+    if (!ctx->scratch_ev) {
+        if ((e = hipEventCreateWithFlags(&ctx->scratch_ev, hipEventDisableTiming)) != hipSuccess) {
+            ctx->scratch_ev = nullptr;
+            return hip_status(e);
+        }
+    }
+    if (ctx->scratch_limbs < limbs) {
+        if (ctx->scratch) {
+            // the old buffer may still be in use on another stream
+            if (ctx->scratch_ev_live && (e = hipEventSynchronize(ctx->scratch_ev)) != hipSuccess) return hip_status(e);
+            e = hipFree(ctx->scratch);
+            ctx->scratch = nullptr;
+            ctx->scratch_limbs = 0;
+            if (e != hipSuccess) return hip_status(e);
+        }
+        e = hipMalloc((void**)&ctx->scratch, limbs * sizeof(uint64_t));
+        if (e != hipSuccess) {
+            ctx->scratch = nullptr;
+            (void)hipGetLastError();
+            return MIDINT_ENOMEM;
+        }
+        ctx->scratch_limbs = limbs;
+    } else if (ctx->scratch_ev_live && (e = hipStreamWaitEvent(stream, ctx->scratch_ev, 0)) != hipSuccess) {
+        return hip_status(e);
+    }
+    return MIDINT_OK;
+}
+
+// Marks the end of the work just enqueued on `stream` that uses the scratch.
+static midint_status release_scratch(midint_ctx* ctx, hipStream_t stream) {
+    hipError_t e = hipEventRecord(ctx->scratch_ev, stream);
+    if (e != hipSuccess) return hip_status(e);
+    ctx->scratch_ev_live = true;
+    return MIDINT_OK;
+}
+
+// Runs `enqueue(scratch)` on `stream` with `limbs` of context scratch.
+template <class F>
+static midint_status with_scratch(midint_ctx* ctx, size_t limbs, hipStream_t stream, F enqueue) {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    midint_status st = acquire_scratch(ctx, limbs, stream);
+    if (st != MIDINT_OK) return st;
+    st = hip_status(enqueue(ctx->scratch));
+    if (st != MIDINT_OK) return st;
+    return release_scratch(ctx, stream);
+}
 
 extern "C" {
 
@@ -88,11 +145,12 @@ midint_status midint_destroy(midint_ctx* ctx) {
         (void)hipFree(kv.second.d_inv);
     }
     if (ctx->scratch) (void)hipFree(ctx->scratch);
+    if (ctx->scratch_ev) (void)hipEventDestroy(ctx->scratch_ev);
     delete ctx;
     return MIDINT_OK;
 }
 
-uint32_t midint_max_limbs_add(void) { return 16u * 16u * 128u; }
+uint32_t midint_max_limbs_add(void) { return kAddMaxLimbs; }
 uint32_t midint_max_limbs_mul(void) { return 2048u; }
 uint32_t midint_max_limbs_ntt(void) { return (1u << kNttMaxLg) / 8u; }
 
@@ -141,33 +199,15 @@ midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
     return hip_status(launch_mul(n, m, d_a, d_b, d_r, true, (hipStream_t)stream));
 }
 
-static midint_status ensure_scratch(midint_ctx* ctx, size_t limbs) {
-    if (ctx->scratch_limbs >= limbs) return MIDINT_OK;
-    if (ctx->scratch) {
-        hipError_t e = hipFree(ctx->scratch);
-        ctx->scratch = nullptr;
-        ctx->scratch_limbs = 0;
-        if (e != hipSuccess) return hip_status(e);
-    }
-    hipError_t e = hipMalloc((void**)&ctx->scratch, limbs * sizeof(uint64_t));
-    if (e != hipSuccess) {
-        ctx->scratch = nullptr;
-        (void)hipGetLastError();
-        return MIDINT_ENOMEM;
-    }
-    ctx->scratch_limbs = limbs;
-    return MIDINT_OK;
-}
-
 midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                           uint64_t* d_r, void* stream) {
     midint_status st = check_binary(ctx, n, m, d_a, d_b, d_r);
     if (st != MIDINT_OK || n == 0) return st;
     if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
     if ((st = bind_device(ctx)) != MIDINT_OK) return st;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
-    return hip_status(launch_poly(n, m, d_a, d_b, d_r, ctx->scratch, (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    return with_scratch(ctx, 4ull * n * m, s,
+                        [&](uint64_t* scr) { return launch_poly(n, m, d_a, d_b, d_r, scr, s); });
 }
 
 // Returns the (cached) twiddle tables of length 2^lg.
@@ -246,9 +286,9 @@ midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
     if ((st = bind_device(ctx)) != MIDINT_OK) return st;
     NttTables t;
     if ((st = get_tables(ctx, ntt_lg_for_limbs(m), &t)) != MIDINT_OK) return st;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
-    return hip_status(launch_poly_ntt(n, m, d_a, d_b, d_r, ctx->scratch, t, (hipStream_t)stream));
+    hipStream_t s = (hipStream_t)stream;
+    return with_scratch(ctx, 4ull * n * m, s,
+                        [&](uint64_t* scr) { return launch_poly_ntt(n, m, d_a, d_b, d_r, scr, t, s); });
 }
 
 // ------------------------------------------------- strided u64 layouts
@@ -277,11 +317,9 @@ static midint_status u64_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m, cons
         case Op::Add6: return hip_status(launch_add(n, m, a, b, r, nullptr, true, s, str));
         case Op::MulLo: return hip_status(launch_mul(n, m, a, b, r, false, s, str));
         case Op::MulFull: return hip_status(launch_mul(n, m, a, b, r, true, s, str));
-        case Op::Poly: {
-            std::lock_guard<std::mutex> lk(ctx->mu);
-            if ((st = ensure_scratch(ctx, 4ull * n * m)) != MIDINT_OK) return st;
-            return hip_status(launch_poly(n, m, a, b, r, ctx->scratch, s, str));
-        }
+        case Op::Poly:
+            return with_scratch(ctx, 4ull * n * m, s,
+                                [&](uint64_t* scr) { return launch_poly(n, m, a, b, r, scr, s, str); });
     }
     return MIDINT_EINVAL;
 }
@@ -338,27 +376,29 @@ static midint_status u32_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m32, co
     if (op == Op::Add && mp > midint_max_limbs_add()) return MIDINT_EUNSUPPORTED;
     const uint32_t out_limbs = op_out_limbs(op, mp);
     const size_t nm = (size_t)n * mp;
-    std::lock_guard<std::mutex> lk(ctx->mu);
-    if ((st = ensure_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == Op::Poly ? 4 * nm : 0))) != MIDINT_OK)
-        return st;
-    uint64_t* pa = ctx->scratch;
-    uint64_t* pb = pa + nm;
-    uint64_t* pr = pb + nm;
-    uint64_t* extra = pr + (size_t)n * out_limbs;
     hipStream_t s = (hipStream_t)stream;
-    hipError_t e = launch_pack_u32(n, m32, mp, a, sa, pa, s);
-    if (e == hipSuccess) e = launch_pack_u32(n, m32, mp, b, sb, pb, s);
-    if (e == hipSuccess) {
-        switch (op) {
-            case Op::Add: e = launch_add(n, mp, pa, pb, pr, nullptr, false, s); break;
-            case Op::Add6: e = launch_add(n, mp, pa, pb, pr, nullptr, true, s); break;
-            case Op::MulLo: e = launch_mul(n, mp, pa, pb, pr, false, s); break;
-            case Op::MulFull: e = launch_mul(n, mp, pa, pb, pr, true, s); break;
-            case Op::Poly: e = launch_poly(n, mp, pa, pb, pr, extra, s); break;
-        }
-    }
-    if (e == hipSuccess) e = launch_unpack_u32(n, out_w32, out_limbs, pr, r, sr, op == Op::Add ? carry : nullptr, s);
-    return hip_status(e);
+    return with_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == Op::Poly ? 4 * nm : 0), s,
+                        [&](uint64_t* scr) {
+                            uint64_t* pa = scr;
+                            uint64_t* pb = pa + nm;
+                            uint64_t* pr = pb + nm;
+                            uint64_t* extra = pr + (size_t)n * out_limbs;
+                            hipError_t e = launch_pack_u32(n, m32, mp, a, sa, pa, s);
+                            if (e == hipSuccess) e = launch_pack_u32(n, m32, mp, b, sb, pb, s);
+                            if (e == hipSuccess) {
+                                switch (op) {
+                                    case Op::Add: e = launch_add(n, mp, pa, pb, pr, nullptr, false, s); break;
+                                    case Op::Add6: e = launch_add(n, mp, pa, pb, pr, nullptr, true, s); break;
+                                    case Op::MulLo: e = launch_mul(n, mp, pa, pb, pr, false, s); break;
+                                    case Op::MulFull: e = launch_mul(n, mp, pa, pb, pr, true, s); break;
+                                    case Op::Poly: e = launch_poly(n, mp, pa, pb, pr, extra, s); break;
+                                }
+                            }
+                            if (e == hipSuccess)
+                                e = launch_unpack_u32(n, out_w32, out_limbs, pr, r, sr, op == Op::Add ? carry : nullptr,
+                                                      s);
+                            return e;
+                        });
 }
 
 midint_status midint_add_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,

@@ -238,6 +238,95 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
     store_window<R, VEC, false, STRIDED>(STRIDED ? r + inst_base * sr : r + base, t, L, m, off0, inst0, sr);
 }
 
+// Instances longer than one workgroup's rows (m > kAddRowMax limbs): one
+// instance per workgroup, walked in chunks of (waves * R * 128) limbs. Each
+// addition of the chain keeps its carry between chunks in a scalar, so any
+// length runs with the registers of R rows. SIX: the six-add chain of
+// add6_kernel, chunk by chunk (each of its additions carries separately).
+template <int R, bool VEC, bool SIX>
+__global__ __launch_bounds__(1024) void add_long_kernel(uint32_t m, const uint64_t* __restrict__ a,
+                                                        const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                        uint8_t* __restrict__ carry, uint64_t sa, uint64_t sb,
+                                                        uint64_t sr) {
+    __shared__ uint32_t lds_gp[16];
+    const uint64_t inst = blockIdx.x;
+    const uint64_t* pa = a + inst * sa;
+    const uint64_t* pb = b + inst * sb;
+    uint64_t* pr = r + inst * sr;
+    const uint32_t chunk = (blockDim.x / kWave) * R * kRowLimbs;
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+    uint32_t c[SIX ? 6 : 1];
+#pragma unroll
+    for (int q = 0; q < (SIX ? 6 : 1); q++) c[q] = 0;
+    for (uint32_t base = 0; base < m; base += chunk) {
+        Rows<R> t;
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t k = base + k_first + v * kRowLimbs;
+            load_pair<VEC, true>(pa, k, m, t.x0[v], t.x1[v]);
+            load_pair<VEC, true>(pb, k, m, t.y0[v], t.y1[v]);
+        }
+        if (!SIX) {
+            uint64_t ax0[R], ax1[R];
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                ax0[v] = t.x0[v];
+                ax1[v] = t.x1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[0]);
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                const uint32_t k = base + k_first + v * kRowLimbs;
+                store_pair<VEC, true>(pr, k, m, t.x0[v], t.x1[v]);
+                if (carry && k == m - 1) carry[inst] = (uint8_t)limb_cout(ax0[v], t.x0[v] - t.c0[v], t.c0[v]);
+                if (carry && k + 1 == m - 1)
+                    carry[inst] = (uint8_t)limb_cout(ax1[v], t.x1[v] - t.c1[v], t.c1[v]);
+            }
+        } else {
+            uint64_t b0[R], b1[R], u0[R], u1[R];
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                b0[v] = t.y0[v];
+                b1[v] = t.y1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[0]);  // t1 = a + b
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                t.y0[v] = t.x0[v];
+                t.y1[v] = t.x1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[1]);  // t2 = t1 + t1
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                t.y0[v] = b0[v];
+                t.y1[v] = b1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[2]);  // t3 = t2 + b
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                u0[v] = t.y0[v] = t.x0[v];
+                u1[v] = t.y1[v] = t.x1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[3]);  // t4 = t3 + t3
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                t.y0[v] = u0[v];
+                t.y1[v] = u1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[4]);  // t5 = t4 + t3
+#pragma unroll
+            for (int v = 0; v < R; v++) {
+                t.y0[v] = b0[v];
+                t.y1[v] = b1[v];
+            }
+            block_add_rows_carry<R>(t, lds_gp, &c[5]);  // r = t5 + b
+#pragma unroll
+            for (int v = 0; v < R; v++)
+                store_pair<VEC, true>(pr, base + k_first + v * kRowLimbs, m, t.x0[v], t.x1[v]);
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launch geometry.
 
@@ -253,22 +342,15 @@ AddGeom add_geometry(uint32_t m) {
             g.rows = 2;
             g.threads = 256;
         }
+    } else if (m <= kAddRowMax) {
+        const uint32_t waves = (m + 2 * kRowLimbs - 1) / (2 * kRowLimbs);
+        g.rows = 2;
+        g.threads = (waves < 4 ? 4 : waves) * kWave;
     } else {
-        g.rows = 0;
-        const uint32_t rows_opts[] = {2, 4, 8, 16};
-        for (uint32_t R : rows_opts) {
-            uint32_t waves = (m + R * kRowLimbs - 1) / (R * kRowLimbs);
-            if (waves <= 16) {
-                g.rows = R;
-                g.threads = (waves < 4 ? 4 : waves) * kWave;
-                break;
-            }
-        }
-        if (g.rows == 0) {
-            g.threads = 0;
-            g.ipb = 0;
-            return g;
-        }
+        g.rows = 0;  // add_long_kernel
+        g.threads = 0;
+        g.ipb = 0;
+        return g;
     }
     uint32_t cap = (g.threads / kWave) * g.rows * kRowLimbs;
     g.ipb = cap / m;
@@ -308,8 +390,29 @@ static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n,
     return hipGetLastError();
 }
 
+// One instance per workgroup of 16 waves, rows of 4 x 128 limbs per wave.
+static hipError_t launch_add_long(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                                  uint8_t* carry, bool six, hipStream_t s, const Strides& st) {
+    const uint64_t sa = st.a ? st.a : m, sb = st.b ? st.b : m, sr = st.r ? st.r : m;
+    const bool vec = ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 15u) == 0 && ((sa | sb | sr) & 1u) == 0;
+    constexpr uint32_t kThreads = 1024;
+    if (six) {
+        if (vec)
+            add_long_kernel<4, true, true><<<n, kThreads, 0, s>>>(m, a, b, r, nullptr, sa, sb, sr);
+        else
+            add_long_kernel<4, false, true><<<n, kThreads, 0, s>>>(m, a, b, r, nullptr, sa, sb, sr);
+    } else {
+        if (vec)
+            add_long_kernel<4, true, false><<<n, kThreads, 0, s>>>(m, a, b, r, carry, sa, sb, sr);
+        else
+            add_long_kernel<4, false, false><<<n, kThreads, 0, s>>>(m, a, b, r, carry, sa, sb, sr);
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, uint8_t* carry,
                       bool six, hipStream_t s, Strides st) {
+    if (m > kAddRowMax) return launch_add_long(n, m, a, b, r, carry, six, s, st);
     AddGeom g = add_geometry(m);
     if (g.ipb == 0) return hipErrorInvalidValue;
     const bool aligned = ((((uintptr_t)a) | ((uintptr_t)b) | ((uintptr_t)r)) & 15u) == 0;
@@ -319,7 +422,6 @@ hipError_t launch_add(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t*
         case 2: return launch_add_r<2>(g, vec, six, n, m, a, b, r, carry, st, s);
         case 4: return launch_add_r<4>(g, vec, six, n, m, a, b, r, carry, st, s);
         case 8: return launch_add_r<8>(g, vec, six, n, m, a, b, r, carry, st, s);
-        case 16: return launch_add_r<16>(g, vec, six, n, m, a, b, r, carry, st, s);
     }
     return hipErrorInvalidValue;
 }

@@ -50,4 +50,30 @@ __device__ __forceinline__ void block_add_rows(Rows<R>& t, uint32_t* lds_gp) {
     }
 }
 
+// x <- x + y + *c over one chunk of a longer integer: no segment starts, the
+// carry *c enters the block's first limb and *c receives the carry out of its
+// last limb. Every wave of the block must call this.
+template <int R>
+__device__ __forceinline__ void block_add_rows_carry(Rows<R>& t, uint32_t* lds_gp, uint32_t* c) {
+    RowState st[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) st[v] = row_build(t.x0[v], t.y0[v], t.x1[v], t.y1[v], false, false);
+    uint32_t cz = 0, co = 1;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        (void)row_chain(st[v], &cz);
+        (void)row_chain(st[v], &co);
+    }
+    uint32_t cw = block_wave_carry_chain(cz, co & ~cz, *c, lds_gp, c);
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        uint64_t C = row_chain(st[v], &cw);
+        row_apply(st[v], C, &t.x0[v], &t.x1[v]);
+        t.c0[v] = (C >> lane) & 1ull;
+        uint64_t Cl1 = st[v].G0 | (st[v].P0 & C);
+        t.c1[v] = (Cl1 >> lane) & 1ull;
+    }
+}
+
 }  // namespace midint

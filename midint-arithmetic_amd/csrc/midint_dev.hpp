@@ -142,6 +142,29 @@ __device__ __forceinline__ uint32_t block_wave_carry_in(uint32_t g, uint32_t p, 
     return c;
 }
 
+// Like block_wave_carry_in, with a carry `cin` into wave 0: returns the carry
+// into this wave and sets *cblock to the carry out of the block's last wave.
+__device__ __forceinline__ uint32_t block_wave_carry_chain(uint32_t g, uint32_t p, uint32_t cin, uint32_t* lds_gp,
+                                                           uint32_t* cblock) {
+    const uint32_t wave = threadIdx.x / kWave;
+    const uint32_t nwaves = blockDim.x / kWave;
+    if (nwaves == 1) {
+        *cblock = g | (p & cin);
+        return cin;
+    }
+    if (lane_id() == 0) lds_gp[wave] = g | (p << 1);
+    __syncthreads();
+    uint32_t c = cin, mine = cin;
+    for (uint32_t u = 0; u < nwaves; u++) {
+        if (u == wave) mine = c;
+        const uint32_t v = lds_gp[u];
+        c = (v & 1u) | ((v >> 1) & c);
+    }
+    __syncthreads();
+    *cblock = c;
+    return mine;
+}
+
 // ---------------------------------------------------------------------------
 // Goldilocks field p = 2^64 - 2^32 + 1. Values are kept canonical (< p).
 constexpr uint64_t kGlP = 0xFFFFFFFF00000001ull;

@@ -18,6 +18,11 @@ struct AddGeom {
     uint32_t ipb;      // instances per block (0: size unsupported)
 };
 AddGeom add_geometry(uint32_t m);
+// Longest instance (limbs) that one workgroup holds in registers (16 waves of
+// 2 rows); longer instances run on add_long_kernel, chunk by chunk.
+constexpr uint32_t kAddRowMax = 16 * 2 * 128;
+// Longest addition operand (limbs).
+constexpr uint32_t kAddMaxLimbs = 1u << 24;
 
 // Instance strides (limbs) of the operands and the result; 0 = the
 // batch-major stride (m, or the output length).

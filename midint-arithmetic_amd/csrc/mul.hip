@@ -472,8 +472,9 @@ hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t
     uint64_t* t2 = scratch + nm;
     uint64_t* t3 = scratch + 2 * nm;
     uint64_t* t4 = scratch + 3 * nm;
-    const uint64_t sa = st.a ? st.a : m, sb = st.b ? st.b : m;
-    // strided operands / result only at the ends of the chain; temporaries are batch-major
+    // strided operands / result only at the ends of the chain; temporaries are
+    // batch-major. A zero stride keeps an addition on its 16-byte vector path.
+    const uint64_t sa = st.a, sb = st.b;
     hipError_t e;
     if ((e = launch_mul(n, m, a, a, t1, false, s, {sa, sa, 0})) != hipSuccess) return e;         // t1 = a^2
     if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s, {0, sb, 0})) != hipSuccess) return e; // t2 = a^2 + b

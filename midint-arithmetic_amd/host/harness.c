@@ -7,8 +7,12 @@
  *   runFFTMuls    -> run_fft_muls    (cuda/main.cu:711-746), validated here;
  *                    Goldilocks, then FftPrime32 as cuda/main.cu:766 runs it
  *
- * For each size (in u32 words, as the reference counts them) the inputs are
- * generated with the reference's ourMkRandom<m,m> (glibc rand, seed 1), the
+ * Inputs come from the reference's mkRandArrays<32,32> (cuda/main.cu:18-33):
+ * at the start of each sweep, ourMkRandom (cuda/helper.h:99-114) fills the
+ * operand arrays a, then b, from ONE glibc rand() stream that is never
+ * reseeded (glibc's default seed 1) and continues from sweep to sweep, and
+ * every size of the sweep reads the first n*m words of those same arrays.
+ * For each size (in u32 words, as the reference counts them) the
  * operation runs through the host drop-ins (midint_gpu_*_host: dry run, then
  * `runs` timed launches), and the result is compared word for word with
  * GMP's mpn functions loaded from libgmp.so.10 (cuda/helper.h:146-170,
@@ -65,22 +69,55 @@ static int validate_exact(const uint64_t* want, const uint64_t* got, size_t coun
 
 enum { OP_ADD, OP_MUL_LO, OP_MUL_FFT };
 
-static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int gmp, uint32_t prime) {
+/* The reference's ourMkRandom<32,32> over `words` u32 words (every word
+ * random): two rand() calls per word, continuing the process's glibc stream. */
+static void mk_random(size_t words, uint32_t* out) {
+    for (size_t k = 0; k < words; k++) {
+        const uint32_t lo = (uint32_t)rand() * 2u;
+        const uint32_t hi = (uint32_t)rand() * 2u;
+        out[k] = (hi << 16) + lo;
+    }
+}
+
+/* mkRandArrays<32,32>(num_instances): a then b, `drawn` words each, from the
+ * continuing stream; only the first `keep` words of each are stored (the
+ * sizes of the sweep use at most that many). */
+static int mk_rand_arrays(size_t drawn, size_t keep, uint32_t** a, uint32_t** b) {
+    *a = malloc(keep * 4);
+    *b = malloc(keep * 4);
+    uint32_t* spill = malloc(4096 * 4);
+    if (!*a || !*b || !spill) {
+        free(*a), free(*b), free(spill);
+        return 0;
+    }
+    uint32_t* outs[2] = {*a, *b};
+    for (int o = 0; o < 2; o++) {
+        mk_random(keep, outs[o]);
+        for (size_t done = keep; done < drawn;) { /* the reference draws these too */
+            const size_t c = drawn - done < 4096 ? drawn - done : 4096;
+            mk_random(c, spill);
+            done += c;
+        }
+    }
+    free(spill);
+    return 1;
+}
+
+static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, const uint32_t* wa, const uint32_t* wb,
+                   int gmp, uint32_t prime) {
     const uint32_t m = m_u32 / 2;                    /* u64 limbs */
     const uint32_t n = (uint32_t)(total_work / m_u32);
     const size_t in_limbs = (size_t)n * m;
     const size_t out_limbs = (op == OP_MUL_FFT) ? 2 * in_limbs : in_limbs;
-    uint64_t* a = malloc(in_limbs * 8);
-    uint64_t* b = malloc(in_limbs * 8);
+    const uint64_t* a = (const uint64_t*)wa; /* a limb is two little-endian u32 words */
+    const uint64_t* b = (const uint64_t*)wb;
     uint64_t* r = malloc(out_limbs * 8);
     uint64_t* want = malloc(out_limbs * 8);
-    if (!a || !b || !r || !want) {
+    if (!r || !want) {
         printf("host allocation failed\n");
+        free(r), free(want);
         return 0;
     }
-    /* both operands from one glibc stream, as mkRandArrays (cuda/main.cu:18-33) */
-    midint_gen_ref_rand(1, n, m_u32, m_u32, (uint32_t*)a);
-    midint_gen_ref_rand(2, n, m_u32, m_u32, (uint32_t*)b);
 
     double usec = 0.0;
     midint_status st;
@@ -93,7 +130,7 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int 
         st = midint_gpu_mul_fft_prime_host(device, prime, n, m, a, b, r, runs, &usec);
     if (st != MIDINT_OK) {
         printf("midint error: %s (hip %d)\n", midint_strerror(st), midint_last_hip_error());
-        free(a), free(b), free(r), free(want);
+        free(r), free(want);
         return 0;
     }
     const unsigned bits = m_u32 * 32;
@@ -137,7 +174,7 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, int 
            name, bits, n, usec, op == OP_ADD ? "GB_per_s" : "Gu32ops_per_s", metric,
            ok < 0 ? "null" : ok ? "true" : "false");
     fflush(stdout);
-    free(a), free(b), free(r), free(want);
+    free(r), free(want);
     return ok != 0;
 }
 
@@ -172,15 +209,27 @@ int main(int argc, char** argv) {
     const int gmp = load_gmp();
     if (!gmp) printf("libgmp.so.10 not found: results are not validated\n");
     const uint32_t sizes[] = {4096, 2048, 1024, 512, 256, 128, 64, 32, 16};
+    const size_t nsz = sizeof(sizes) / sizeof(sizes[0]);
     int ok = 1;
-    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_ADD, device, total_work, sizes[i], gmp, 0);
-    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], gmp, 0);
-    for (size_t i = 0; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp, MIDINT_PRIME_GOLDILOCKS);
+    uint32_t *a = NULL, *b = NULL;
+    /* runAdditions: mkRandArrays<32,32>(total_work/32) -> total_work words each */
+    if (!mk_rand_arrays(total_work, total_work, &a, &b)) return 1;
+    for (size_t i = 0; i < nsz; i++) ok &= run_one(OP_ADD, device, total_work, sizes[i], a, b, gmp, 0);
+    free(a), free(b);
+    /* runNaiveMuls: the same, continuing the stream */
+    if (!mk_rand_arrays(total_work, total_work, &a, &b)) return 1;
+    for (size_t i = 0; i < nsz; i++) ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], a, b, gmp, 0);
+    free(a), free(b);
+    /* runFFTMuls<FftPrime32>: mkRandArrays<32,32>(total_work/16) draws
+     * 2*total_work words per operand; the FFT sizes run total_work/16 words.
+     * The Goldilocks sweep (no reference counterpart) reuses those inputs. */
+    const uint64_t fft_work = total_work / 16;
+    if (!mk_rand_arrays(2 * total_work, fft_work, &a, &b)) return 1;
     /* p32 products take 8-bit coefficients: up to 2048 u32 words (1024 limbs) */
-    for (size_t i = 1; i < sizeof(sizes) / sizeof(sizes[0]); i++)
-        ok &= run_one(OP_MUL_FFT, device, total_work / 16, sizes[i], gmp, MIDINT_PRIME_P32);
+    for (size_t i = 1; i < nsz; i++)
+        ok &= run_one(OP_MUL_FFT, device, fft_work, sizes[i], a, b, gmp, MIDINT_PRIME_P32);
+    for (size_t i = 0; i < nsz; i++)
+        ok &= run_one(OP_MUL_FFT, device, fft_work, sizes[i], a, b, gmp, MIDINT_PRIME_GOLDILOCKS);
+    free(a), free(b);
     return ok ? 0 : 1;
 }

@@ -22,5 +22,7 @@ def test_harness_sweeps_validate(mi):
     ops = {x["op"] for x in lines}
     assert ops == {"add", "mul_lo", "mul_ntt", "mul_ntt_p32"}, ops
     assert len(lines) == 9 * 3 + 8
-    if "not found" not in out.stdout:
-        assert all(x["valid"] is True for x in lines), [x for x in lines if x["valid"] is not True]
+    if "libgmp.so.10 not found" in out.stdout:
+        pytest.skip("libgmp.so.10 is not installed on this machine: the harness ran every sweep but "
+                    "could not validate the results against GMP")
+    assert all(x["valid"] is True for x in lines), [x for x in lines if x["valid"] is not True]

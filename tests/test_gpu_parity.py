@@ -144,6 +144,63 @@ def test_add6(ctx, torch_dev, orc, m):
     assert np.array_equal(host(dr), orc.add6(a, b, m))
 
 
+# Instances longer than one workgroup's rows (4096 limbs) run chunk by chunk
+# with the carry kept between chunks: exact chunk multiples, one limb past
+# them, odd lengths (scalar loads) and a length that needs many chunks.
+LONG_ADD_SIZES = [4097, 8192, 8193, 16385, 32768, 100001]
+
+
+@pytest.mark.parametrize("m", LONG_ADD_SIZES)
+def test_add_long_instances(ctx, torch_dev, orc, m):
+    torch = torch_dev
+    n = 5
+    a, b = operands(m * 3 + 1, n, m)
+    # instance 4: a carry chain through every chunk boundary (all-ones plus 1
+    # in its lowest limb, with a zero limb near the end to stop it there)
+    A, B = a.reshape(n, m), b.reshape(n, m)
+    A[4] = MASK
+    B[4] = 0
+    B[4, 0] = 1
+    A[4, m - 3] = 0
+    da, db = dev(torch, a), dev(torch, b)
+    dr = empty(torch, n * m)
+    dc = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+    ctx.add(da, db, dr, m, carry=dc)
+    dr6 = empty(torch, n * m)
+    ctx.add6(da, db, dr6, m)
+    torch.cuda.synchronize()
+    want, wc = orc.add(a, b, m)
+    got = host(dr)
+    assert np.array_equal(got, want), (m, int(np.flatnonzero(got != want)[0]))
+    assert np.array_equal(dc.cpu().numpy(), wc), m
+    assert np.array_equal(host(dr6), orc.add6(a, b, m)), m
+    # 8-byte aligned only (scalar path) and in place
+    big_a = dev(torch, np.concatenate([[np.uint64(0)], a]))
+    big_b = dev(torch, np.concatenate([[np.uint64(0)], b]))
+    ctx.add(big_a[1:], big_b[1:], big_a[1:], m)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(big_a)[1:], want), m
+
+
+def test_add_long_strided(mi, ctx, torch_dev, orc):
+    """Long instances at instance strides above their length (the gaps stay untouched)."""
+    torch = torch_dev
+    n, m, sa, sb, sr = 3, 9000, 9003, 9010, 9001
+    rng = np.random.default_rng(77)
+    A = rng.integers(0, 2**64, size=n * sa, dtype=np.uint64)
+    B = rng.integers(0, 2**64, size=n * sb, dtype=np.uint64)
+    R = np.full(n * sr, 0x5A5A5A5A5A5A5A5A, dtype=np.uint64)
+    a = np.stack([A[i * sa:i * sa + m] for i in range(n)]).reshape(-1)
+    b = np.stack([B[i * sb:i * sb + m] for i in range(n)]).reshape(-1)
+    for op, want in (("add", orc.add(a, b, m)[0]), ("add6", orc.add6(a, b, m))):
+        dr = dev(torch, R)
+        ctx.strided(op, n, m, dev(torch, A), sa, dev(torch, B), sb, dr, sr)
+        torch.cuda.synchronize()
+        got = host(dr).reshape(n, sr)
+        assert np.array_equal(got[:, :m].reshape(-1), want), op
+        assert np.all(got[:, m:] == np.uint64(0x5A5A5A5A5A5A5A5A)), op
+
+
 MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256, 512, 777, 1024, 2048]
 
 
@@ -322,6 +379,46 @@ def test_poly_ntt(ctx, torch_dev, orc, m):
         assert np.array_equal(host(dc), host(dr))
 
 
+def test_scratch_users_on_two_streams(ctx, torch_dev, orc):
+    """poly, poly_ntt and the repacking u32 path share the context scratch;
+    calls on two streams (default priority) must each give their own exact
+    result. Both batches are large enough to overlap on the device if they
+    were not ordered."""
+    torch = torch_dev
+    n, m = 2048, 64
+    a1, b1 = operands(901, n, m)
+    a2, b2 = operands(902, n, m)
+    d = [dev(torch, x) for x in (a1, b1, a2, b2)]
+    r = [empty(torch, n * m) for _ in range(4)]
+    m32 = 2 * m - 1  # odd word count: the repacking path
+    w1 = a1.view(np.uint32)[: n * m32].copy()
+    w2 = b1.view(np.uint32)[: n * m32].copy()
+    dw1 = torch.from_numpy(w1.view(np.int32)).to("cuda:0")
+    dw2 = torch.from_numpy(w2.view(np.int32)).to("cuda:0")
+    rw = torch.empty(n * m32, dtype=torch.int32, device="cuda:0")
+    torch.cuda.synchronize()
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+    for _ in range(3):
+        ctx.poly(d[0], d[1], r[0], m, stream=s1)
+        ctx.poly(d[2], d[3], r[1], m, stream=s2)
+        ctx.poly_ntt(d[2], d[3], r[2], m, stream=s1)
+        ctx.mul_lo_u32(dw1, dw2, rw, m32, n, stream=s2)
+        ctx.poly_ntt(d[0], d[1], r[3], m, stream=s2)
+    torch.cuda.synchronize()
+    want1, want2 = orc.poly(a1, b1, m), orc.poly(a2, b2, m)
+    assert np.array_equal(host(r[0]), want1)
+    assert np.array_equal(host(r[1]), want2)
+    assert np.array_equal(host(r[2]), want2)
+    assert np.array_equal(host(r[3]), want1)
+    # u32 words: product mod 2^(32 m32) from the u64 oracle on zero-extended operands
+    za = np.zeros(n * m, dtype=np.uint64)
+    zb = np.zeros(n * m, dtype=np.uint64)
+    za.view(np.uint32).reshape(n, 2 * m)[:, :m32] = w1.reshape(n, m32)
+    zb.view(np.uint32).reshape(n, 2 * m)[:, :m32] = w2.reshape(n, m32)
+    want_w = orc.mul_lo(za, zb, m).view(np.uint32).reshape(n, 2 * m)[:, :m32].reshape(-1)
+    assert np.array_equal(rw.cpu().numpy().view(np.uint32), want_w)
+
+
 def midint_max_mul(ctx):
     from __graft_entry__ import load_package
 
@@ -429,7 +526,7 @@ def test_errors(ctx, torch_dev, mi):
     assert L.midint_add(h, 1, 8, p + 4, p, p, None, None) == 1      # misaligned
     assert L.midint_mul_full(h, 1, 4097, p, p, p, None) == 2         # too large
     assert L.midint_mul_ntt(h, 1, 1 << 20, p, p, p, None) == 2
-    assert L.midint_add(h, 1, 1 << 20, p, p, p, None, None) == 2
+    assert L.midint_add(h, 1, L.midint_max_limbs_add() + 1, p, p, p, None, None) == 2  # rejected before launch
 
 
 # ---------------------------------------------------------------- full size
