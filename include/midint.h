@@ -38,7 +38,8 @@ typedef enum {
     MIDINT_EHIP = 3,         /* HIP runtime error (launch, allocation, copy)   */
     MIDINT_ENOMEM = 4,       /* host or device allocation failed              */
     MIDINT_EBUSY = 5,        /* resource temporarily unavailable (admission)  */
-    MIDINT_ENOTFOUND = 6     /* unknown request / handle                       */
+    MIDINT_ENOTFOUND = 6,    /* unknown request / handle                       */
+    MIDINT_ETOOBIG = 7       /* request exceeds a fixed-size static buffer     */
 } midint_status;
 
 typedef struct midint_ctx midint_ctx;
@@ -174,6 +175,20 @@ midint_status midint_mul_full_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, con
                                   const uint32_t* d_b, uint32_t* d_r, void* stream);
 midint_status midint_poly_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                               uint32_t* d_r, void* stream);
+
+/* midint_mul_lo_u32 without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). Odd word counts (or pointers not 8-byte aligned) are repacked
+ * in a fixed per-device static array of 48 MiB instead of context scratch,
+ * so 3 * n * ceil(m32 / 2) must not exceed 6 * 2^20 limbs; larger requests
+ * return MIDINT_ETOOBIG before anything is enqueued. Even word counts with
+ * 8-byte aligned pointers use no scratch at all. Every call on one GPU shares
+ * that GPU's array: calls that overlap on one GPU (different streams,
+ * contexts or threads) race and must be serialised by the caller. Results
+ * equal midint_mul_lo_u32's. */
+midint_status midint_mul_lo_u32_noalloc(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                        const uint32_t* d_b, uint32_t* d_r, void* stream);
 
 /* ------------------------------------------------- prime-field variants */
 /* The reference's FFT primes (FftPrime32 / FftPrime64,

@@ -65,6 +65,7 @@ _SIGS = {
     "midint_mul_lo_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_full_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_poly_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_mul_lo_u32_noalloc": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_ntt_prime": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_prepare": (_int, [_vp, _u32]),
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),

@@ -109,6 +109,7 @@ const char* midint_strerror(midint_status s) {
         case MIDINT_ENOMEM: return "out of memory";
         case MIDINT_EBUSY: return "resource busy";
         case MIDINT_ENOTFOUND: return "not found";
+        case MIDINT_ETOOBIG: return "request exceeds the fixed-size static scratch";
     }
     return "unknown status";
 }
@@ -128,6 +129,10 @@ midint_status midint_init(midint_ctx** out, int device) {
     midint_ctx* c = new (std::nothrow) midint_ctx;
     if (!c) return MIDINT_ENOMEM;
     c->device = device;
+    // device address of the no-allocation entry point's static scratch, looked
+    // up once per device with the device current; a failure is kept and
+    // reported by that entry point only
+    (void)noalloc_setup(device);
     *out = c;
     return MIDINT_OK;
 }
@@ -420,6 +425,33 @@ midint_status midint_mul_full_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, con
 midint_status midint_poly_u32(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a, const uint32_t* d_b,
                               uint32_t* d_r, void* stream) {
     return u32_op(ctx, Op::Poly, n, m32, d_a, 0, d_b, 0, d_r, 0, nullptr, stream);
+}
+
+midint_status midint_mul_lo_u32_noalloc(midint_ctx* ctx, uint32_t n, uint32_t m32, const uint32_t* d_a,
+                                        const uint32_t* d_b, uint32_t* d_r, void* stream) {
+    if (!ctx || m32 == 0) return MIDINT_EINVAL;
+    if (n == 0) return MIDINT_OK;
+    if (!d_a || !d_b || !d_r || ((((uintptr_t)d_a) | ((uintptr_t)d_b) | ((uintptr_t)d_r)) & 3u)) return MIDINT_EINVAL;
+    const uint32_t m = (m32 + 1) / 2;
+    if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
+    midint_status st = bind_device(ctx);
+    if (st != MIDINT_OK) return st;
+    hipStream_t s = (hipStream_t)stream;
+    if (!(m32 & 1u) && aligned8(d_a) && aligned8(d_b) && aligned8(d_r))  // the u64 layout: no scratch
+        return hip_status(launch_mul(n, m, (const uint64_t*)d_a, (const uint64_t*)d_b, (uint64_t*)d_r, false, s));
+    const size_t nm = (size_t)n * m;
+    if (3 * nm > kNoAllocScratchLimbs) return MIDINT_ETOOBIG;
+    uint64_t* scr = nullptr;
+    hipError_t e = noalloc_scratch(ctx->device, &scr);
+    if (e != hipSuccess) return hip_status(e);
+    uint64_t* pa = scr;
+    uint64_t* pb = pa + nm;
+    uint64_t* pr = pb + nm;
+    e = launch_pack_u32(n, m32, m, d_a, m32, pa, s);
+    if (e == hipSuccess) e = launch_pack_u32(n, m32, m, d_b, m32, pb, s);
+    if (e == hipSuccess) e = launch_mul(n, m, pa, pb, pr, false, s);
+    if (e == hipSuccess) e = launch_unpack_u32(n, m32, m, pr, d_r, m32, nullptr, s);
+    return hip_status(e);
 }
 
 // ------------------------------------------------------------ CGBN layout

@@ -99,6 +99,12 @@ hipError_t launch_pack_u32(uint64_t n, uint32_t m32, uint32_t m, const uint32_t*
                            hipStream_t s);
 hipError_t launch_unpack_u32(uint64_t n, uint32_t w32, uint32_t ms, const uint64_t* src, uint32_t* dst,
                              uint64_t ds, uint8_t* carry, hipStream_t s);
+// Static scratch of midint_mul_lo_u32_noalloc (u32.hip): limbs, the lookup of
+// its device address on the current device (once per device, at midint_init)
+// and that address for a device.
+constexpr size_t kNoAllocScratchLimbs = (size_t)6 << 20;  // 48 MiB per device
+hipError_t noalloc_setup(int device);
+hipError_t noalloc_scratch(int device, uint64_t** out);
 
 // NTT over the reference's primes (p32, p64) and Goldilocks: tables in the
 // field's multiplier form (Montgomery for p32/p64), and the launcher.

@@ -4,6 +4,8 @@
 // that is byte-for-byte the u64 layout with m32/2 limbs; for odd m32 the
 // instances are repacked into whole u64 limbs (top half zero) in scratch
 // and back, which these two kernels do.
+#include <mutex>
+
 #include "midint_internal.h"
 
 namespace midint {
@@ -55,6 +57,45 @@ hipError_t launch_unpack_u32(uint64_t n, uint32_t w32, uint32_t ms, const uint64
     if (2ull * ms < (uint64_t)w32 + (carry ? 1 : 0) || ds < w32) return hipErrorInvalidValue;  // carry word must exist
     unpack_u32_kernel<<<grid_for(n * w32), 256, 0, s>>>(n, w32, ms, src, dst, ds, carry);
     return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Static scratch of midint_mul_lo_u32_noalloc: the repacked operands and the
+// product (3 * n * ceil(m32 / 2) limbs) live in this fixed device array
+// instead of context scratch, so that entry point never allocates. The
+// array's name is a host-side stand-in in host code; its device address is
+// looked up once per device (noalloc_setup, called by midint_init with the
+// device current) and only that address is handed to kernels.
+static __device__ __align__(256) uint64_t scratch[kNoAllocScratchLimbs];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+uint64_t* g_noalloc_ptr[kMaxDevices];
+hipError_t g_noalloc_err[kMaxDevices];
+}  // namespace
+
+hipError_t noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return hipSuccess;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    g_noalloc_err[device] = e;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return e;
+    }
+    g_noalloc_ptr[device] = (uint64_t*)p;
+    return hipSuccess;
+}
+
+hipError_t noalloc_scratch(int device, uint64_t** out) {
+    if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    *out = g_noalloc_ptr[device];
+    if (*out) return hipSuccess;
+    return g_noalloc_err[device] != hipSuccess ? g_noalloc_err[device] : hipErrorNotInitialized;
 }
 
 }  // namespace midint
