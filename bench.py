@@ -2,17 +2,32 @@
 
 Headline (``value``): big-integer multiplications per second at 8192 bits --
 16384 pairs of 128-limb operands, full 256-limb products (classical
-schoolbook, on the kernel the tuning file selects for 128 limbs),
-inputs resident in HBM, one step = one batch. The same JSON line also
-carries the batched-addition bandwidth (65536 x 64 limbs, HBM roofline) and
-the NTT multiplication rate (1024 x 2048 limbs).
+schoolbook, on the kernel the tuning file selects for 128 limbs), inputs
+resident in HBM, one step = one batch. The same JSON line carries, under
+``extra``:
+
+  * add_65536x64   -- batched addition (HBM roofline) beside the streaming
+                      ceiling of the same access pattern (r = a + b per limb,
+                      no carries, same launch shape and buffers), and the
+                      six-add composite (a6pb10Ker) on the same buffers;
+  * ntt_1024x2048  -- NTT multiplication of 131072-bit integers;
+  * composites     -- polyKerQ / polyFttKer equivalents with the reference's
+                      Gu32ops metric (cuda/main.cu:317-351, 526-570);
+  * ragged_50000   -- one persistent hipGraph launch vs plain launches per
+                      power-of-two size class and per exact operand sizes;
+  * serving        -- Poisson streams up to a saturating burst;
+  * cpu            -- GMP (the library the reference's CPU path calls,
+                      cuda/helper.h:146-170) on every workload, all host
+                      threads used and one core.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu]
+                       [--only all|core] [--calibrate]
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -26,6 +41,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 INT8_PEAK_TOPS = 5000.0  # MI355X dense int8 MFMA, 5.0 POP/s
 MIB = 1 << 20
+CPU_THREAD_CAP = 16  # host CPU share of one GPU on the measurement boxes
 
 
 def log(msg):
@@ -70,6 +86,13 @@ def mfma_work(m, wpi=0):
     return 2 * D * D, 2 * 32768 * mfmas
 
 
+def gu32ops_of_mul(m):
+    """The reference's numAd32OpsOfMultInst (cuda/helper.h:36-43) for an
+    operand of m u64 limbs: 300 n log2 n with n = 2m u32 words."""
+    n = 2 * m
+    return 300 * n * int(math.log2(n))
+
+
 def timed(torch, fn, steps, warmup):
     """Mean ms per call of fn() over `steps` calls after `warmup` untimed
     calls, HIP events on torch's current stream (the stream the kernels are
@@ -101,33 +124,148 @@ def measured_traffic(name):
         return None
 
 
-def cpu_baseline_mul(a, b, m, threads, budget_s=1.0):
-    """Oracle restatement (port of the reference algorithm) and GMP mpn_mul on
-    the host cores, repeated over the batch until ~budget_s of wall time."""
-    import oracle
+def cpu_threads():
+    """Host threads for the CPU baselines: the CPUs this process may run on,
+    capped at one GPU's share of a measurement box."""
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover - non-Linux
+        avail = os.cpu_count() or 1
+    return max(1, min(CPU_THREAD_CAP, avail))
 
-    out = {}
-    for kind, fn in (("port", lambda x, y: oracle.mul_full(x, y, m, threads=threads)),
-                     ("gmp", lambda x, y: oracle.gmp_mul_full(x, y, m, threads=threads))):
-        if kind == "gmp" and not oracle.gmp_available():
-            continue
-        reps, t0 = 0, time.perf_counter()
-        while True:
-            fn(a, b)
-            reps += 1
-            el = time.perf_counter() - t0
-            if el >= budget_s:
-                break
-        out[kind] = (reps * (a.size // m)) / el
+
+def rate(fn, units, budget_s):
+    """Units per second of fn() repeated for about budget_s of wall time
+    (at least twice; the first call is a warm-up)."""
+    fn()
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        fn()
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s and reps >= 2:
+            return reps * units / el
+
+
+def cpu_baselines(orc, inputs, threads, budget_s=1.2):
+    """GMP on each workload with `threads` threads and with one, plus the
+    oracle's plain-C restatement of the classical kernel on the headline."""
+    out = {"threads": threads, "host_logical_cpus": os.cpu_count(), "library": "libgmp (mpn_add_n / mpn_mul / "
+           "mpn_add), the library the reference's CPU path calls (cuda/helper.h:146-170, mpz_add / mpz_mul)"}
+    if not orc.gmp_available():
+        out["gmp"] = None
+        return out
+    a1, b1 = inputs["add"]
+    a2, b2 = inputs["mul"]
+    a3, b3 = inputs["ntt"]
+    ops, ra, rb, rl = inputs["ragged"]
+    # D2 sample: 2048 of the 16384 pairs; D3: 64 of the 1024 pairs
+    k2, k3 = 2048, 64
+    s2a, s2b = a2[: k2 * 128], b2[: k2 * 128]
+    s3a, s3b = a3[: k3 * 2048], b3[: k3 * 2048]
+    n_mul = sum(1 for o in ops if o[0] == 1)
+    prep = orc.gmp_ragged_prepare(ops)
+    rbuf = np.zeros(rl, dtype=np.uint64)
+    for t, key in ((threads, "all"), (1, "one_core")):
+        out[key] = {
+            "add_65536x64_GB_per_s": rate(lambda: orc.gmp_add_batch(a1, b1, 64, threads=t), 3 * a1.size * 8,
+                                          budget_s) / 1e9,
+            "mul_16384x128_muls_per_s": rate(lambda: orc.gmp_mul_full(s2a, s2b, 128, threads=t), k2, budget_s),
+            "mul_1024x2048_muls_per_s": rate(lambda: orc.gmp_mul_full(s3a, s3b, 2048, threads=t), k3, budget_s),
+            "ragged_50000_ops_per_s": rate(lambda: orc.gmp_ragged(prep, ra, rb, rl, threads=t, out=rbuf), len(ops),
+                                           budget_s),
+        }
+        out[key]["ragged_50000_muls_per_s"] = out[key]["ragged_50000_ops_per_s"] * n_mul / len(ops)
+    out["port_mul_16384x128_muls_per_s"] = rate(lambda: orc.mul_full(s2a, s2b, 128, threads=threads), k2, budget_s)
+    out["samples"] = {"add": "all 65536 pairs", "mul_16384x128": f"{k2} of the 16384 pairs",
+                      "mul_1024x2048": f"{k3} of the 1024 pairs", "ragged": "all 50000 ops"}
     return out
 
 
-def bench_ragged(torch, mi, ctx, args):
-    """Ragged batch (50000 ops): one persistent hipGraph launch vs one
-    fixed-size launch per (op, power-of-two size class)."""
-    from workloads import make_ragged, size_class
+# --------------------------------------------------------------------------- workloads
+def bench_add(torch, mi, ctx, args):
+    """65536 x 64-limb addition over 4 rotating 96 MiB buffer sets (past the
+    256 MiB Infinity Cache), the streaming ceiling of the same access
+    pattern, and the six-add composite."""
+    from workloads import make_inputs
 
-    ops, ra, rb, rlimbs = make_ragged(mi.gen_limbs)
+    na, ma = 65536, 64
+    aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
+    sets = [(to_dev(torch, aa), to_dev(torch, ba), torch.empty(na * ma, dtype=torch.int64, device="cuda:0"))
+            for _ in range(4)]
+    it = [0]
+
+    def nxt():
+        x = sets[it[0] & 3]
+        it[0] += 1
+        return x
+
+    def add_step():
+        x, y, z = nxt()
+        ctx.add(x, y, z, ma)
+
+    def stream_step():
+        x, y, z = nxt()
+        mi.probe_stream(x, y, z)
+
+    def add6_step():
+        x, y, z = nxt()
+        ctx.add6(x, y, z, ma)
+
+    steps, warm = 2 * args.steps, max(4, args.warmup)
+    # interleaved repeats so that the add and its ceiling see the same clocks
+    ms_add, ms_ceil = [], []
+    for _ in range(3):
+        ms_add.append(timed(torch, add_step, steps, warm))
+        ms_ceil.append(timed(torch, stream_step, steps, warm))
+    ms_a, ms_c = min(ms_add), min(ms_ceil)
+    ms_6 = timed(torch, add6_step, steps, warm)
+    byts = 3 * na * ma * 8
+    gbs, gbs_c, gbs_6 = (byts / (t * 1e-3) / 1e9 for t in (ms_a, ms_c, ms_6))
+    log(f"add {na} x {ma}: {ms_a * 1e3:.2f} us ({gbs:.0f} GB/s); streaming ceiling {ms_c * 1e3:.2f} us "
+        f"({gbs_c:.0f} GB/s); add6 {ms_6 * 1e3:.2f} us ({gbs_6:.0f} GB/s)")
+    del sets
+    return {
+        "us_per_launch": ms_a * 1e3, "GB_per_s": gbs,
+        "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": gbs / HBM_PEAK_GBS, "traffic": measured_traffic("add_65536x64")},
+        "stream_ceiling": {"us_per_launch": ms_c * 1e3, "GB_per_s": gbs_c, "frac_of_peak": gbs_c / HBM_PEAK_GBS,
+                           "add_frac_of_ceiling": gbs / gbs_c,
+                           "what": "r = a + b per u64 limb, no carries, same 4 rotating buffer sets and the same "
+                                   "launch shape (4-wave blocks, 2 rows of 128 limbs per wave, 16-byte "
+                                   "non-temporal accesses); best of 3 interleaved runs of each"},
+        "add6": {"us_per_launch": ms_6 * 1e3, "GB_per_s": gbs_6, "mil_instances_per_s": na / ms_6 * 1e-3,
+                 "metric": "GB/s = 3 n m 8 / t, as the reference's SIX Additions line (cuda/main.cu:145-153)"},
+        "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)",
+    }
+
+
+def bench_composites(torch, mi, ctx, args, d2, d3):
+    """poly (polyKerQ) on the headline operands and poly_ntt (polyFttKer) on
+    the NTT operands; Gu32ops = 4 n numAd32OpsOfMultInst(m) / t as the
+    reference reports them (cuda/main.cu:340-346, 560-566)."""
+    (da, db, n, m), (dan, dbn, nn, mn) = d2, d3
+    r = torch.empty(n * m, dtype=torch.int64, device="cuda:0")
+    ms_p = timed(torch, lambda: ctx.poly(da, db, r, m), max(5, args.steps // 4), 3)
+    rn = torch.empty(nn * mn, dtype=torch.int64, device="cuda:0")
+    ms_pn = timed(torch, lambda: ctx.poly_ntt(dan, dbn, rn, mn), max(3, args.steps // 40), 2)
+    out = {}
+    for key, ms, nb, mb in (("poly_16384x128", ms_p, n, m), ("poly_ntt_1024x2048", ms_pn, nn, mn)):
+        out[key] = {"us_per_batch": ms * 1e3, "polys_per_s": nb / (ms * 1e-3),
+                    "Gu32ops_per_s": 4 * nb * gu32ops_of_mul(mb) / (ms * 1e-3) / 1e9,
+                    "algorithmic_hbm_bytes": 3 * nb * mb * 8}
+        log(f"{key}: {ms * 1e3:.1f} us/batch")
+    del r, rn
+    return out
+
+
+def bench_ragged(torch, mi, ctx, args, rag):
+    """Ragged batch (50000 ops): one persistent hipGraph launch vs plain
+    fixed-size launches, one per (op, power-of-two size class) and one per
+    exact (op, n_a, n_b), on zero-padded operands."""
+    from workloads import size_class
+
+    ops, ra, rb, rlimbs = rag
     dra, drb = to_dev(torch, ra), to_dev(torch, rb)
     drr = torch.zeros(rlimbs, dtype=torch.int64, device="cuda:0")
     n_mul = sum(1 for o in ops if o[0] == 1)
@@ -135,62 +273,96 @@ def bench_ragged(torch, mi, ctx, args):
     rsteps = max(5, args.steps // 20)
     ms_rag = timed(torch, plan.launch, rsteps, 2)
     pstats = plan.stats()
-    # plain per-size launches: one fixed-size launch per (op, power-of-two class)
-    classes = {}
-    for i, (kind, na, nb, oa, ob, orr) in enumerate(ops):
-        classes.setdefault((kind, size_class(max(na, nb))), []).append(i)
-    plain = []
-    for (kind, M), idx in sorted(classes.items()):
-        n_c = len(idx)
-        pa = np.zeros((n_c, M), dtype=np.uint64)
-        pb = np.zeros((n_c, M), dtype=np.uint64)
-        for j, i in enumerate(idx):
-            _, na, nb, oa, ob, _ = ops[i]
-            pa[j, :na] = ra[oa:oa + na]
-            pb[j, :nb] = rb[ob:ob + nb]
-        out_l = 2 * n_c * M if kind == 1 else n_c * M
-        plain.append((kind, M, to_dev(torch, pa), to_dev(torch, pb),
-                      torch.empty(out_l, dtype=torch.int64, device="cuda:0"),
-                      torch.empty(n_c, dtype=torch.uint8, device="cuda:0")))
 
-    def plain_step():
-        for kind, M, x, y, z, c in plain:
+    def grouped(key):
+        groups = {}
+        for i, (kind, na, nb, oa, ob, orr) in enumerate(ops):
+            groups.setdefault(key(kind, na, nb), []).append(i)
+        return groups
+
+    def plain_launches(groups):
+        """groups: [(kind, M, op indices)]. One packed padded buffer per
+        operand; each group is a batch-major slice of it at its size M."""
+        metas, tot_in, tot_out = [], 0, 0
+        for kind, M, idx in groups:
+            n_c = len(idx)
+            metas.append((kind, M, idx, tot_in, tot_out))
+            tot_in += n_c * M
+            tot_out += n_c * (2 * M if kind == 1 else M)
+        pa = np.zeros(tot_in, dtype=np.uint64)
+        pb = np.zeros(tot_in, dtype=np.uint64)
+        for kind, M, idx, off, _ in metas:
+            for j, i in enumerate(idx):
+                _, na, nb, oa, ob, _ = ops[i]
+                pa[off + j * M: off + j * M + na] = ra[oa:oa + na]
+                pb[off + j * M: off + j * M + nb] = rb[ob:ob + nb]
+        x, y = to_dev(torch, pa), to_dev(torch, pb)
+        z = torch.empty(tot_out, dtype=torch.int64, device="cuda:0")
+        c = torch.empty(len(ops), dtype=torch.uint8, device="cuda:0")
+        calls = []
+        for kind, M, idx, off, oo in metas:
+            n_c = len(idx)
+            xa, yb = x[off:off + n_c * M], y[off:off + n_c * M]
             if kind == 1:
-                ctx.mul_full(x, y, z, M)
+                calls.append((ctx.mul_full, (xa, yb, z[oo:oo + 2 * n_c * M], M)))
             else:
-                ctx.add(x, y, z, M, carry=c)
+                calls.append((ctx.add, (xa, yb, z[oo:oo + n_c * M], M, c[:n_c])))
 
-    ms_plain = timed(torch, plain_step, rsteps, 2)
-    log(f"ragged {len(ops)} ops: persistent {ms_rag:.3f} ms ({pstats}), plain per-size {ms_plain:.3f} ms, "
-        f"speedup {ms_plain / ms_rag:.2f}x")
-    del plan, plain, dra, drb, drr
+        def step():
+            for f, a in calls:
+                f(*a)
+
+        return step, len(calls), (x, y, z, c)
+
+    classes = grouped(lambda k, na, nb: (k, size_class(max(na, nb))))
+    step_c, launches_c, keep_c = plain_launches([(k, M, idx) for (k, M), idx in sorted(classes.items())])
+    ms_plain = timed(torch, step_c, rsteps, 2)
+    del keep_c
+    # exact sizes: the kernels are square, so each distinct (op, n_a, n_b)
+    # is one launch at M = max(n_a, n_b)
+    exact = grouped(lambda k, na, nb: (k, na, nb))
+    step_e, launches_e, keep_e = plain_launches([(k, max(na, nb), idx) for (k, na, nb), idx in sorted(exact.items())])
+    ms_exact = timed(torch, step_e, 2, 1)
+    del keep_e
+    log(f"ragged {len(ops)} ops: persistent {ms_rag:.3f} ms ({pstats}); plain per size class {ms_plain:.3f} ms "
+        f"({launches_c} launches, {ms_plain / ms_rag:.2f}x); plain per exact size {ms_exact:.1f} ms "
+        f"({launches_e} launches, {ms_exact / ms_rag:.1f}x)")
+    del plan, dra, drb, drr
     return {"ops": len(ops), "muls": n_mul, "persistent_ms": ms_rag, "plain_per_size_ms": ms_plain,
             "speedup": ms_plain / ms_rag, "muls_per_s_persistent": n_mul / (ms_rag * 1e-3),
-            "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plan": pstats,
-            "note": "one hipGraph launch (counter reset + persistent kernel) vs one fixed-size "
-                    "launch per (op, power-of-two size class) on zero-padded operands"}
+            "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plain_launches": launches_c,
+            "plain_per_exact_size_ms": ms_exact, "plain_exact_launches": launches_e,
+            "speedup_vs_exact": ms_exact / ms_rag, "plan": pstats,
+            "note": "one hipGraph launch (counter reset, matrix-core node and persistent kernel) vs one fixed-size "
+                    "launch per (op, power-of-two size class) and vs one launch per distinct (op, n_a, n_b), each "
+                    "on operands zero-padded to the launch size; eager launches from Python on one stream"}
 
 
 def bench_serving(mi, ctx):
-    """Serving loop under Poisson arrivals at three offered rates."""
+    """Serving loop: Poisson arrivals at three offered rates and a burst in
+    which every request arrives at once (the saturated server)."""
     from workloads import make_requests
 
     reqs = make_requests(mi.gen_limbs, 3000)
     serving = []
-    for qps in (1000, 4000, 16000):
+    for qps in (1000, 16000, 64000, float("inf")):
         srv = mi.Server(ctx, hbm_budget=4 << 30, streams=4)
-        res = mi.serve_poisson(srv, reqs, qps=qps, seed=qps, keep_results=False)
+        res = mi.serve_poisson(srv, reqs, qps=qps, seed=int(qps) if qps != float("inf") else 1,
+                               keep_results=False)
         st = srv.stats()
         srv.close()
         res.pop("results")
         res["batches"] = st["batches_launched"]
         res["mean_batch_ops"] = st["ops_in_batches"] / max(1, st["batches_launched"])
+        res["deferred_admissions"] = st["deferred_admissions"]
         serving.append(res)
-        log(f"serving qps {qps}: p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, "
-            f"{res['ops_per_s']:.0f} ops/s, {res['batches']} batches")
-
+        log(f"serving {'burst' if qps == float('inf') else f'qps {qps}'}: p50 {res['p50_ms']:.2f} ms, "
+            f"p99 {res['p99_ms']:.2f} ms, {res['ops_per_s']:.0f} ops/s, {res['batches']} batches, "
+            f"{res['mean_batch_ops']:.1f} ops/batch")
     return {"requests": len(reqs), "mix": "70% mul / 30% add, operands 8..4096 limbs log-uniform",
-            "points": serving}
+            "points": serving,
+            "note": "offered_qps null = every request submitted at time 0 (saturation); latency = completion "
+                    "- scheduled arrival on the host clock"}
 
 
 def main():
@@ -215,7 +387,7 @@ def main():
     import torch
 
     from __graft_entry__ import load_package
-    from workloads import make_inputs
+    from workloads import make_inputs, make_ragged
 
     mi = load_package()
     mi.lib()
@@ -256,7 +428,7 @@ def main():
                     "frac": tops / INT8_PEAK_TOPS, "traffic": measured_traffic("mul_full_16384x128"),
                     "work": f"int8 ops per launch = {n} x 2 x {8 * m + 1}^2 (balanced base-256 digit products "
                             f"of the {64 * m}-bit operands); MFMA-issued ops incl. tile padding: "
-                            f"{n * issued / (ms * 1e-3) / 1e12:.0f} TOP/s",
+                            f"{n * issued / (ms * 1e-3) / 1e12:.0f} TOP/s ({issued / useful:.2f}x useful)",
                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                     "peak_source": "MI355X published dense int8 MFMA rate"}
     else:
@@ -270,31 +442,19 @@ def main():
     ctx.ntt_prepare(mn)
     ms_ntt = timed(torch, lambda: ctx.mul_ntt(dan, dbn, drn, mn), max(5, args.steps // 20), 2)
     ntt_pp = nn * 360448 * 4  # radix-2-equivalent modmuls per product (3 transforms + pointwise), 4 pp each
-    log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch")
-    del dan, dbn, drn
+    log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch ({ms_ntt * 1e3 / nn:.3f} us per product)")
+    del drn
 
     # ---------------------------------------------------------- add (HBM)
-    na, ma = 65536, 64
-    aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
-    sets = []
-    for _ in range(4):  # 4 x 96 MiB > the 256 MiB Infinity Cache
-        sets.append((to_dev(torch, aa), to_dev(torch, ba), torch.empty(na * ma, dtype=torch.int64, device="cuda:0")))
-    it = [0]
+    add = bench_add(torch, mi, ctx, args)
 
-    def add_step():
-        x, y, z = sets[it[0] & 3]
-        it[0] += 1
-        ctx.add(x, y, z, ma)
-
-    ms_add = timed(torch, add_step, 2 * args.steps, max(4, args.warmup))
-    add_bytes = 3 * na * ma * 8
-    add_gbs = add_bytes / (ms_add * 1e-3) / 1e9
-    log(f"add {na} x {ma}: {ms_add * 1e3:.2f} us/launch, {add_gbs:.0f} GB/s")
-
-    ragged, serving = None, None
+    composites = ragged = serving = None
+    rag = make_ragged(mi.gen_limbs)
     if args.only == "all":
-        ragged = bench_ragged(torch, mi, ctx, args)
+        composites = bench_composites(torch, mi, ctx, args, (da, db, n, m), (dan, dbn, nn, mn))
+        ragged = bench_ragged(torch, mi, ctx, args, rag)
         serving = bench_serving(mi, ctx)
+    del dan, dbn
 
     if args.calibrate:
         src = torch.ones(32 * MIB, dtype=torch.int64, device="cuda:0")
@@ -304,17 +464,24 @@ def main():
         torch.cuda.synchronize()
         del src, dst
 
-    # ---------------------------------------------------------- CPU baseline
-    cpu = None
+    # ---------------------------------------------------------- CPU baselines
+    cpu, cpu_line = None, None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        k = 2048
-        base = cpu_baseline_mul(a[: k * m], b[: k * m], m, threads)
-        cpu = {"value": base.get("port"), "unit": "muls/s", "cores": threads, "kind": "port",
-               "sample": f"{k} of the 16384 pairs (128 x 128 limbs, full product), repeated for ~1 s wall "
-                         f"on {threads} threads; oracle C restatement",
-               "gmp_mpn_mul_muls_per_s": base.get("gmp")}
-        log(f"cpu baseline: {cpu}")
+        import oracle
+
+        aa, ba = make_inputs("add_65536x64", mi.gen_limbs)
+        threads = cpu_threads()
+        cpu = cpu_baselines(oracle, {"add": (aa, ba), "mul": (a, b), "ntt": (an, bn), "ragged": rag}, threads)
+        if cpu.get("all"):
+            cpu_line = {"value": cpu["all"]["mul_16384x128_muls_per_s"], "unit": "muls/s", "cores": threads,
+                        "kind": "reference",
+                        "sample": "2048 of the 16384 pairs (128 x 128 limbs, full product) with libgmp mpn_mul "
+                                  "(the library the reference's CPU path calls, cuda/helper.h:146-170), repeated "
+                                  f"for ~1.2 s on {threads} threads (OpenMP over pairs)",
+                        "one_core": cpu["one_core"]["mul_16384x128_muls_per_s"],
+                        "host_logical_cpus": os.cpu_count(),
+                        "port_muls_per_s": cpu["port_mul_16384x128_muls_per_s"]}
+        log(f"cpu baselines: {json.dumps(cpu)}")
     out = {
         "metric": "big-int muls/s at 8192-bit",
         "value": muls_per_s,
@@ -326,20 +493,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "i8" if on_mfma else "u64",
+        "dtype": "u64",
+        "dtype_note": ("exact u64 limb arithmetic; the products are computed through int8 matrix-core (MFMA) "
+                       "digit products of balanced base-256 digits, every result bit-exact against GMP")
+        if on_mfma else "exact u64 limb arithmetic on the VALU",
         "data": "synthetic (xoshiro256** limbs, seed 0x6D6964696E74 + 1)",
         "config": {"workload": "mul_16384x128: 16384 pairs of 8192-bit ints, full 256-limb products, "
                                + ("classical kernel on the int8 matrix cores (balanced 8-bit digits)" if on_mfma
                                   else "classical kernel (u32 words, VALU)"), "instances": n, "limbs": m},
         "roofline": roofline,
-        "cpu_baseline": cpu,
+        "cpu_baseline": cpu_line,
         "extra": {
-            "add_65536x64": {"us_per_launch": ms_add * 1e3, "GB_per_s": add_gbs,
-                             "roofline": {"bound": "hbm", "achieved": add_gbs, "peak": HBM_PEAK_GBS,
-                                          "unit": "GB/s", "frac": add_gbs / HBM_PEAK_GBS,
-                                          "traffic": measured_traffic("add_65536x64")},
-                             "note": "3*N*m*8 bytes per launch; 4 rotating buffer sets (384 MiB)"},
-            "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "muls_per_s": nn / (ms_ntt * 1e-3),
+            "add_65536x64": add,
+            "ntt_1024x2048": {"us_per_batch": ms_ntt * 1e3, "us_per_product": ms_ntt * 1e3 / nn,
+                              "muls_per_s": nn / (ms_ntt * 1e-3),
                               "roofline": {"bound": "valu", "achieved": ntt_pp / (ms_ntt * 1e-3) / 1e12,
                                            "peak": mad_rate / 1e12,
                                            "unit": "T partial-products/s (360448 radix-2-equivalent modmuls x 4 "
@@ -347,9 +514,11 @@ def main():
                                            "frac": ntt_pp / (ms_ntt * 1e-3) / mad_rate,
                                            "traffic": measured_traffic("mul_ntt_1024x2048")}},
             "mul_valu_equivalent": valu_roofline,
+            "composites": composites,
             "ragged_50000": ragged,
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
             "serving": serving,
+            "cpu": cpu,
         },
     }
     ctx.close()

@@ -355,6 +355,15 @@ midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_p
  * operands: the ceiling of the multiplication loop without LDS traffic. */
 midint_status midint_probe_mac(int device, double* pp_per_s);
 
+/* HBM streaming ceiling of the addition's access pattern: d_r = d_a + d_b
+ * per u64 limb, no carries, in the add kernel's launch shape (blocks of 4
+ * waves, 2 rows of 128 limbs per wave, 16-byte non-temporal accesses) on the
+ * current device. `limbs` even, pointers 16-byte aligned. Asynchronous on
+ * `stream`. The add's bandwidth is reported against it. No reference
+ * counterpart (the reference times only its own add, cuda/main.cu:91-115). */
+midint_status midint_probe_stream(uint64_t limbs, const uint64_t* d_a, const uint64_t* d_b, uint64_t* d_r,
+                                  void* stream);
+
 /* ------------------------------------------------------ data generation */
 /* xoshiro256** stream (state seeded by four splitmix64 draws of `seed`)
  * filling `count` limbs; the generator of the benchmark workloads. */

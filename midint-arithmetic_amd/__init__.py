@@ -66,6 +66,7 @@ _SIGS = {
     "midint_mul_full_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_poly_u32": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_mul_lo_u32_noalloc": (_int, [_vp, _u32, _u32, _vp, _vp, _vp, _vp]),
+    "midint_probe_stream": (_int, [_u64, _vp, _vp, _vp, _vp]),
     "midint_mul_ntt_prime": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_prepare": (_int, [_vp, _u32]),
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
@@ -394,16 +395,20 @@ class Server:
 
 def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s=120.0):
     """Drives `server` with `requests` [(kind, a, b)] arriving as a Poisson
-    stream of rate `qps`. Latency of a request = completion - scheduled
-    arrival (host monotonic clock; includes time waiting in the queue).
-    Returns dict(p50_ms, p99_ms, mean_ms, ops_per_s, results{index: limbs})."""
+    stream of rate `qps` (qps = inf: every request arrives at time 0, a
+    burst that saturates the server). Latency of a request = completion -
+    scheduled arrival (host monotonic clock; includes time waiting in the
+    queue). Returns dict(p50_ms, p99_ms, mean_ms, ops_per_s, offered_qps
+    (None for a burst), results{index: limbs})."""
+    import math
     import time
 
     import numpy as np
 
     rng = np.random.default_rng(seed)
     n = len(requests)
-    arrivals = np.cumsum(rng.exponential(1.0 / qps, size=n))
+    burst = math.isinf(qps)
+    arrivals = np.zeros(n) if burst else np.cumsum(rng.exponential(1.0 / qps, size=n))
     out_len = [(len(a) + len(b)) if k == 1 else max(len(a), len(b)) + 1 for k, a, b in requests]
     id2idx, lat, results = {}, np.zeros(n), {}
     submit_late = np.zeros(n)
@@ -430,7 +435,8 @@ def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s
             last = time.perf_counter()
     span = last - (t0 + arrivals[0])
     return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
-            "mean_ms": float(lat.mean() * 1e3), "ops_per_s": n / span, "offered_qps": qps, "requests": n,
+            "mean_ms": float(lat.mean() * 1e3), "ops_per_s": n / span,
+            "offered_qps": None if burst else qps, "requests": n,
             "results": results}
 
 
@@ -470,6 +476,11 @@ def tune_get(key: str) -> int:
     v = ctypes.c_int64()
     check(lib().midint_tune_get(key.encode(), ctypes.byref(v)), f"midint_tune_get({key})")
     return v.value
+
+
+def probe_stream(a, b, r, stream=None):
+    """r = a + b per u64 limb without carries (the add's streaming ceiling)."""
+    check(lib().midint_probe_stream(a.numel(), _ptr(a), _ptr(b), _ptr(r), _stream(stream)), "midint_probe_stream")
 
 
 def probe_valu(device=0):

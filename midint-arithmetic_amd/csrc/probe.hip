@@ -69,9 +69,47 @@ __global__ __launch_bounds__(256) void probe_mac(uint32_t iters, uint32_t seed, 
     if (s == 0x123456789ull) out[0] = s;
 }
 
+// HBM streaming ceiling of the addition's access pattern: r = a + b per
+// u64 limb with no carries, in the add kernel's launch shape for short
+// instances (blocks of 4 waves, each wave 2 rows of 128 limbs, lane l holding
+// limbs 2l and 2l+1 of a row, 16-byte non-temporal loads and stores).
+typedef unsigned long long u64x2_t __attribute__((ext_vector_type(2)));
+__global__ __launch_bounds__(256) void probe_stream_add(uint64_t limbs, const uint64_t* __restrict__ a,
+                                                        const uint64_t* __restrict__ b, uint64_t* __restrict__ r) {
+    constexpr int R = 2;
+    const uint64_t base = (uint64_t)blockIdx.x * (256 / 64) * R * 128 + (threadIdx.x / 64) * R * 128 +
+                          2 * (threadIdx.x % 64);
+    u64x2_t x[R], y[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint64_t k = base + v * 128;
+        if (k + 1 < limbs) {
+            x[v] = __builtin_nontemporal_load(reinterpret_cast<const u64x2_t*>(a + k));
+            y[v] = __builtin_nontemporal_load(reinterpret_cast<const u64x2_t*>(b + k));
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint64_t k = base + v * 128;
+        if (k + 1 < limbs) __builtin_nontemporal_store(x[v] + y[v], reinterpret_cast<u64x2_t*>(r + k));
+    }
+}
+
 }  // namespace midint
 
 using namespace midint;
+
+extern "C" midint_status midint_probe_stream(uint64_t limbs, const uint64_t* d_a, const uint64_t* d_b,
+                                             uint64_t* d_r, void* stream) {
+    if (limbs == 0) return MIDINT_OK;
+    if (!d_a || !d_b || !d_r || (limbs & 1u) || ((((uintptr_t)d_a) | ((uintptr_t)d_b) | ((uintptr_t)d_r)) & 15u))
+        return MIDINT_EINVAL;
+    constexpr uint64_t kPerBlock = 4 * 2 * 128;
+    const uint64_t blocks = (limbs + kPerBlock - 1) / kPerBlock;
+    if (blocks > 0x7FFFFFFFull) return MIDINT_EUNSUPPORTED;
+    probe_stream_add<<<(uint32_t)blocks, 256, 0, (hipStream_t)stream>>>(limbs, d_a, d_b, d_r);
+    return hipGetLastError() == hipSuccess ? MIDINT_OK : MIDINT_EHIP;
+}
 
 extern "C" midint_status midint_probe_mac(int device, double* pp_per_s) {
     if (!pp_per_s) return MIDINT_EINVAL;

@@ -67,6 +67,8 @@ def lib():
             "orc_gmp_mul": (None, [_u64p, _u64p, _sz, _u64p, _sz]),
             "orc_gmp_add_batch": (None, [_sz, _sz, _u64p, _u64p, _u64p, ctypes.c_int]),
             "orc_gmp_mul_full_batch": (None, [_sz, _sz, _u64p, _u64p, _u64p, ctypes.c_int]),
+            "orc_gmp_ragged": (None, [_sz, _u32p, _u32p, _u32p, _u64p, _u64p, _u64p, _u64p, _u64p, _u64p,
+                                      ctypes.c_int]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -245,3 +247,29 @@ def from_int(v: int, m: int) -> np.ndarray:
         out[i] = v & 0xFFFFFFFFFFFFFFFF
         v >>= 64
     return out
+
+
+def gmp_ragged(ops, a, b, r_limbs, threads=1, out=None):
+    """A ragged batch [(kind, n_a, n_b, off_a, off_b, off_r)] op by op with
+    GMP (mpn_mul / mpn_add), results in the packed layout. `ops` may be the
+    prepared tuple of gmp_ragged_prepare and `out` a result buffer of
+    r_limbs limbs to reuse (both keep conversions out of timed loops)."""
+    if not isinstance(ops, tuple):
+        ops = gmp_ragged_prepare(ops)
+    kind, na, nb, oa, ob, orr = ops
+    a, b = _c(a), _c(b)
+    r = np.zeros(r_limbs, dtype=np.uint64) if out is None else out
+    lib().orc_gmp_ragged(kind.size, kind.ctypes.data_as(_u32p), na.ctypes.data_as(_u32p),
+                         nb.ctypes.data_as(_u32p), _p64(oa), _p64(ob), _p64(orr), _p64(a), _p64(b), _p64(r),
+                         threads)
+    return r
+
+
+def gmp_ragged_prepare(ops):
+    """Column arrays of a ragged op list for gmp_ragged."""
+    ops = np.asarray(ops, dtype=np.uint64).reshape(-1, 6)
+    kind = np.ascontiguousarray(ops[:, 0].astype(np.uint32))
+    na = np.ascontiguousarray(ops[:, 1].astype(np.uint32))
+    nb = np.ascontiguousarray(ops[:, 2].astype(np.uint32))
+    oa, ob, orr = (np.ascontiguousarray(ops[:, k]) for k in (3, 4, 5))
+    return kind, na, nb, oa, ob, orr

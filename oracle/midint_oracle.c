@@ -339,8 +339,11 @@ typedef unsigned long (*gmpn_add_n_fn)(unsigned long*, const unsigned long*,
                                        const unsigned long*, long);
 typedef unsigned long (*gmpn_mul_fn)(unsigned long*, const unsigned long*, long,
                                      const unsigned long*, long);
+typedef unsigned long (*gmpn_add_fn)(unsigned long*, const unsigned long*, long,
+                                     const unsigned long*, long);
 static gmpn_add_n_fn p_add_n;
 static gmpn_mul_fn p_mul;
+static gmpn_add_fn p_add;
 static int gmp_state = -1;
 
 int orc_gmp_available(void) {
@@ -349,7 +352,8 @@ int orc_gmp_available(void) {
     if (!h) return gmp_state = 0;
     p_add_n = (gmpn_add_n_fn)dlsym(h, "__gmpn_add_n");
     p_mul = (gmpn_mul_fn)dlsym(h, "__gmpn_mul");
-    gmp_state = (p_add_n && p_mul) ? 1 : 0;
+    p_add = (gmpn_add_fn)dlsym(h, "__gmpn_add");
+    gmp_state = (p_add_n && p_mul && p_add) ? 1 : 0;
     return gmp_state;
 }
 
@@ -385,4 +389,33 @@ void orc_gmp_mul_full_batch(size_t n, size_t m, const uint64_t* a, const uint64_
     for (size_t i = 0; i < n; i++)
         p_mul((unsigned long*)(r + 2 * i * m), (const unsigned long*)(a + i * m), (long)m,
               (const unsigned long*)(b + i * m), (long)m);
+}
+
+/* A ragged batch op by op with GMP (the CPU baseline of the ragged
+ * workload): op i is a product (kind 1: na + nb limbs out, mpn_mul with the
+ * longer operand first) or a sum (kind 0: max(na, nb) + 1 limbs out,
+ * mpn_add with the longer operand first and the carry as the top limb). */
+void orc_gmp_ragged(size_t n_ops, const uint32_t* kind, const uint32_t* na, const uint32_t* nb,
+                    const uint64_t* off_a, const uint64_t* off_b, const uint64_t* off_r, const uint64_t* a,
+                    const uint64_t* b, uint64_t* r, int threads) {
+    if (!orc_gmp_available()) abort();
+    if (threads < 1) threads = 1;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(threads)
+    for (size_t i = 0; i < n_ops; i++) {
+        const unsigned long* x = (const unsigned long*)(a + off_a[i]);
+        const unsigned long* y = (const unsigned long*)(b + off_b[i]);
+        unsigned long* z = (unsigned long*)(r + off_r[i]);
+        long nx = (long)na[i], ny = (long)nb[i];
+        if (nx < ny) {
+            const unsigned long* t = x;
+            x = y;
+            y = t;
+            nx = (long)nb[i];
+            ny = (long)na[i];
+        }
+        if (kind[i] == 1)
+            p_mul(z, x, nx, y, ny);
+        else
+            z[nx] = p_add(z, x, nx, y, ny);
+    }
 }

@@ -88,6 +88,10 @@ void orc_gmp_mul(uint64_t* r, const uint64_t* a, size_t na, const uint64_t* b, s
 /* Batched GMP calls with OpenMP over instances (cpu baseline timing). */
 void orc_gmp_add_batch(size_t n, size_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, int threads);
 void orc_gmp_mul_full_batch(size_t n, size_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, int threads);
+/* A ragged batch op by op with GMP (mpn_mul / mpn_add), OpenMP over ops. */
+void orc_gmp_ragged(size_t n_ops, const uint32_t* kind, const uint32_t* na, const uint32_t* nb,
+                    const uint64_t* off_a, const uint64_t* off_b, const uint64_t* off_r, const uint64_t* a,
+                    const uint64_t* b, uint64_t* r, int threads);
 /* Batched restatement with OpenMP over instances (cpu baseline timing). */
 void orc_mul_full_batch_mt(size_t n, size_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, int threads);
 

@@ -32,6 +32,26 @@ def test_stream_matches_oracle(mi, ctx, orc):
         assert np.array_equal(out["results"][j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
 
 
+def test_saturated_burst_batches_and_matches(mi, ctx, orc):
+    """Every request arrives at once (offered rate above what the server
+    sustains): micro-batches hold many requests, and each result equals the
+    oracle's and the result of the same request served alone."""
+    reqs = make_requests(mi.gen_limbs, 600, seed=21)
+    srv = mi.Server(ctx, hbm_budget=512 << 20, streams=4)
+    out = mi.serve_poisson(srv, reqs, qps=float("inf"), seed=7)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs)
+    assert st["ops_in_batches"] / st["batches_launched"] > 4
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out["results"][j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
+    for j in (0, 299, 599):
+        solo = mi.Server(ctx, streams=1)
+        alone = mi.serve_poisson(solo, [reqs[j]], qps=1000, seed=3)["results"][0]
+        solo.close()
+        assert np.array_equal(alone, out["results"][j])
+
+
 def test_alone_equals_mixed(mi, ctx, orc):
     reqs = make_requests(mi.gen_limbs, 120, seed=9)
     srv = mi.Server(ctx, streams=3)

@@ -231,3 +231,14 @@ def test_ref_rand_generator(orc):
     assert int(w[0]) & 0xFFFF == lo & 0xFFFF
     assert not (w & 1).any()
     assert not w[6:8].any() and not w[14:16].any()
+
+
+def test_gmp_ragged_matches_op_by_op(orc):
+    """The GMP ragged baseline (mpn_mul / mpn_add op by op) equals the
+    restatement's unbalanced products and sums, operands in either order."""
+    if not orc.gmp_available():
+        pytest.skip("libgmp.so.10 not installed")
+    from workloads import make_ragged, ragged_reference
+
+    ops, a, b, rl = make_ragged(orc.gen_limbs, n_ops=400, seed=11)
+    assert np.array_equal(orc.gmp_ragged(ops, a, b, rl, threads=4), ragged_reference(orc, ops, a, b, rl))
