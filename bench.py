@@ -449,7 +449,7 @@ def main():
     add = bench_add(torch, mi, ctx, args)
 
     composites = ragged = serving = None
-    rag = make_ragged(mi.gen_limbs)
+    rag = make_ragged(mi.gen_limbs) if (args.only == "all" or not args.no_cpu) else None
     if args.only == "all":
         composites = bench_composites(torch, mi, ctx, args, (da, db, n, m), (dan, dbn, nn, mn))
         ragged = bench_ragged(torch, mi, ctx, args, rag)

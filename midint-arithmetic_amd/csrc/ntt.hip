@@ -207,7 +207,7 @@ template <int LG, int V, int MODE>
 __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
                                         const uint64_t* __restrict__ twf) {
     using Pl = NttPlan<LG, V>;
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = opaque_u32(threadIdx.x);
     constexpr int R0 = Pl::radix(0);
     // products: the 4m <= L/2 coefficients sit in the lower half, which is
     // where k < 2^(R0-1) of the first pass's groups point
@@ -248,7 +248,7 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
 #pragma unroll
     for (int k = 0; k < V; k++) v[k] = gl_mul(v[k], ah[k]);
     __syncthreads();
-    NttInv<LG, V, Pl::NP - 1>::run(x, v, tid, twi);
+    NttInv<LG, V, Pl::NP - 1>::run(x, v, opaque_u32(tid), twi);
 
     if (MODE == 1) {
         uint64_t* gr = r + (size_t)blockIdx.x * L;

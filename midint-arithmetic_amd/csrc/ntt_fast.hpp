@@ -25,6 +25,13 @@ namespace midint {
 
 __device__ __forceinline__ uint32_t ntt_pad(uint32_t i) { return i + (i >> 5); }
 
+// x, with the compiler unable to see its value: per-phase index arithmetic
+// derived from it is recomputed instead of being kept live across phases.
+__device__ __forceinline__ uint32_t opaque_u32(uint32_t x) {
+    asm volatile("" : "+v"(x));
+    return x;
+}
+
 template <int LG, int V>
 struct NttPlan {
     static constexpr int RM = V == 32 ? 5 : 4;  // stages per full pass
