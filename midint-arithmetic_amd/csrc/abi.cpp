@@ -223,9 +223,9 @@ static midint_status get_tables(midint_ctx* ctx, uint32_t lg, NttTables* out) {
         *out = it->second;
         return MIDINT_OK;
     }
-    const size_t half = (size_t)1 << (lg - 1);
-    uint64_t* hf = (uint64_t*)malloc(half * sizeof(uint64_t));
-    uint64_t* hi = (uint64_t*)malloc(half * sizeof(uint64_t));
+    const size_t len = (size_t)1 << lg;  // w^i and w^-i for i < L
+    uint64_t* hf = (uint64_t*)malloc(len * sizeof(uint64_t));
+    uint64_t* hi = (uint64_t*)malloc(len * sizeof(uint64_t));
     if (!hf || !hi) {
         free(hf);
         free(hi);
@@ -234,10 +234,10 @@ static midint_status get_tables(midint_ctx* ctx, uint32_t lg, NttTables* out) {
     NttTables t;
     t.lg = lg;
     ntt_host_tables(lg, hf, hi, &t.inv_len);
-    hipError_t e = hipMalloc((void**)&t.d_fwd, half * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMalloc((void**)&t.d_inv, half * sizeof(uint64_t));
-    if (e == hipSuccess) e = hipMemcpy(t.d_fwd, hf, half * sizeof(uint64_t), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(t.d_inv, hi, half * sizeof(uint64_t), hipMemcpyHostToDevice);
+    hipError_t e = hipMalloc((void**)&t.d_fwd, len * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMalloc((void**)&t.d_inv, len * sizeof(uint64_t));
+    if (e == hipSuccess) e = hipMemcpy(t.d_fwd, hf, len * sizeof(uint64_t), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(t.d_inv, hi, len * sizeof(uint64_t), hipMemcpyHostToDevice);
     free(hf);
     free(hi);
     if (e != hipSuccess) return hip_status(e);

@@ -184,22 +184,22 @@ __device__ __forceinline__ uint32_t lo32(uint64_t x) { return (uint32_t)x; }
 __device__ __forceinline__ uint32_t hi32(uint64_t x) { return (uint32_t)(x >> 32); }
 __device__ __forceinline__ uint64_t join32(uint32_t lo, uint32_t hi) { return ((uint64_t)hi << 32) | lo; }
 
-// a + b mod p: s = a + b (carry c1), t = s + eps (carry c2); t when c1 | c2
-// (c1 and c2 are never both set).
+// a + b mod p for canonical a, b: b' = b + eps cannot overflow (b < p), and
+// t = a + b' = a + b - p + 2^64 carries exactly when a + b >= p; then t is
+// the result, otherwise t - eps (= a + b, no borrow since t >= eps). 7 VALU.
 __device__ __forceinline__ uint64_t gl_add(uint64_t a, uint64_t b) {
-    uint32_t s0, s1, t0, t1;
-    uint64_t k1, k2;
-    asm("v_add_co_u32_e64 %[s0], %[k1], %[a0], %[b0]\n\t"
-        "v_addc_co_u32_e64 %[s1], %[k1], %[a1], %[b1], %[k1]\n\t"
-        "v_add_co_u32_e64 %[t0], %[k2], %[s0], -1\n\t"
-        "v_addc_co_u32_e64 %[t1], %[k2], %[s1], 0, %[k2]\n\t"
-        "v_cndmask_b32_e64 %[s0], %[s0], %[t0], %[k2]\n\t"
-        "v_cndmask_b32_e64 %[s1], %[s1], %[t1], %[k2]\n\t"
-        "v_cndmask_b32_e64 %[s0], %[s0], %[t0], %[k1]\n\t"
-        "v_cndmask_b32_e64 %[s1], %[s1], %[t1], %[k1]"
-        : [s0] "=&v"(s0), [s1] "=&v"(s1), [t0] "=&v"(t0), [t1] "=&v"(t1), [k1] "=&s"(k1), [k2] "=&s"(k2)
+    uint32_t t0, t1, u0, u1, e;
+    uint64_t k;
+    asm("v_add_co_u32_e64 %[u0], %[k], %[b0], -1\n\t"
+        "v_addc_co_u32_e64 %[u1], %[k], %[b1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[t0], %[k], %[a0], %[u0]\n\t"
+        "v_addc_co_u32_e64 %[t1], %[k], %[a1], %[u1], %[k]\n\t"
+        "v_cndmask_b32_e64 %[e], -1, 0, %[k]\n\t"
+        "v_sub_co_u32_e64 %[t0], %[k], %[t0], %[e]\n\t"
+        "v_subb_co_u32_e64 %[t1], %[k], %[t1], 0, %[k]"
+        : [t0] "=&v"(t0), [t1] "=&v"(t1), [u0] "=&v"(u0), [u1] "=&v"(u1), [e] "=&v"(e), [k] "=&s"(k)
         : [a0] "v"(lo32(a)), [a1] "v"(hi32(a)), [b0] "v"(lo32(b)), [b1] "v"(hi32(b)));
-    return join32(s0, s1);
+    return join32(t0, t1);
 }
 
 // a - b mod p: d = a - b, and on borrow d - eps (= d + p mod 2^64), which

@@ -73,13 +73,13 @@ hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t
 // NTT multiplication.
 struct NttTables {
     uint32_t lg;          // log2 of the transform length
-    uint64_t* d_fwd;      // w^i, i < L/2
-    uint64_t* d_inv;      // w^-i, i < L/2
+    uint64_t* d_fwd;      // w^i, i < L
+    uint64_t* d_inv;      // w^-i, i < L
     uint64_t inv_len;     // L^-1 mod p
 };
 constexpr uint32_t kNttMaxLg = 14;  // 16384-point transform: 128 KiB of LDS
 uint32_t ntt_lg_for_limbs(uint32_t m);
-// Host computation of twiddle tables (allocated by the caller).
+// Host computation of twiddle tables w^i, w^-i for i < L (allocated by the caller).
 void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_len);
 // lo: write only the low m limbs of each product (stride m), else all 2m.
 hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,

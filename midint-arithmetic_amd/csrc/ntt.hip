@@ -203,9 +203,11 @@ __device__ __forceinline__ uint64_t ntt_coef(const uint64_t* __restrict__ g, uin
 
 // Forward transform of one operand read straight from global memory (in the
 // first pass's grouping); returns with the last pass's values in v.
-template <int LG, int V, int MODE>
+// SCALE: multiply the input by `scale` first (the inverse's L^-1, applied to
+// a's nonzero inputs instead of to all L transformed values).
+template <int LG, int V, int MODE, bool SCALE = false>
 __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
-                                        const uint64_t* __restrict__ twf) {
+                                        const uint64_t* __restrict__ twf, uint64_t scale = 0) {
     using Pl = NttPlan<LG, V>;
     const uint32_t tid = opaque_u32(threadIdx.x);
     constexpr int R0 = Pl::radix(0);
@@ -215,10 +217,14 @@ __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t
 #pragma unroll
     for (int j = 0; j < (V >> R0); j++)
 #pragma unroll
-        for (int k = 0; k < (1 << R0); k++)
-            v[(j << R0) + k] = (ZHI && k >= (1 << (R0 - 1)))
-                                   ? 0ull
-                                   : ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
+        for (int k = 0; k < (1 << R0); k++) {
+            if (ZHI && k >= (1 << (R0 - 1))) {
+                v[(j << R0) + k] = 0ull;
+            } else {
+                const uint64_t c = ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
+                v[(j << R0) + k] = SCALE ? gl_mul(c, scale) : c;
+            }
+        }
     NttFwd<LG, V, 0, ZHI>::run(x, v, tid, twf);
 }
 
@@ -240,9 +246,7 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
     const uint64_t* gb = b + (size_t)blockIdx.x * (MODE == 0 ? m : L);
 
     uint64_t ah[V], v[V];
-    ntt_fwd<LG, V, MODE>(ga, m, x, ah, twf);
-#pragma unroll
-    for (int k = 0; k < V; k++) ah[k] = gl_mul(ah[k], inv_len);
+    ntt_fwd<LG, V, MODE, true>(ga, m, x, ah, twf, inv_len);  // transform of a / L
     __syncthreads();  // every thread is done reading LDS for a's last pass
     ntt_fwd<LG, V, MODE>(gb, m, x, v, twf);
 #pragma unroll
@@ -328,10 +332,11 @@ static hipError_t launch_fast(uint32_t n, uint32_t m, uint32_t rl, const uint64_
 }
 
 // Values per thread: tunable per transform length ("ntt.vals@L", 16 or 32;
-// 0 for the default): 32 at 2^14 points, where 16 values would need 1024
-// threads and leave too few registers for both transforms.
+// 0 for the default). 16 by default: T = L / 16 threads (up to 1024 at 2^14
+// points, 4 waves per SIMD) hide latency better than 32 values per thread,
+// and both transforms fit in registers either way.
 static uint32_t ntt_vals(uint32_t lg) {
-    const uint32_t def = lg >= 14 ? 32 : 16;
+    const uint32_t def = 16;
     const int64_t v = tune_get_class("ntt.vals", 1u << lg, def);
     return (v == 16 || (v == 32 && lg >= 11)) ? (uint32_t)v : def;
 }
@@ -376,7 +381,7 @@ void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_le
     const uint64_t w = h_pow(h_pow(7, (kGlP - 1) / L), 5);
     const uint64_t wi = h_pow(w, kGlP - 2);
     uint64_t f = 1, g = 1;
-    for (uint64_t i = 0; i < L / 2; i++) {
+    for (uint64_t i = 0; i < L; i++) {
         fwd[i] = f;
         inv[i] = g;
         f = h_mulmod(f, w);

@@ -40,10 +40,115 @@ struct NttPlan {
     static constexpr uint32_t T = (1u << LG) / V;
 };
 
+// Radix-4 units of the passes with table twiddles. Two consecutive stages s
+// (local span J) and s + 1 (span J / 2) mix the four values at local offsets
+// k, k + J/2, k + J, k + 3J/2 of a group. With W = w^E, E the stage-s
+// exponent of offset k, the stage-s twiddle of offset k + J/2 is W w^(L/4),
+// and w^(L/4) = 2^48 (the root has w^(L/64) = 8). So the forward pair of
+// decimation-in-frequency stages is
+//   b0 = x0 + x2, b2 = x0 - x2, b1 = x1 + x3, b3 = (x1 - x3) 2^48,
+//   y0 = b0 + b1, y1 = (b0 - b1) W^2, y2 = (b2 + b3) W, y3 = (b2 - b3) W^3
+// (slots k, k + J/2, k + J, k + 3J/2): three table twiddles and one shift per
+// four values instead of four table twiddles. The inverse pair (twiddles
+// w^-E, 2^-48 = -2^48) undoes it up to the factor 4 that L^-1 absorbs:
+//   Y1 = y1 W^-2, Y2 = y2 W^-1, Y3 = y3 W^-3, c0 = y0 + Y1, c1 = y0 - Y1,
+//   c2 = Y2 + Y3, c3 = (Y3 - Y2) 2^48,
+//   x0 = c0 + c2, x2 = c0 - c2, x1 = c1 + c3, x3 = c1 - c3.
+// The twiddle tables hold w^i and w^-i for i < L (3E can exceed L / 2).
+template <bool INV, int LG, bool ZERO_HI>
+__device__ __forceinline__ void ntt_radix4(uint64_t& s0, uint64_t& s1, uint64_t& s2, uint64_t& s3, uint32_t e,
+                                           const uint64_t* __restrict__ tw) {
+    constexpr uint32_t kMask = (1u << LG) - 1;
+    const uint64_t w1 = tw[e], w2 = tw[2 * e], w3 = tw[(3 * e) & kMask];
+    if (!INV) {
+        const uint64_t x0 = s0, x1 = s1;
+        uint64_t b0, b1, b2, b3;
+        if (ZERO_HI) {  // x2 = x3 = 0
+            b0 = b2 = x0;
+            b1 = x1;
+            b3 = gl_mul_pow2(x1, 48);
+        } else {
+            const uint64_t x2 = s2, x3 = s3;
+            b0 = gl_add(x0, x2);
+            b2 = gl_sub(x0, x2);
+            b1 = gl_add(x1, x3);
+            b3 = gl_mul_pow2(gl_sub(x1, x3), 48);
+        }
+        s0 = gl_add(b0, b1);
+        s1 = gl_mul(gl_sub(b0, b1), w2);
+        s2 = gl_mul(gl_add(b2, b3), w1);
+        s3 = gl_mul(gl_sub(b2, b3), w3);
+    } else {
+        const uint64_t y0 = s0, Y1 = gl_mul(s1, w2), Y2 = gl_mul(s2, w1), Y3 = gl_mul(s3, w3);
+        const uint64_t c0 = gl_add(y0, Y1), c1 = gl_sub(y0, Y1);
+        const uint64_t c2 = gl_add(Y2, Y3), c3 = gl_mul_pow2(gl_sub(Y3, Y2), 48);
+        s0 = gl_add(c0, c2);
+        s2 = gl_sub(c0, c2);
+        s1 = gl_add(c1, c3);
+        s3 = gl_sub(c1, c3);
+    }
+}
+
+// The stages of one group (values v[base .. base + 2^R)) of a pass with table
+// twiddles, as radix-4 units: forward, units at spans (2^jj, 2^(jj-1)) for
+// jj = R-1, R-3, ..., then the span-1 stage alone if R is odd; the inverse
+// runs the mirror order. U counts stage positions; the recursion keeps every
+// index a compile-time constant.
+template <int R, bool INV, int V, int LG, int S0, bool ZHI, int U>
+struct NttUnits {
+    static __device__ __forceinline__ void run(uint64_t (&v)[V], int base, uint32_t g_lo,
+                                               const uint64_t* __restrict__ tw) {
+        constexpr int K = 1 << R;
+        constexpr int lgmin = (LG - 1 - S0) - (R - 1);
+        constexpr bool odd = (R & 1) != 0;
+        // forward position U covers jj = R-1-U; inverse position U covers jj = U
+        constexpr bool single = odd && (INV ? U == 0 : U == R - 1);
+        constexpr bool pair = INV ? ((odd ? (U & 1) == 1 : (U & 1) == 0) && U + 1 < R) : ((U & 1) == 0 && U + 1 < R);
+        if constexpr (single) {
+            constexpr int s = S0 + (R - 1);  // span h_min
+            const uint64_t wt = tw[g_lo << s];
+#pragma unroll
+            for (int k = 0; k < K; k += 2) {
+                uint64_t& a = v[base + k];
+                uint64_t& b = v[base + k + 1];
+                if constexpr (!INV) {
+                    const uint64_t x = a, y = b;
+                    a = gl_add(x, y);
+                    b = gl_mul(gl_sub(x, y), wt);
+                } else {
+                    const uint64_t x = a, y = gl_mul(b, wt);
+                    a = gl_add(x, y);
+                    b = gl_sub(x, y);
+                }
+            }
+        } else if constexpr (pair) {
+            constexpr int big = INV ? U + 1 : R - 1 - U;  // spans 2^big and 2^(big-1)
+            constexpr int s = S0 + (R - 1 - big);
+            constexpr int J = 1 << big, J2 = J >> 1;
+            constexpr bool zhi = !INV && ZHI && S0 == 0 && s == 0;
+#pragma unroll
+            for (int k = 0; k < K; k++) {
+                if (k & (J | J2)) continue;
+                const uint32_t klo = (uint32_t)(k & (J2 - 1));
+                const uint32_t e = (g_lo + (klo << lgmin)) << s;
+                ntt_radix4<INV, LG, zhi>(v[base + k], v[base + k + J2], v[base + k + J], v[base + k + J + J2], e, tw);
+            }
+        }
+        NttUnits<R, INV, V, LG, S0, ZHI, U + 1>::run(v, base, g_lo, tw);
+    }
+};
+template <int R, bool INV, int V, int LG, int S0, bool ZHI>
+struct NttUnits<R, INV, V, LG, S0, ZHI, R> {
+    static __device__ __forceinline__ void run(uint64_t (&)[V], int, uint32_t, const uint64_t* __restrict__) {}
+};
+
 // One pass of R stages starting at stage S0 on the thread's registers v[V]
 // (V / 2^R groups of 2^R values, group j at v[j << R ...]). In the last
 // pass h_min = 1, so g_lo = 0 and every twiddle index is a compile-time
 // constant; the butterflies whose index is 0 (twiddle 1) skip the multiply.
+// The other passes run their stages as radix-4 units (ntt_radix4): pairs
+// from the largest span down in the forward direction, a leftover single
+// stage of span 1 last; the inverse mirrors that order.
 // ZHI (forward, first pass): the upper half of the input is zero, so the
 // first stage is a = u, b = u w.
 template <int R, bool INV, int V, int LG, int S0, bool ZHI = false>
@@ -53,6 +158,15 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
     constexpr uint32_t T = (1u << LG) / V;
     constexpr int lgH = LG - 1 - S0;      // H = 2^lgH
     constexpr int lgmin = lgH - (R - 1);  // h_min
+    if constexpr (lgmin > 0) {
+#pragma unroll
+        for (int j = 0; j < G; j++) {
+            const uint32_t g = tid + T * j;
+            const uint32_t g_lo = g & ((1u << lgmin) - 1);
+            NttUnits<R, INV, V, LG, S0, ZHI, 0>::run(v, j * K, g_lo, tw);
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < G; j++) {
         const uint32_t g = tid + T * j;
