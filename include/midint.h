@@ -89,11 +89,13 @@ midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
                               const uint64_t* d_b, uint64_t* d_r, void* stream);
 
 /* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance.
- * Replaces polyKerQ (cuda/ker-classic-mul.cu.h:500-575). Uses 4*n*m limbs
- * of context scratch (allocated on first need, grown when too small). Calls
- * that use context scratch (poly, poly_ntt, poly_strided and the repacking
- * u32 path) on different streams of one context are ordered on the device:
- * each waits for the previous scratch user to finish. */
+ * Replaces polyKerQ (cuda/ker-classic-mul.cu.h:500-575): one kernel per
+ * call, the four products on the int8 matrix cores and the intermediates in
+ * LDS, for every m up to midint_max_limbs_mul(). (With the tuning key
+ * "poly.fused" = 0 it runs as a chain of launches through 4*n*m limbs of
+ * context scratch instead.) Calls that use context scratch (that chain,
+ * poly_ntt and the repacking u32 path) on different streams of one context
+ * are ordered on the device: each waits for the previous scratch user. */
 midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                           uint64_t* d_r, void* stream);
 
@@ -140,7 +142,7 @@ midint_status midint_mul_lo_strided(midint_ctx* ctx, uint32_t n, uint32_t m, con
 midint_status midint_mul_full_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
                                       const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr,
                                       void* stream);
-/* Uses 4*n*m limbs of context scratch for its temporaries. */
+/* poly_strided runs the fused kernel of midint_poly at these strides. */
 midint_status midint_poly_strided(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, uint64_t sa,
                                   const uint64_t* d_b, uint64_t sb, uint64_t* d_r, uint64_t sr, void* stream);
 

@@ -211,6 +211,7 @@ midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_
     if (m > midint_max_limbs_mul()) return MIDINT_EUNSUPPORTED;
     if ((st = bind_device(ctx)) != MIDINT_OK) return st;
     hipStream_t s = (hipStream_t)stream;
+    if (poly_uses_fused(m)) return hip_status(launch_poly(n, m, d_a, d_b, d_r, nullptr, s));
     return with_scratch(ctx, 4ull * n * m, s,
                         [&](uint64_t* scr) { return launch_poly(n, m, d_a, d_b, d_r, scr, s); });
 }
@@ -323,6 +324,7 @@ static midint_status u64_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m, cons
         case Op::MulLo: return hip_status(launch_mul(n, m, a, b, r, false, s, str));
         case Op::MulFull: return hip_status(launch_mul(n, m, a, b, r, true, s, str));
         case Op::Poly:
+            if (poly_uses_fused(m)) return hip_status(launch_poly(n, m, a, b, r, nullptr, s, str));
             return with_scratch(ctx, 4ull * n * m, s,
                                 [&](uint64_t* scr) { return launch_poly(n, m, a, b, r, scr, s, str); });
     }
@@ -382,7 +384,8 @@ static midint_status u32_op(midint_ctx* ctx, Op op, uint32_t n, uint32_t m32, co
     const uint32_t out_limbs = op_out_limbs(op, mp);
     const size_t nm = (size_t)n * mp;
     hipStream_t s = (hipStream_t)stream;
-    return with_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (op == Op::Poly ? 4 * nm : 0), s,
+    const bool poly_chain = op == Op::Poly && !poly_uses_fused(mp);
+    return with_scratch(ctx, 2 * nm + (size_t)n * out_limbs + (poly_chain ? 4 * nm : 0), s,
                         [&](uint64_t* scr) {
                             uint64_t* pa = scr;
                             uint64_t* pb = pa + nm;

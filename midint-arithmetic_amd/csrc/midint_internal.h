@@ -66,9 +66,15 @@ struct MfRgConfig {
     uint32_t pow8;
 };
 bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out);
-// r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ).
+// r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ): the launch
+// chain over 4*n*m limbs of scratch, and the fused one-kernel version
+// (mul_mfma.hip; no scratch) where its LDS layout fits.
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                        uint64_t* scratch, hipStream_t s, Strides st = {});
+bool poly_fused_supported(uint32_t m);
+bool poly_uses_fused(uint32_t m);
+hipError_t launch_poly_fused(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                             hipStream_t s, Strides st = {});
 
 // NTT multiplication.
 struct NttTables {

@@ -463,10 +463,15 @@ hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t*
                 : launch_mul_full_or_lo<false>(n, m, a, b, r, st, s);
 }
 
-// Reference polyKerQ (cuda/ker-classic-mul.cu.h:533-559) as a chain of
-// launches over caller-provided scratch of 4*n*m limbs.
+// Whether launch_poly runs the fused kernel (and needs no scratch).
+bool poly_uses_fused(uint32_t m) { return tune_get_class("poly.fused", m, 1) != 0 && poly_fused_supported(m); }
+
+// Reference polyKerQ (cuda/ker-classic-mul.cu.h:533-559): the fused kernel
+// where it fits (scratch unused), else a chain of launches over
+// caller-provided scratch of 4*n*m limbs.
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                        uint64_t* scratch, hipStream_t s, Strides st) {
+    if (poly_uses_fused(m)) return launch_poly_fused(n, m, a, b, r, s, st);
     const size_t nm = (size_t)n * m;
     uint64_t* t1 = scratch;
     uint64_t* t2 = scratch + nm;

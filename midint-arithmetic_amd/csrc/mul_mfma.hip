@@ -550,6 +550,86 @@ __global__ __launch_bounds__(64 * kMfRgWaves) void mul_mfma_rg_kernel(const MfRg
     mf_block<R>(g, 1, it.a, 0, it.b, 0, it.r, 0, pow8, lds64, lds_gp);
 }
 
+// z <- x + y per instance (mod B^m) over the ipb instances of a workgroup:
+// x, y, z at instance strides sx, sy, sz (LDS or global; only instances
+// below ninst are read or written), R rows of 128 limbs per wave covering
+// ipb m limbs. Every wave of the workgroup must call it.
+template <int R>
+__device__ __forceinline__ void mf_seg_add(const uint64_t* x, uint64_t sx, const uint64_t* y, uint64_t sy,
+                                           uint64_t* z, uint64_t sz, uint32_t m, uint32_t mg_m, uint32_t ipb,
+                                           uint32_t ninst, uint32_t* lds_gp) {
+    const uint32_t Lw = ipb * m;
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * (threadIdx.x & 63);
+    SegPos sp = mf_segpos(k_first, m, mg_m);
+    const uint32_t sq = mf_div(kRowLimbs, mg_m), srr = kRowLimbs - sq * m;
+    Rows<R> rows;
+    uint32_t off0[R], inst0[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        const bool wrap = sp.off + 1 == m;
+        const uint32_t i1 = sp.inst + (wrap ? 1u : 0u), o1 = wrap ? 0u : sp.off + 1;
+        const bool v0 = k < Lw && sp.inst < ninst, v1 = k + 1 < Lw && i1 < ninst;
+        rows.x0[v] = v0 ? x[sp.inst * sx + sp.off] : 0ull;
+        rows.y0[v] = v0 ? y[sp.inst * sy + sp.off] : 0ull;
+        rows.x1[v] = v1 ? x[i1 * sx + o1] : 0ull;
+        rows.y1[v] = v1 ? y[i1 * sy + o1] : 0ull;
+        rows.s0[v] = sp.off == 0 && k < Lw;
+        rows.s1[v] = wrap && k + 1 < Lw;
+        off0[v] = sp.off;
+        inst0[v] = sp.inst;
+        segpos_next(&sp, m, sq, srr);
+    }
+    block_add_rows<R>(rows, lds_gp);
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint32_t k = k_first + v * kRowLimbs;
+        const bool wrap = off0[v] + 1 == m;
+        const uint32_t i1 = inst0[v] + (wrap ? 1u : 0u), o1 = wrap ? 0u : off0[v] + 1;
+        if (k < Lw && inst0[v] < ninst) z[inst0[v] * sz + off0[v]] = rows.x0[v];
+        if (k + 1 < Lw && i1 < ninst) z[i1 * sz + o1] = rows.x1[v];
+    }
+}
+
+// The reference's polyKerQ (cuda/ker-classic-mul.cu.h:500-575) in one
+// kernel: r = ((a^2 + b)(b^2 + b) + ab) mod B^m, every step mod B^m in the
+// reference's order. The three products of the operands come from global
+// memory into LDS buffers (T1 = a^2, T2 = b^2, T3 = ab), the additions run
+// in LDS (T1 += b, T2 += b), the fourth product reads T1, T2 from LDS into
+// T4, and r = T4 + T3 is the only global store. The matrix-core phases are
+// mf_block's, with low-half products (PL = m).
+template <int R>
+__global__ __launch_bounds__(1024) void poly_mfma_kernel(uint32_t n_inst, uint32_t m, uint32_t P, uint32_t dmax,
+                                                         uint32_t N, uint32_t NT, uint32_t wpi, uint32_t ipb,
+                                                         uint32_t PLp, uint32_t mg_m, uint32_t t_off, uint32_t pow8,
+                                                         const uint64_t* __restrict__ a,
+                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
+                                                         uint64_t sa, uint64_t sb, uint64_t sr) {
+    extern __shared__ __align__(16) uint64_t lds64[];
+    __shared__ uint32_t lds_gp[16];
+    const uint64_t inst_base = (uint64_t)blockIdx.x * ipb;
+    const uint32_t ninst = (uint32_t)min((uint64_t)ipb, (uint64_t)n_inst - inst_base);
+    const MfGeom g = {m, m, P, P, dmax, N, NT, wpi, ipb, m, PLp, mg_m};
+    const uint64_t* ga = a + inst_base * sa;
+    const uint64_t* gb = b + inst_base * sb;
+    uint64_t* T1 = lds64 + t_off;  // t_off: u64 words of mf_block's region
+    uint64_t* T2 = T1 + (size_t)ipb * m;
+    uint64_t* T3 = T2 + (size_t)ipb * m;
+    uint64_t* T4 = T3 + (size_t)ipb * m;
+    mf_block<R>(g, ninst, ga, sa, ga, sa, T1, m, pow8, lds64, lds_gp);  // T1 = a^2
+    __syncthreads();
+    mf_block<R>(g, ninst, gb, sb, gb, sb, T2, m, pow8, lds64, lds_gp);  // T2 = b^2
+    __syncthreads();
+    mf_block<R>(g, ninst, ga, sa, gb, sb, T3, m, pow8, lds64, lds_gp);  // T3 = a b
+    __syncthreads();
+    mf_seg_add<R>(T1, m, gb, sb, T1, m, m, mg_m, ipb, ninst, lds_gp);   // T1 = a^2 + b
+    mf_seg_add<R>(T2, m, gb, sb, T2, m, m, mg_m, ipb, ninst, lds_gp);   // T2 = b^2 + b
+    __syncthreads();
+    mf_block<R>(g, ninst, T1, m, T2, m, T4, m, pow8, lds64, lds_gp);    // T4 = T1 T2
+    __syncthreads();
+    mf_seg_add<R>(T4, m, T3, m, r + inst_base * sr, sr, m, mg_m, ipb, ninst, lds_gp);  // r = T4 + T3
+}
+
 namespace {
 
 struct MfmaLayout {
@@ -599,7 +679,65 @@ hipError_t launch_mfma_t(const MfmaLayout& L, uint32_t n, uint32_t m, const uint
     return hipGetLastError();
 }
 
+// The fused poly kernel: mfma_layout's low-half layout with four m-limb LDS
+// buffers per instance after mf_block's region, fewer instances per
+// workgroup if they do not fit.
+bool poly_layout(uint32_t m, MfmaLayout* L, uint32_t* t_off) {
+    if (!mfma_layout(m, false, L)) return false;
+    for (;;) {
+        const size_t region = (L->lds + 15) & ~(size_t)15;
+        const size_t total = region + (size_t)4 * L->ipb * m * 8;
+        if (total <= 160 * 1024 - 256) {
+            *t_off = (uint32_t)(region / 8);
+            L->lds = total;
+            return true;
+        }
+        if (L->ipb == 1) return false;
+        L->ipb /= 2;
+        L->waves = L->ipb * L->wpi;
+        uint32_t R = 1;
+        while ((uint64_t)L->waves * R * kRowLimbs < (uint64_t)L->ipb * L->PL) R *= 2;
+        L->R = R;
+        L->lds = (size_t)16 * L->ipb * L->PLp + mf_a_bytes(L->P, L->ipb) + (size_t)L->ipb * mf_br_bytes(L->P);
+    }
+}
+
+template <int R>
+hipError_t launch_poly_t(const MfmaLayout& L, uint32_t t_off, uint32_t n, uint32_t m, const uint64_t* a,
+                         const uint64_t* b, uint64_t* r, const Strides& st, hipStream_t s) {
+    const void* fn = (const void*)poly_mfma_kernel<R>;
+    if (L.lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)L.lds);
+        if (e != hipSuccess) return e;
+    }
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + L.ipb - 1) / L.ipb);
+    poly_mfma_kernel<R><<<blocks, 64 * L.waves, L.lds, s>>>(n, m, L.P, L.dmax, L.N, L.NT, L.wpi, L.ipb, L.PLp,
+                                                            mf_magic(m), t_off, 256u, a, b, r, st.a ? st.a : m,
+                                                            st.b ? st.b : m, st.r ? st.r : m);
+    return hipGetLastError();
+}
+
 }  // namespace
+
+bool poly_fused_supported(uint32_t m) {
+    MfmaLayout L;
+    uint32_t t_off;
+    return poly_layout(m, &L, &t_off);
+}
+
+hipError_t launch_poly_fused(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                             hipStream_t s, Strides st) {
+    MfmaLayout L;
+    uint32_t t_off;
+    if (!poly_layout(m, &L, &t_off)) return hipErrorInvalidValue;
+    if (n == 0) return hipSuccess;
+    switch (L.R) {
+        case 1: return launch_poly_t<1>(L, t_off, n, m, a, b, r, st, s);
+        case 2: return launch_poly_t<2>(L, t_off, n, m, a, b, r, st, s);
+        case 4: return launch_poly_t<4>(L, t_off, n, m, a, b, r, st, s);
+        default: return launch_poly_t<8>(L, t_off, n, m, a, b, r, st, s);
+    }
+}
 
 bool mul_mfma_supported(uint32_t m, bool full) {
     MfmaLayout L;

@@ -12,6 +12,8 @@
 //   add.rows      rows of 128 limbs per wave for small-m additions (1..8)
 //   add.threads   threads per block for small-m additions (64..1024)
 //   add.nt        1: non-temporal (streaming) 16-byte loads and stores
+//   poly.fused    1: the poly composite as one matrix-core kernel (0: a chain of
+//                 launches through context scratch)
 //   ntt.fast      1: register-pass NTT kernel for transforms of 2^10..2^14 points
 //   ntt.vals      values per thread of that kernel, per transform length
 //                 ("ntt.vals@L"): 16 or 32, 0 for the built-in choice
@@ -37,7 +39,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
     };
     return t;
 }

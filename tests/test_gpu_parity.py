@@ -350,15 +350,40 @@ def _check_mul(ctx, torch_dev, orc, m):
     assert np.array_equal(host(rl), orc.mul_lo(a, b, m))
 
 
-@pytest.mark.parametrize("m", [1, 4, 16, 33, 128])
-def test_poly(ctx, torch_dev, orc, m):
+POLY_SIZES = [1, 2, 4, 16, 33, 64, 127, 128, 129, 300, 512, 1024, 2048]
+
+
+@pytest.mark.parametrize("fused", [1, 0], ids=["fused", "chain"])
+@pytest.mark.parametrize("m", POLY_SIZES)
+def test_poly(mi, ctx, torch_dev, orc, m, fused):
+    """polyKerQ: the one-kernel matrix-core version and the launch chain,
+    batch counts that leave the last workgroup partial."""
     torch = torch_dev
-    n = 13
-    a, b = operands(m * 3 + 1, n, m)
+    for n in ((13, 1) if m < 1024 else (5, 1)):
+        a, b = operands(m * 3 + 1 + n, n, m)
+        dr = empty(torch, n * m)
+        with tuned(mi, m, {"poly.fused": fused}):
+            ctx.poly(dev(torch, a), dev(torch, b), dr, m)
+            torch.cuda.synchronize()
+        got, want = host(dr)[: n * m], orc.poly(a, b, m)
+        assert np.array_equal(got, want), (m, n, fused, int(np.flatnonzero(got != want)[0]))
+
+
+@pytest.mark.parametrize("m,wpi,ipb", [(64, 1, 1), (64, 1, 8), (128, 1, 2), (128, 2, 4), (256, 1, 4), (256, 2, 2),
+                                       (512, 1, 2), (512, 4, 1), (1024, 1, 1), (1024, 2, 2), (2048, 2, 1)])
+def test_poly_fused_layouts(mi, ctx, torch_dev, orc, m, wpi, ipb):
+    """The fused poly kernel under the matrix-core layout keys (waves per
+    instance, instances per workgroup; the kernel halves the instances per
+    workgroup when its LDS buffers do not fit), including the 8-row
+    normalisation."""
+    torch = torch_dev
+    n = 7
+    a, b = operands(m * 5 + wpi + 3 * ipb, n, m)
     dr = empty(torch, n * m)
-    ctx.poly(dev(torch, a), dev(torch, b), dr, m)
-    torch.cuda.synchronize()
-    assert np.array_equal(host(dr), orc.poly(a, b, m))
+    with tuned(mi, m, {"poly.fused": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb}):
+        ctx.poly(dev(torch, a), dev(torch, b), dr, m)
+        torch.cuda.synchronize()
+    assert np.array_equal(host(dr), orc.poly(a, b, m)), (m, wpi, ipb)
 
 
 @pytest.mark.parametrize("m", [1, 3, 64, 100, 1024, 2048])
