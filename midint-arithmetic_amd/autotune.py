@@ -8,7 +8,10 @@ can never trade correctness for speed. The result is written as
 ``key@class value`` lines (see midint_tune_set) to tune/mi355x.txt, which
 Context() applies when it is created (and C callers with midint_tune_load).
 
-    python midint-arithmetic_amd/autotune.py [--out PATH] [--quick]
+    python midint-arithmetic_amd/autotune.py [--out PATH] [--quick] [--only add,mul,ntt,ragged]
+
+--only re-tunes the listed parts and keeps the other parts' lines of an
+existing tuning file.
 """
 from __future__ import annotations
 
@@ -53,6 +56,14 @@ def _timed(torch, fn, steps, warm=3):
 def _set(mi, params, cls):
     for k, v in params.items():
         mi.tune_set(f"{k}@{cls}", v)
+
+
+def _get(mi, key):
+    """Value of a tuning key; an unset "key@C" reads as the global key."""
+    try:
+        return mi.tune_get(key)
+    except Exception:
+        return mi.tune_get(key.split("@")[0])
 
 
 def grid_of(grid):
@@ -114,51 +125,94 @@ def tune_op(mi, torch, ctx, op, cls, cands, defaults, n, rng, log, sets=1, key_c
     return best, rows
 
 
-def tune_ragged(mi, torch, ctx, log):
-    """Ragged batch launch parameters on the 50000-op batch: block size of the
-    persistent kernel, the matrix-core threshold (shorter operand limbs) and
-    whether the two graph branches overlap. Returns the best as a dict."""
+def tune_ragged(mi, torch, ctx, log, classes=(1024, 512, 256, 128, 64, 32, 16, 8, 4)):
+    """Ragged batch launch parameters on the 50000-op batch. First the
+    global keys: the matrix-core threshold (shorter operand limbs) and
+    whether the graph branches overlap; then, size class by size class
+    (coordinate descent, largest class first), the persistent kernel's
+    threads and columns per lane group (ragged.threads@C, ragged.q@C) and
+    the matrix-core node's waves per workgroup (ragged.mfma_waves@C). Every
+    candidate's output must equal the first plan's before it is timed, and a
+    class setting is kept only if the batch gets at least 1 % faster.
+    Returns the tuning-file lines."""
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
     from workloads import make_ragged
 
     ops, a, b, rl = make_ragged(mi.gen_limbs)
     da = torch.from_numpy(a.view(np.int64)).to("cuda:0")
     db = torch.from_numpy(b.view(np.int64)).to("cuda:0")
-    ref = None
-    best, best_ms = None, None
-    for th in (256, 512):
-        for mf_min in (0, 32, 64, 128):
-            for ov in (0, 1):
-                if mf_min == 0 and ov == 1:
-                    continue
-                cand = {"ragged.threads": th, "ragged.mfma_min": mf_min, "ragged.overlap": ov}
-                for k, v in cand.items():
-                    mi.tune_set(k, v)
-                dr = torch.zeros(rl, dtype=torch.int64, device="cuda:0")
-                plan = mi.RaggedPlan(ctx, ops, da, db, dr, threads=th)
-                plan.launch()
-                torch.cuda.synchronize()
-                if ref is None:
-                    ref = dr.clone()
-                elif not torch.equal(dr, ref):
-                    log(f"  ragged {cand}: output differs -- rejected")
-                    plan.close()
-                    continue
-                ms = _timed(torch, plan.launch, 10)
-                plan.close()
-                log(f"  ragged {cand}: {ms:.3f} ms")
-                if best_ms is None or ms < best_ms:
-                    best, best_ms = cand, ms
+    state = {"ref": None}
+
+    def trial(cand):
+        for k, v in cand.items():
+            mi.tune_set(k, v)
+        dr = torch.zeros(rl, dtype=torch.int64, device="cuda:0")
+        plan = mi.RaggedPlan(ctx, ops, da, db, dr)
+        plan.launch()
+        torch.cuda.synchronize()
+        if state["ref"] is None:
+            state["ref"] = dr.clone()
+        elif not torch.equal(dr, state["ref"]):
+            log(f"  ragged {cand}: output differs -- rejected")
+            plan.close()
+            return None
+        ms = _timed(torch, plan.launch, 30)
+        plan.close()
+        return ms
+
+    best, best_ms = {}, None
+    for mf_min in (0, 32, 64, 128):
+        for ov in (0, 1):
+            if mf_min == 0 and ov == 1:
+                continue
+            cand = {"ragged.mfma_min": mf_min, "ragged.overlap": ov}
+            ms = trial(cand)
+            if ms is None:
+                continue
+            log(f"  ragged {cand}: {ms:.3f} ms")
+            if best_ms is None or ms < best_ms:
+                best, best_ms = cand, ms
     for k, v in best.items():
         mi.tune_set(k, v)
-    return best
+    lines = [f"{k} {v}" for k, v in best.items()]
+
+    # classes present in the batch, by where their ops go
+    mf_min = best["ragged.mfma_min"]
+    has_rg, has_mf = set(), set()
+    for kind, na, nb, *_ in ops:
+        c = 1
+        while c < max(na, nb):
+            c *= 2
+        (has_mf if kind == 1 and mf_min > 0 and min(na, nb) >= mf_min else has_rg).add(c)
+    for c in classes:
+        cands = []
+        if c in has_rg:
+            cands += [{f"ragged.q@{c}": q, f"ragged.threads@{c}": th} for q in (4, 8) for th in (256, 512)]
+        if c in has_mf:
+            cands += [{f"ragged.mfma_waves@{c}": w} for w in (1, 2, 4, 8)]
+        for cand in cands:
+            keep = {k: _get(mi, k) for k in cand}
+            ms = trial(cand)
+            if ms is not None and ms < 0.99 * best_ms:  # at least 1 %: below that is timing noise
+                best_ms = ms
+                log(f"  ragged {cand}: {ms:.3f} ms (kept)")
+            else:
+                for k, v in keep.items():
+                    mi.tune_set(k, v)
+        lines += [f"{k} {_get(mi, k)}" for k in sorted({k for cand in cands for k in cand})]
+    log(f"  ragged: best {best_ms:.3f} ms")
+    return lines
 
 
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", default=os.path.join(HERE, "tune", "mi355x.txt"))
     ap.add_argument("--quick", action="store_true", help="fewer classes and candidates")
+    ap.add_argument("--only", default="add,mul,ntt,ragged", help="comma-separated parts to tune")
+    ap.add_argument("--base", default=os.path.join(HERE, "tune", "mi355x.txt"),
+                    help="with --only: the table whose other parts are kept")
     args = ap.parse_args(argv)
+    parts = set(args.only.split(","))
 
     import torch
 
@@ -177,7 +231,16 @@ def main(argv=None):
         torch.cuda.synchronize()
     del lead
 
-    add_classes = [16, 64, 256] if args.quick else [1, 2, 4, 8, 16, 32, 64, 128, 256]
+    if parts != {"add", "mul", "ntt", "ragged"} and os.path.exists(args.base):
+        # keep the parts not re-tuned (and apply them: they shape the others' timing)
+        with open(args.base) as f:
+            for line in f:
+                kv = line.split()
+                if len(kv) == 2 and not line.startswith("#") and kv[0].split(".")[0] not in parts:
+                    lines.append(line.strip())
+                    mi.tune_set(kv[0], int(kv[1]))
+
+    add_classes = [] if "add" not in parts else [16, 64, 256] if args.quick else [1, 2, 4, 8, 16, 32, 64, 128, 256]
     add_grid = {"add.rows": [1, 2, 4], "add.threads": [64, 128, 256, 512], "add.nt": [0, 1]}
     for cls in add_classes:
         # 32 MiB per operand, 4 rotating sets (384 MiB > the 256 MiB Infinity Cache)
@@ -185,7 +248,8 @@ def main(argv=None):
                           (1 << 22) // cls, rng, log, sets=4)
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
 
-    mul_classes = [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512, 1024, 2048]
+    mul_classes = [] if "mul" not in parts else [64, 128, 512] if args.quick else [4, 8, 16, 32, 64, 128, 256, 512,
+                                                                                     1024, 2048]
     # word kernel: Q x register park x prefetch x block size; digit kernel:
     # register park x prefetch; matrix-core kernel: waves per instance
     mul_cands = grid_of({"mul.mfma": [0], "mul.digits": [0], "mul.q": [8, 16], "mul.rpark": [0, 1],
@@ -206,7 +270,7 @@ def main(argv=None):
         lines += [f"{k}@{cls} {v}" for k, v in best.items()]
 
     # NTT values per thread, keyed by transform length L = 8m (2^11..2^14)
-    ntt_classes = [2048] if args.quick else [256, 512, 1024, 2048]
+    ntt_classes = [] if "ntt" not in parts else [2048] if args.quick else [256, 512, 1024, 2048]
     for cls in ntt_classes:
         L = 8 * cls
         n = max(256, (1 << 21) // cls)
@@ -214,7 +278,8 @@ def main(argv=None):
                           key_cls=L)
         lines += [f"{k}@{L} {v}" for k, v in best.items()]
 
-    lines += [f"{k} {v}" for k, v in tune_ragged(mi, torch, ctx, log).items()]
+    if "ragged" in parts:
+        lines += tune_ragged(mi, torch, ctx, log)
 
     os.makedirs(os.path.dirname(args.out), exist_ok=True)
     with open(args.out, "w") as f:

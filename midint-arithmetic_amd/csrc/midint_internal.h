@@ -45,7 +45,8 @@ bool mul_mfma_supported(uint32_t m, bool full);
 hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                            hipStream_t s, Strides st = {});
 // Ragged multiplications on the matrix cores: one product (n_a >= n_b limbs,
-// full n_a + n_b limb result) per workgroup of kMfRgWaves waves.
+// full n_a + n_b limb result) per workgroup of 1, 2, 4 or 8 waves (default
+// kMfRgWaves).
 constexpr uint32_t kMfRgWaves = 4;
 struct MfRgItem {
     const uint64_t* a;  // n_a limbs (the longer operand)
@@ -65,7 +66,8 @@ struct MfRgConfig {
     size_t lds;
     uint32_t pow8;
 };
-bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out);
+bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out,
+                    uint32_t waves = kMfRgWaves);
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ): the launch
 // chain over 4*n*m limbs of scratch, and the fused one-kernel version
 // (mul_mfma.hip; no scratch) where its LDS layout fits.
@@ -129,6 +131,7 @@ struct PrimeTables {
 };
 
 // Ragged batches.
+constexpr uint32_t kMulQ = 4;          // default columns per lane group of ragged products
 constexpr uint32_t kRgWindow = 2048;   // output limbs per work item
 constexpr uint32_t kRgMaxLanes = 512;  // virtual lanes per multiplication item
 constexpr uint32_t kRgMaxLimbs = 1024; // operand limbs
@@ -152,10 +155,15 @@ struct RgItem {
 };
 
 size_t ragged_lds_size(uint32_t threads, uint32_t conv_cap);
-// Packs ops (offsets relative to base_a / base_b / base_r) into work items.
+// Packs ops (offsets relative to base_a / base_b / base_r) into work items
+// for a persistent kernel whose products use q (4 or 8) columns per lane group.
 midint_status ragged_pack(const midint_op* ops, size_t n_ops, uint64_t base_a, uint64_t base_b, uint64_t base_r,
-                          std::vector<RgItem>& items, std::vector<RgOp>& rgops, uint32_t* conv_cap);
-const void* ragged_kernel_ptr(uint32_t threads);
+                          std::vector<RgItem>& items, std::vector<RgOp>& rgops, uint32_t* conv_cap,
+                          uint32_t q = kMulQ);
+// The persistent kernel for `threads` (256 or 512) and q (4 or 8); its
+// arguments are (items, ops, n_items, counter, A, B, R, conv_cap).
+const void* ragged_kernel_ptr(uint32_t threads, uint32_t q = kMulQ);
+// Zeroes counters (uint32_t*, uint32_t n): one block of 64 threads, n <= 64.
 const void* ragged_reset_ptr();
 
 // Serving: products of operands above 2048 limbs are split into four

@@ -450,7 +450,5 @@ __device__ __forceinline__ void digit_lane_store(const ColGroups<8>& g, bool two
     digit_group_store(fl, fh, k2, E, O, P);
 }
 
-// The ragged kernel uses Q = 4.
-constexpr uint32_t kMulQ = 4;
 
 }  // namespace midint

@@ -527,10 +527,10 @@ __global__ __launch_bounds__(1024) void mul_mfma_kernel(uint32_t n_inst, uint32_
 }
 
 // Ragged batches: one product per workgroup (items[blockIdx.x], n_a >= n_b),
-// all waves on its column tiles.
-template <int R>
-__global__ __launch_bounds__(64 * kMfRgWaves) void mul_mfma_rg_kernel(const MfRgItem* __restrict__ items,
-                                                                      uint32_t pow8) {
+// all WAVES waves on its column tiles.
+template <int R, int WAVES>
+__global__ __launch_bounds__(64 * WAVES) void mul_mfma_rg_kernel(const MfRgItem* __restrict__ items,
+                                                                 uint32_t pow8) {
     extern __shared__ __align__(16) uint64_t lds64[];
     __shared__ uint32_t lds_gp[16];
     const MfRgItem it = items[blockIdx.x];
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(64 * kMfRgWaves) void mul_mfma_rg_kernel(const MfRg
     g.N = (g.PL + 3) / 4;
     g.NT = (g.N + 31) / 32;
     g.PLp = 4 * g.N + 4;
-    g.wpi = kMfRgWaves;
+    g.wpi = WAVES;
     g.ipb = 1;
     g.mg_pl = it.mg_pl;
     mf_block<R>(g, 1, it.a, 0, it.b, 0, it.r, 0, pow8, lds64, lds_gp);
@@ -768,24 +768,35 @@ MfRgItem mfma_rg_item(const uint64_t* a, const uint64_t* b, uint64_t* r, uint32_
     return it;
 }
 
-bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out) {
+template <int WAVES>
+static const void* mfma_rg_func(uint32_t R) {
+    switch (R) {
+        case 1: return (const void*)mul_mfma_rg_kernel<1, WAVES>;
+        case 2: return (const void*)mul_mfma_rg_kernel<2, WAVES>;
+        case 4: return (const void*)mul_mfma_rg_kernel<4, WAVES>;
+        default: return (const void*)mul_mfma_rg_kernel<8, WAVES>;
+    }
+}
+
+bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves) {
     if (max_na == 0 || max_nb == 0 || max_na > kMulMaxLimbs || max_nb > max_na || max_pl > max_na + max_nb)
         return false;
+    if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
     uint32_t R = 1;
-    while (kMfRgWaves * R * kRowLimbs < max_pl) R *= 2;
+    while (waves * R * kRowLimbs < max_pl) R *= 2;
     if (R > 8) return false;
     // mf_div bound of the normalisation: window limb index x PL < 2^32
-    if (((uint64_t)kMfRgWaves * R * kRowLimbs + 2) * max_pl >= (1ull << 32)) return false;
+    if (((uint64_t)waves * R * kRowLimbs + 2) * max_pl >= (1ull << 32)) return false;
     const uint32_t N = (max_pl + 3) / 4, Pa = max_na / 4 + 1, Pb = max_nb / 4 + 1;
     out->lds = (size_t)16 * (4 * N + 4) + mf_a_bytes(Pa, 1) + mf_br_bytes(Pb);
     if (out->lds > 160 * 1024 - 256) return false;
-    out->threads = 64 * kMfRgWaves;
+    out->threads = 64 * waves;
     out->pow8 = 256u;
-    switch (R) {
-        case 1: out->func = (const void*)mul_mfma_rg_kernel<1>; break;
-        case 2: out->func = (const void*)mul_mfma_rg_kernel<2>; break;
-        case 4: out->func = (const void*)mul_mfma_rg_kernel<4>; break;
-        default: out->func = (const void*)mul_mfma_rg_kernel<8>; break;
+    switch (waves) {
+        case 1: out->func = mfma_rg_func<1>(R); break;
+        case 2: out->func = mfma_rg_func<2>(R); break;
+        case 4: out->func = mfma_rg_func<4>(R); break;
+        default: out->func = mfma_rg_func<8>(R); break;
     }
     return true;
 }

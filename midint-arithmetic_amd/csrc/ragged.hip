@@ -33,11 +33,14 @@ struct RgCfg {
     static constexpr int kRounds = kRgMaxLanes / THREADS;
 };
 
-__global__ void ragged_reset(uint32_t* counter) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) counter[0] = 0u;
+// Zeroes the n work counters of a plan (one per persistent launch).
+__global__ void ragged_reset(uint32_t* counter, uint32_t n) {
+    if (blockIdx.x == 0 && threadIdx.x < n) counter[threadIdx.x] = 0u;
 }
 
-template <int THREADS>
+// Q: u32 columns per lane group of the products (4 or 8), the same for every
+// item of the launch (the planner packs the ops of a launch with its Q).
+template <int THREADS, int Q>
 __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restrict__ items,
                                                          const RgOp* __restrict__ ops, uint32_t n_items,
                                                          uint32_t* __restrict__ counter,
@@ -45,7 +48,6 @@ __global__ __launch_bounds__(THREADS) void ragged_kernel(const RgItem* __restric
                                                          const uint64_t* __restrict__ B, uint64_t* __restrict__ Rr,
                                                          uint32_t conv_cap) {
     using C = RgCfg<THREADS>;
-    constexpr uint32_t Q = kMulQ;
     constexpr int R = C::kRows;
     extern __shared__ __align__(16) uint64_t lds64[];
     __shared__ uint32_t s_item;
@@ -214,8 +216,9 @@ size_t ragged_lds_size(uint32_t threads, uint32_t conv_cap) {
     return threads == 512 ? ragged_lds_bytes<512>(conv_cap) : ragged_lds_bytes<256>(conv_cap);
 }
 
-const void* ragged_kernel_ptr(uint32_t threads) {
-    return threads == 512 ? (const void*)ragged_kernel<512> : (const void*)ragged_kernel<256>;
+const void* ragged_kernel_ptr(uint32_t threads, uint32_t q) {
+    if (q == 8) return threads == 512 ? (const void*)ragged_kernel<512, 8> : (const void*)ragged_kernel<256, 8>;
+    return threads == 512 ? (const void*)ragged_kernel<512, 4> : (const void*)ragged_kernel<256, 4>;
 }
 
 const void* ragged_reset_ptr() { return (const void*)ragged_reset; }

@@ -1,8 +1,17 @@
 // Host side of ragged batches: packing ops into work items, upload, and the
-// hipGraph replayed by midint_ragged_launch: counter reset node -> matrix-core
-// node (multiplications whose shorter operand has at least ragged.mfma_min
-// limbs, one product per workgroup, largest first) -> persistent kernel node
-// (every other op; with ragged.overlap = 1 it depends on the reset only).
+// hipGraph replayed by midint_ragged_launch. The graph is a counter reset
+// node followed by
+//  * the multiplications whose shorter operand has at least ragged.mfma_min
+//    limbs on the matrix cores, one product per workgroup of
+//    ragged.mfma_waves@C waves (1, 2, 4 or 8) for size class C (a power of
+//    two; the class of an op is that of its longer operand, (C/2, C] limbs):
+//    one node per distinct wave count, largest products first, LDS sized for
+//    the node's largest product;
+//  * one persistent kernel per (threads, Q) pair that the size classes of
+//    the other ops select (ragged.threads@C: 256 or 512; ragged.q@C: 4 or 8
+//    columns per lane group), each with its own work counter;
+// all as parallel branches after the reset (ragged.overlap = 1), or the
+// persistent kernels after every matrix-core node (0).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,19 +24,32 @@
 
 using namespace midint;
 
-struct midint_ragged_plan {
-    int device = 0;
+// One persistent launch: items of the ops whose size class selects these
+// threads and Q.
+struct RgLaunch {
+    uint32_t threads = 256, q = 4;
     RgItem* d_items = nullptr;
     RgOp* d_ops = nullptr;
-    uint32_t* d_counter = nullptr;
-    MfRgItem* d_mf_items = nullptr;
-    uint32_t n_items = 0;
-    uint32_t n_mf_items = 0;
-    MfRgConfig mf = {};
-    uint32_t threads = 256;
-    uint32_t conv_cap = 0;
-    uint32_t grid = 0;
+    uint32_t* d_counter = nullptr;  // element of the plan's counter array
+    uint32_t n_items = 0, conv_cap = 0, grid = 0;
     size_t lds = 0;
+    const void* fn = nullptr;
+};
+
+// One matrix-core node: the products of the size classes with one wave count.
+struct MfLaunch {
+    uint32_t waves = 0;
+    MfRgItem* d_items = nullptr;
+    uint32_t n_items = 0;
+    MfRgConfig cfg = {};
+};
+
+struct midint_ragged_plan {
+    int device = 0;
+    uint32_t* d_counter = nullptr;  // one counter per persistent launch
+    std::vector<RgLaunch> rg;
+    std::vector<MfLaunch> mf;
+    uint32_t n_counters = 0;
     const uint64_t* A = nullptr;
     const uint64_t* B = nullptr;
     uint64_t* R = nullptr;
@@ -39,16 +61,19 @@ namespace {
 
 constexpr uint32_t kConvLimit = 4 * kRgWindow;  // words: the L/H window area is reused for staging
 
-uint32_t words_pad(uint32_t limbs) { return (2 * limbs + kMulQ - 1) / kMulQ * kMulQ; }
+uint32_t words_pad(uint32_t limbs, uint32_t Q) { return (2 * limbs + Q - 1) / Q * Q; }
 
 void free_plan(midint_ragged_plan* p) {
     if (!p) return;
     if (p->exec) (void)hipGraphExecDestroy(p->exec);
     if (p->graph) (void)hipGraphDestroy(p->graph);
-    if (p->d_items) (void)hipFree(p->d_items);
-    if (p->d_ops) (void)hipFree(p->d_ops);
+    for (RgLaunch& l : p->rg) {
+        if (l.d_items) (void)hipFree(l.d_items);
+        if (l.d_ops) (void)hipFree(l.d_ops);
+    }
+    for (MfLaunch& l : p->mf)
+        if (l.d_items) (void)hipFree(l.d_items);
     if (p->d_counter) (void)hipFree(p->d_counter);
-    if (p->d_mf_items) (void)hipFree(p->d_mf_items);
     delete p;
 }
 
@@ -60,8 +85,8 @@ namespace midint {
 // for malformed or oversized ops. Offsets in the returned RgOps are the ops'
 // own limb offsets plus the given bases.
 midint_status ragged_pack(const midint_op* ops, size_t n_ops, uint64_t base_a, uint64_t base_b, uint64_t base_r,
-                          std::vector<RgItem>& h_items, std::vector<RgOp>& h_ops, uint32_t* conv_cap_out) {
-    constexpr uint32_t Q = kMulQ;
+                          std::vector<RgItem>& h_items, std::vector<RgOp>& h_ops, uint32_t* conv_cap_out, uint32_t Q) {
+    if (Q != 4 && Q != 8) return MIDINT_EINVAL;
     std::vector<RgOp> muls, adds;
     for (size_t i = 0; i < n_ops; i++) {
         const midint_op& o = ops[i];
@@ -76,8 +101,8 @@ midint_status ragged_pack(const midint_op* ops, size_t n_ops, uint64_t base_a, u
             r.off_l = r.swap ? base_a + o.off_a : base_b + o.off_b;
             r.ns = r.swap ? o.n_b : o.n_a;
             r.nl = r.swap ? o.n_a : o.n_b;
-            r.ns_w = words_pad(r.ns);
-            r.nl_w = words_pad(r.nl);
+            r.ns_w = words_pad(r.ns, Q);
+            r.nl_w = words_pad(r.nl, Q);
             r.lanes = r.nl_w / Q;
             r.seg_len = (r.ns_w + r.nl_w) / 2;
             r.out_len = o.n_a + o.n_b;
@@ -172,39 +197,32 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
                                         midint_ragged_plan** out) {
     if (!ctx || !out || (n_ops > 0 && (!ops || !d_a || !d_b || !d_r))) return MIDINT_EINVAL;
     *out = nullptr;
-    // multiplications for the matrix-core node: shorter operand >= mfma_min
-    // limbs (0: none); the rest go to the persistent kernel
+    // multiplications for the matrix-core nodes: shorter operand >= mfma_min
+    // limbs (0: none), grouped by the size class of the longer operand; the
+    // other ops go to the persistent launch their class selects
     const int64_t mf_min = tune_get("ragged.mfma_min", 32);
-    std::vector<midint_op> rest;
-    std::vector<MfRgItem> h_mf;
-    uint32_t mf_na = 0, mf_nb = 0, mf_pl = 0;
-    rest.reserve(n_ops);
+    std::vector<std::vector<MfRgItem>> mf_by_class(32);  // index: log2 of the size class
+    std::vector<midint_op> rest_by_launch[4];  // index: (threads == 512) * 2 + (q == 8)
     for (size_t i = 0; i < n_ops; i++) {
         const midint_op& o = ops[i];
-        const bool ok = o.reserved == 0 && o.kind == MIDINT_OP_MUL && o.n_a <= kRgMaxLimbs && o.n_b <= kRgMaxLimbs;
-        if (ok && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
+        if (o.reserved != 0 || (o.kind != MIDINT_OP_ADD && o.kind != MIDINT_OP_MUL)) return MIDINT_EINVAL;
+        if (o.n_a == 0 || o.n_b == 0) return MIDINT_EINVAL;
+        if (o.n_a > kRgMaxLimbs || o.n_b > kRgMaxLimbs) return MIDINT_EUNSUPPORTED;
+        const uint32_t nmax = std::max(o.n_a, o.n_b);
+        if (o.kind == MIDINT_OP_MUL && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
             const bool sw = o.n_a < o.n_b;  // the longer operand is the kernel's a
             const uint64_t* x = sw ? d_b + o.off_b : d_a + o.off_a;
             const uint64_t* y = sw ? d_a + o.off_a : d_b + o.off_b;
             const uint32_t nx = (uint32_t)(sw ? o.n_b : o.n_a), ny = (uint32_t)(sw ? o.n_a : o.n_b);
-            h_mf.push_back(mfma_rg_item(x, y, d_r + o.off_r, nx, ny));
-            mf_na = std::max(mf_na, nx);
-            mf_nb = std::max(mf_nb, ny);
-            mf_pl = std::max(mf_pl, nx + ny);
+            uint32_t c = 0;
+            while ((1u << c) < nmax) c++;
+            mf_by_class[c].push_back(mfma_rg_item(x, y, d_r + o.off_r, nx, ny));
         } else {
-            rest.push_back(o);
+            int64_t th = (tuning && tuning->threads) ? tuning->threads : tune_get_class("ragged.threads", nmax, 256);
+            const int64_t q = tune_get_class("ragged.q", nmax, 4);
+            rest_by_launch[(th == 512 ? 2 : 0) + (q == 8 ? 1 : 0)].push_back(o);
         }
     }
-    std::stable_sort(h_mf.begin(), h_mf.end(), [](const MfRgItem& x, const MfRgItem& y) {
-        return (uint64_t)x.n_a * x.n_b > (uint64_t)y.n_a * y.n_b;
-    });
-    MfRgConfig mf = {};
-    if (!h_mf.empty() && !mfma_rg_config(mf_na, mf_nb, mf_pl, &mf)) return MIDINT_EUNSUPPORTED;
-    std::vector<RgItem> h_items;
-    std::vector<RgOp> h_ops;
-    uint32_t conv_cap = 0;
-    midint_status st = ragged_pack(rest.data(), rest.size(), 0, 0, 0, h_items, h_ops, &conv_cap);
-    if (st != MIDINT_OK) return st;
 
     midint_ragged_plan* p = new (std::nothrow) midint_ragged_plan;
     if (!p) return MIDINT_ENOMEM;
@@ -213,50 +231,106 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         free_plan(p);
         return MIDINT_EHIP;
     }
-    const int64_t th = (tuning && tuning->threads) ? tuning->threads : tune_get("ragged.threads", 256);
-    p->threads = th == 512 ? 512 : 256;
-    p->n_items = (uint32_t)h_items.size();
-    p->n_mf_items = (uint32_t)h_mf.size();
-    p->mf = mf;
-    p->conv_cap = conv_cap;
-    p->lds = ragged_lds_size(p->threads, p->conv_cap);
     p->A = d_a;
     p->B = d_b;
     p->R = d_r;
+    int cus = 0;
+    hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
 
-    hipError_t e = hipMalloc((void**)&p->d_counter, sizeof(uint32_t));
-    if (e == hipSuccess && !h_items.empty()) e = hipMalloc((void**)&p->d_items, h_items.size() * sizeof(RgItem));
-    if (e == hipSuccess && !h_ops.empty()) e = hipMalloc((void**)&p->d_ops, h_ops.size() * sizeof(RgOp));
-    if (e == hipSuccess && !h_items.empty())
-        e = hipMemcpy(p->d_items, h_items.data(), h_items.size() * sizeof(RgItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !h_ops.empty())
-        e = hipMemcpy(p->d_ops, h_ops.data(), h_ops.size() * sizeof(RgOp), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !h_mf.empty()) e = hipMalloc((void**)&p->d_mf_items, h_mf.size() * sizeof(MfRgItem));
-    if (e == hipSuccess && !h_mf.empty())
-        e = hipMemcpy(p->d_mf_items, h_mf.data(), h_mf.size() * sizeof(MfRgItem), hipMemcpyHostToDevice);
-    if (e == hipSuccess && !h_mf.empty() && mf.lds > 64 * 1024)
-        e = hipFuncSetAttribute(mf.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mf.lds);
-    const void* kfn = ragged_kernel_ptr(p->threads);
-    if (e == hipSuccess && p->lds > 64 * 1024)
-        e = hipFuncSetAttribute(kfn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)p->lds);
-    int per_cu = 0, cus = 0;
-    if (e == hipSuccess)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, (int)p->threads, p->lds);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
+    // ---- matrix-core nodes, one per wave count. A class whose products do
+    // not fit its wave count's rows (one wave holds at most 1024 output
+    // limbs) runs with the default count.
+    std::vector<MfRgItem> mf_by_waves[4];  // index: log2(waves)
+    for (int c = 0; c < 32; c++) {
+        if (mf_by_class[c].empty()) continue;
+        uint32_t na = 0, nb = 0, pl = 0;
+        for (const MfRgItem& it : mf_by_class[c]) {
+            na = std::max(na, it.n_a);
+            nb = std::max(nb, it.n_b);
+            pl = std::max(pl, it.n_a + it.n_b);
+        }
+        const int64_t w = tune_get_class("ragged.mfma_waves", 1u << c, kMfRgWaves);
+        MfRgConfig probe;
+        const uint32_t waves = (w == 1 || w == 2 || w == 4 || w == 8) && mfma_rg_config(na, nb, pl, &probe, (uint32_t)w)
+                                   ? (uint32_t)w
+                                   : kMfRgWaves;
+        const int lw = waves == 1 ? 0 : waves == 2 ? 1 : waves == 8 ? 3 : 2;
+        mf_by_waves[lw].insert(mf_by_waves[lw].end(), mf_by_class[c].begin(), mf_by_class[c].end());
+    }
+    for (int lw = 3; e == hipSuccess && lw >= 0; lw--) {
+        std::vector<MfRgItem>& v = mf_by_waves[lw];
+        if (v.empty()) continue;
+        std::stable_sort(v.begin(), v.end(), [](const MfRgItem& x, const MfRgItem& y) {
+            return (uint64_t)x.n_a * x.n_b > (uint64_t)y.n_a * y.n_b;
+        });
+        uint32_t na = 0, nb = 0, pl = 0;
+        for (const MfRgItem& it : v) {
+            na = std::max(na, it.n_a);
+            nb = std::max(nb, it.n_b);
+            pl = std::max(pl, it.n_a + it.n_b);
+        }
+        MfLaunch l;
+        l.waves = 1u << lw;
+        if (!mfma_rg_config(na, nb, pl, &l.cfg, l.waves)) {
+            free_plan(p);
+            return MIDINT_EUNSUPPORTED;
+        }
+        l.n_items = (uint32_t)v.size();
+        e = hipMalloc((void**)&l.d_items, v.size() * sizeof(MfRgItem));
+        if (e == hipSuccess) e = hipMemcpy(l.d_items, v.data(), v.size() * sizeof(MfRgItem), hipMemcpyHostToDevice);
+        if (e == hipSuccess && l.cfg.lds > 64 * 1024)
+            e = hipFuncSetAttribute(l.cfg.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.cfg.lds);
+        p->mf.push_back(l);
+    }
+
+    // ---- persistent launches
+    for (int k = 0; e == hipSuccess && k < 4; k++) {
+        if (rest_by_launch[k].empty()) continue;
+        RgLaunch l;
+        l.threads = (k & 2) ? 512 : 256;
+        l.q = (k & 1) ? 8 : 4;
+        std::vector<RgItem> h_items;
+        std::vector<RgOp> h_ops;
+        midint_status st = ragged_pack(rest_by_launch[k].data(), rest_by_launch[k].size(), 0, 0, 0, h_items, h_ops,
+                                       &l.conv_cap, l.q);
+        if (st != MIDINT_OK) {
+            free_plan(p);
+            return st;
+        }
+        l.n_items = (uint32_t)h_items.size();
+        l.lds = ragged_lds_size(l.threads, l.conv_cap);
+        l.fn = ragged_kernel_ptr(l.threads, l.q);
+        if (!h_items.empty()) e = hipMalloc((void**)&l.d_items, h_items.size() * sizeof(RgItem));
+        if (e == hipSuccess && !h_ops.empty()) e = hipMalloc((void**)&l.d_ops, h_ops.size() * sizeof(RgOp));
+        if (e == hipSuccess && !h_items.empty())
+            e = hipMemcpy(l.d_items, h_items.data(), h_items.size() * sizeof(RgItem), hipMemcpyHostToDevice);
+        if (e == hipSuccess && !h_ops.empty())
+            e = hipMemcpy(l.d_ops, h_ops.data(), h_ops.size() * sizeof(RgOp), hipMemcpyHostToDevice);
+        if (e == hipSuccess && l.lds > 64 * 1024)
+            e = hipFuncSetAttribute(l.fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)l.lds);
+        int per_cu = 0;
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, l.fn, (int)l.threads, l.lds);
+        if (per_cu < 1) per_cu = 1;
+        if (tuning && tuning->blocks_per_cu > 0 && (int)tuning->blocks_per_cu < per_cu)
+            per_cu = (int)tuning->blocks_per_cu;
+        l.grid = (uint32_t)std::max(1, std::min<int>(per_cu * cus, (int)std::max<uint32_t>(1, l.n_items)));
+        p->rg.push_back(l);
+    }
+    p->n_counters = (uint32_t)p->rg.size();
+    if (e == hipSuccess) e = hipMalloc((void**)&p->d_counter, 4 * sizeof(uint32_t));
+    for (size_t k = 0; k < p->rg.size(); k++) p->rg[k].d_counter = p->d_counter + k;
     if (e != hipSuccess) {
         free_plan(p);
         return MIDINT_EHIP;
     }
-    if (per_cu < 1) per_cu = 1;
-    if (tuning && tuning->blocks_per_cu > 0 && (int)tuning->blocks_per_cu < per_cu) per_cu = (int)tuning->blocks_per_cu;
-    p->grid = (uint32_t)std::max(1, std::min<int>(per_cu * cus, (int)std::max<uint32_t>(1, p->n_items)));
 
-    // ---- graph: reset the work counter, the matrix-core products, then the
-    // persistent kernel
+    // ---- graph: reset the work counters, then the matrix-core nodes and the
+    // persistent launches (the vectors are final: kernel arguments point into
+    // them)
     e = hipGraphCreate(&p->graph, 0);
-    hipGraphNode_t n_reset = nullptr, n_mf = nullptr, n_main = nullptr;
+    hipGraphNode_t n_reset = nullptr;
     if (e == hipSuccess) {
-        void* rargs[] = {(void*)&p->d_counter};
+        void* rargs[] = {(void*)&p->d_counter, (void*)&p->n_counters};
         hipKernelNodeParams kp = {};
         kp.func = const_cast<void*>(ragged_reset_ptr());
         kp.gridDim = dim3(1);
@@ -266,32 +340,38 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         kp.extra = nullptr;
         e = hipGraphAddKernelNode(&n_reset, p->graph, nullptr, 0, &kp);
     }
-    hipGraphNode_t n_prev = n_reset;
-    if (e == hipSuccess && p->n_mf_items > 0) {
-        void* margs[] = {(void*)&p->d_mf_items, (void*)&p->mf.pow8};
+    std::vector<hipGraphNode_t> mf_nodes;
+    for (MfLaunch& l : p->mf) {
+        if (e != hipSuccess) break;
+        void* margs[] = {(void*)&l.d_items, (void*)&l.cfg.pow8};
         hipKernelNodeParams kp = {};
-        kp.func = const_cast<void*>(p->mf.func);
-        kp.gridDim = dim3(p->n_mf_items);
-        kp.blockDim = dim3(p->mf.threads);
-        kp.sharedMemBytes = (unsigned)p->mf.lds;
+        kp.func = const_cast<void*>(l.cfg.func);
+        kp.gridDim = dim3(l.n_items);
+        kp.blockDim = dim3(l.cfg.threads);
+        kp.sharedMemBytes = (unsigned)l.cfg.lds;
         kp.kernelParams = margs;
         kp.extra = nullptr;
-        e = hipGraphAddKernelNode(&n_mf, p->graph, &n_reset, 1, &kp);
-        // ragged.overlap = 1: the persistent kernel does not wait for the
-        // matrix-core node (parallel branches after the reset)
-        if (tune_get("ragged.overlap", 1) == 0) n_prev = n_mf;
+        hipGraphNode_t n = nullptr;
+        e = hipGraphAddKernelNode(&n, p->graph, &n_reset, 1, &kp);
+        mf_nodes.push_back(n);
     }
-    if (e == hipSuccess && p->n_items > 0) {
-        void* kargs[] = {(void*)&p->d_items, (void*)&p->d_ops, (void*)&p->n_items, (void*)&p->d_counter,
-                         (void*)&p->A,       (void*)&p->B,     (void*)&p->R,       (void*)&p->conv_cap};
+    // ragged.overlap = 1: the persistent kernels do not wait for the
+    // matrix-core nodes (the branches write disjoint result ranges)
+    std::vector<hipGraphNode_t> deps = {n_reset};
+    if (tune_get("ragged.overlap", 1) == 0 && !mf_nodes.empty()) deps = mf_nodes;
+    for (RgLaunch& l : p->rg) {
+        if (e != hipSuccess || l.n_items == 0) continue;
+        void* kargs[] = {(void*)&l.d_items, (void*)&l.d_ops, (void*)&l.n_items, (void*)&l.d_counter,
+                         (void*)&p->A,      (void*)&p->B,    (void*)&p->R,      (void*)&l.conv_cap};
         hipKernelNodeParams kp = {};
-        kp.func = const_cast<void*>(kfn);
-        kp.gridDim = dim3(p->grid);
-        kp.blockDim = dim3(p->threads);
-        kp.sharedMemBytes = (unsigned)p->lds;
+        kp.func = const_cast<void*>(l.fn);
+        kp.gridDim = dim3(l.grid);
+        kp.blockDim = dim3(l.threads);
+        kp.sharedMemBytes = (unsigned)l.lds;
         kp.kernelParams = kargs;
         kp.extra = nullptr;
-        e = hipGraphAddKernelNode(&n_main, p->graph, &n_prev, 1, &kp);
+        hipGraphNode_t n = nullptr;
+        e = hipGraphAddKernelNode(&n, p->graph, deps.data(), deps.size(), &kp);
     }
     if (e == hipSuccess) e = hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0);
     if (e != hipSuccess) {
@@ -304,7 +384,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
 
 midint_status midint_ragged_launch(midint_ragged_plan* p, void* stream) {
     if (!p) return MIDINT_EINVAL;
-    if (p->n_items == 0 && p->n_mf_items == 0) return MIDINT_OK;
+    if (p->rg.empty() && p->mf.empty()) return MIDINT_OK;
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess) return MIDINT_EHIP;
     if (cur != p->device && hipSetDevice(p->device) != hipSuccess) return MIDINT_EHIP;
@@ -314,9 +394,16 @@ midint_status midint_ragged_launch(midint_ragged_plan* p, void* stream) {
 midint_status midint_ragged_plan_stats(const midint_ragged_plan* p, uint64_t* n_items, uint64_t* grid,
                                        uint64_t* lds_bytes) {
     if (!p) return MIDINT_EINVAL;
-    if (n_items) *n_items = (uint64_t)p->n_items + p->n_mf_items;
-    if (grid) *grid = p->grid;
-    if (lds_bytes) *lds_bytes = p->lds;
+    uint64_t items = 0, g = 0, lds = 0;
+    for (const RgLaunch& l : p->rg) {
+        items += l.n_items;
+        g += l.grid;
+        lds = std::max<uint64_t>(lds, l.lds);
+    }
+    for (const MfLaunch& l : p->mf) items += l.n_items;
+    if (n_items) *n_items = items;
+    if (grid) *grid = g;
+    if (lds_bytes) *lds_bytes = lds;
     return MIDINT_OK;
 }
 

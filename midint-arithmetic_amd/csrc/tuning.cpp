@@ -17,7 +17,12 @@
 //   ntt.fast      1: register-pass NTT kernel for transforms of 2^10..2^14 points
 //   ntt.vals      values per thread of that kernel, per transform length
 //                 ("ntt.vals@L"): 16 or 32, 0 for the built-in choice
-//   ragged.threads  256 or 512
+//   ragged.threads  threads of the persistent kernel for ops of a size class
+//                 ("ragged.threads@C"): 256 or 512
+//   ragged.q      columns per lane group of that kernel's products, per size
+//                 class ("ragged.q@C"): 4 or 8
+//   ragged.mfma_waves waves per workgroup of the ragged matrix-core node of a
+//                 size class ("ragged.mfma_waves@C"): 1, 2, 4 or 8
 //   ragged.mfma_min ragged multiplications whose shorter operand has at least
 //                 this many limbs run on the matrix-core kernel (0: none)
 //   ragged.overlap  1: the persistent kernel runs beside the matrix-core node
@@ -39,7 +44,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
     };
     return t;
 }

@@ -87,10 +87,12 @@ def test_tuning_table(mi, tmp_path):
 
 def test_shipped_tuning_table_loads(mi):
     """Every key of the shipped table is one the library knows (an unknown
-    key fails the load), including the matrix-core and ragged-graph keys."""
+    key fails the load), including the matrix-core keys and the ragged
+    graph's global and per-size-class keys."""
     path = os.path.join(mi.HERE, "tune", "mi355x.txt")
     keys = [ln.split()[0] for ln in open(path) if ln.strip() and not ln.startswith("#")]
-    assert {"mul.mfma@128", "ragged.mfma_min", "ragged.overlap"} <= set(keys)
+    assert {"mul.mfma@128", "ragged.mfma_min", "ragged.overlap", "ragged.q@1024", "ragged.threads@1024",
+            "ragged.mfma_waves@1024", "ntt.vals@16384"} <= set(keys)
     saved = {k: mi.tune_get(k) for k in ("ragged.mfma_min", "ragged.overlap", "mul.mfma_ipb")}
     assert mi.tune_load(path)
     for k, v in saved.items():

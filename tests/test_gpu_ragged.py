@@ -141,30 +141,69 @@ def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min):
     assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
 
 
-@pytest.mark.parametrize("seed", range(8))
+RAGGED_CLASSES = [4, 8, 16, 32, 64, 128, 256, 512, 1024]
+
+
+def _set_keys(mi, cfg):
+    """Sets tuning keys (global or key@C); returns what restores them: a
+    class key that was unset gets the global value (the same lookup)."""
+    saved = {}
+    for k, v in cfg.items():
+        try:
+            saved[k] = mi.tune_get(k)
+        except Exception:
+            saved[k] = mi.tune_get(k.split("@")[0])
+        mi.tune_set(k, v)
+    return saved
+
+
+@pytest.mark.parametrize("seed", range(12))
 def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
     """Random batches (op count, size range, add/mul mix) under random
-    settings of every ragged tuning key: threads per workgroup of the
-    persistent kernel, the matrix-core threshold, and whether the persistent
-    kernel runs beside the matrix-core node or after it. Two replays of the
-    graph, each equal to the oracle."""
+    settings of every ragged tuning key: the matrix-core threshold, whether
+    the persistent kernels run beside the matrix-core nodes or after them,
+    and per size class the persistent kernel's threads and columns per lane
+    group (Q) and the matrix-core node's waves per workgroup -- or one thread
+    count for every class through the plan's own tuning argument. Two
+    replays of the graph, each equal to the oracle."""
     rng = np.random.default_rng(0xA11 + seed)
     lo = int(rng.integers(6, 11))
     hi = int(rng.integers(lo + 1, 17))
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=int(rng.integers(50, 500)), seed=100 + seed,
                                 mul_frac=float(rng.choice([0.3, 0.7, 1.0])), lo_bits=lo, hi_bits=hi)
     cfg = {"ragged.mfma_min": int(rng.choice([0, 1, 8, 32, 64])), "ragged.overlap": int(rng.integers(0, 2))}
-    threads = int(rng.choice([256, 512]))
-    saved = {k: mi.tune_get(k) for k in cfg}
-    for k, v in cfg.items():
-        mi.tune_set(k, v)
+    for c in RAGGED_CLASSES:
+        cfg[f"ragged.q@{c}"] = int(rng.choice([4, 8]))
+        cfg[f"ragged.threads@{c}"] = int(rng.choice([256, 512]))
+        cfg[f"ragged.mfma_waves@{c}"] = int(rng.choice([1, 2, 4, 8]))
+    threads = int(rng.choice([0, 256, 512]))
+    saved = _set_keys(mi, cfg)
     try:
         got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, threads=threads, reps=2)
     finally:
-        for k, v in saved.items():
-            mi.tune_set(k, v)
+        _set_keys(mi, saved)
     want = ragged_reference(orc, ops, a, b, rl)
     assert np.array_equal(got, want), (cfg, threads, int(np.flatnonzero(got != want)[0]))
+
+
+@pytest.mark.parametrize("q,threads,waves", [(8, 256, 1), (8, 512, 2), (4, 512, 8), (8, 256, 8)])
+def test_ragged_uniform_class_keys(mi, ctx, torch_dev, orc, q, threads, waves):
+    """Every class on one (Q, threads, waves) setting, with the matrix-core
+    threshold off (every product on the persistent kernel) and on."""
+    ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=800, seed=31, lo_bits=6, hi_bits=16)
+    want = ragged_reference(orc, ops, a, b, rl)
+    for mf_min in (0, 32):
+        cfg = {"ragged.mfma_min": mf_min}
+        for c in RAGGED_CLASSES:
+            cfg[f"ragged.q@{c}"] = q
+            cfg[f"ragged.threads@{c}"] = threads
+            cfg[f"ragged.mfma_waves@{c}"] = waves
+        saved = _set_keys(mi, cfg)
+        try:
+            got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
+        finally:
+            _set_keys(mi, saved)
+        assert np.array_equal(got, want), (q, threads, waves, mf_min, int(np.flatnonzero(got != want)[0]))
 
 
 def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc):
