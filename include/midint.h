@@ -93,9 +93,9 @@ midint_status midint_mul_full(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
  * call, the four products on the int8 matrix cores and the intermediates in
  * LDS, for every m up to midint_max_limbs_mul(). (With the tuning key
  * "poly.fused" = 0 it runs as a chain of launches through 4*n*m limbs of
- * context scratch instead.) Calls that use context scratch (that chain,
- * poly_ntt and the repacking u32 path) on different streams of one context
- * are ordered on the device: each waits for the previous scratch user. */
+ * context scratch instead.) Calls that use context scratch (that chain and
+ * the repacking u32 path) on different streams of one context are ordered
+ * on the device: each waits for the previous scratch user. */
 midint_status midint_poly(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                           uint64_t* d_r, void* stream);
 
@@ -114,8 +114,9 @@ midint_status midint_ntt_conv(midint_ctx* ctx, uint32_t n, uint32_t lg, const ui
 
 /* r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance with NTT products, the
  * composite of midint_poly. Replaces polyFttKer (cuda/ker-fft-mul.cu.h:
- * 456-509). Uses 4*n*m limbs of context scratch (ordered across streams as
- * for midint_poly); m <= midint_max_limbs_ntt(). */
+ * 456-509): one kernel per call, one instance per workgroup, eight
+ * transforms with the intermediates in LDS, no scratch memory;
+ * m <= midint_max_limbs_ntt(). */
 midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uint64_t* d_a, const uint64_t* d_b,
                               uint64_t* d_r, void* stream);
 

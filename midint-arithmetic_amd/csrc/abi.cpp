@@ -291,10 +291,8 @@ midint_status midint_poly_ntt(midint_ctx* ctx, uint32_t n, uint32_t m, const uin
     if (m > midint_max_limbs_ntt()) return MIDINT_EUNSUPPORTED;
     if ((st = bind_device(ctx)) != MIDINT_OK) return st;
     NttTables t;
-    if ((st = get_tables(ctx, ntt_lg_for_limbs(m), &t)) != MIDINT_OK) return st;
-    hipStream_t s = (hipStream_t)stream;
-    return with_scratch(ctx, 4ull * n * m, s,
-                        [&](uint64_t* scr) { return launch_poly_ntt(n, m, d_a, d_b, d_r, scr, t, s); });
+    if ((st = get_tables(ctx, poly_ntt_lg(m), &t)) != MIDINT_OK) return st;
+    return hip_status(launch_poly_ntt(n, m, d_a, d_b, d_r, t, (hipStream_t)stream));
 }
 
 // ------------------------------------------------- strided u64 layouts

@@ -92,9 +92,11 @@ void ntt_host_tables(uint32_t lg, uint64_t* fwd, uint64_t* inv, uint64_t* inv_le
 // lo: write only the low m limbs of each product (stride m), else all 2m.
 hipError_t launch_mul_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                           const NttTables& t, hipStream_t s, bool lo = false);
-// r = ((a^2 + b)(b^2 + b) + ab) mod B^m with NTT products; 4*n*m limbs of scratch.
+// r = ((a^2 + b)(b^2 + b) + ab) mod B^m with NTT products, one kernel, no
+// scratch; t must be the tables of 2^poly_ntt_lg(m) points.
+uint32_t poly_ntt_lg(uint32_t m);
 hipError_t launch_poly_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                           uint64_t* scratch, const NttTables& t, hipStream_t s);
+                           const NttTables& t, hipStream_t s);
 // Raw cyclic convolution of n instances of L = 2^lg values (< p each).
 hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const NttTables& t,
                            hipStream_t s);

@@ -196,7 +196,7 @@ __global__ __launch_bounds__(1024) void ntt_kernel(uint32_t m, uint32_t rl, uint
 // Coefficient i of operand `g`: 16-bit pieces of the m-limb integer for
 // products (MODE 0), raw values for the convolution mode (MODE 1).
 template <int MODE>
-__device__ __forceinline__ uint64_t ntt_coef(const uint64_t* __restrict__ g, uint32_t i, uint32_t m) {
+__device__ __forceinline__ uint64_t ntt_coef(const uint64_t* g, uint32_t i, uint32_t m) {
     if (MODE == 0) return i < 4 * m ? (g[i >> 2] >> (16 * (i & 3))) & 0xFFFFull : 0ull;
     return g[i];
 }
@@ -205,8 +205,10 @@ __device__ __forceinline__ uint64_t ntt_coef(const uint64_t* __restrict__ g, uin
 // first pass's grouping); returns with the last pass's values in v.
 // SCALE: multiply the input by `scale` first (the inverse's L^-1, applied to
 // a's nonzero inputs instead of to all L transformed values).
-template <int LG, int V, int MODE, bool SCALE = false>
-__device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
+// SYNC: a barrier between loading the inputs and the first pass's LDS
+// stores (for an input that lives in the transform's own LDS area).
+template <int LG, int V, int MODE, bool SCALE = false, bool SYNC = false>
+__device__ __forceinline__ void ntt_fwd(const uint64_t* g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
                                         const uint64_t* __restrict__ twf, uint64_t scale = 0) {
     using Pl = NttPlan<LG, V>;
     const uint32_t tid = opaque_u32(threadIdx.x);
@@ -225,7 +227,92 @@ __device__ __forceinline__ void ntt_fwd(const uint64_t* __restrict__ g, uint32_t
                 v[(j << R0) + k] = SCALE ? gl_mul(c, scale) : c;
             }
         }
+    if (SYNC) __syncthreads();
     NttFwd<LG, V, 0, ZHI>::run(x, v, tid, twf);
+}
+
+// Exact recombination of the 16-bit-spaced coefficients of a product,
+// natural order in x (padded, after NttInv): limb j is c[4j] + c[4j+1] 2^16 +
+// c[4j+2] 2^32 + c[4j+3] 2^48 (< 2^112), split into L (low 64 bits, at limb
+// j) and H (high bits, at limb j + 1) in x, then one carry-propagating row
+// addition L + H: limbs [0, outl) of the product in rows (R = V / 8 rows of
+// 128 limbs per wave over L / 4 >= outl limbs; limb k = wave R 128 + v 128 +
+// 2 lane). The low outl limbs need nothing above them: carries only go up.
+template <int LG, int V>
+__device__ __forceinline__ void ntt_recombine_rows(uint64_t* x, uint32_t outl, Rows<V / 8>& rows, uint32_t* lds_gp) {
+    constexpr uint32_t T = NttPlan<LG, V>::T;
+    constexpr uint32_t LPT = V / 4;  // limbs per thread: outl <= L / 4 = LPT * T
+    constexpr int R = V / 8;         // rows per wave: (T / 64) * R * 128 = L / 4 limbs
+    const uint32_t tid = threadIdx.x;
+    uint64_t lo_v[LPT], hi_v[LPT];
+#pragma unroll
+    for (uint32_t e = 0; e < LPT; e++) {
+        const uint32_t j = tid + T * e;
+        uint64_t lo = 0, hi = 0;
+        if (j < outl) {
+            const uint64_t c0 = x[ntt_pad(4 * j)], c1 = x[ntt_pad(4 * j + 1)], c2 = x[ntt_pad(4 * j + 2)],
+                           c3 = x[ntt_pad(4 * j + 3)];
+            unsigned __int128 acc = (unsigned __int128)c0 + ((unsigned __int128)c1 << 16) +
+                                    ((unsigned __int128)c2 << 32) + ((unsigned __int128)c3 << 48);
+            lo = (uint64_t)acc;
+            hi = (uint64_t)(acc >> 64);
+        }
+        lo_v[e] = lo;
+        hi_v[e] = hi;
+    }
+    __syncthreads();
+    uint64_t* Lb = x;
+    uint64_t* Hb = x + outl;
+    if (tid == 0) Hb[0] = 0;
+#pragma unroll
+    for (uint32_t e = 0; e < LPT; e++) {
+        const uint32_t j = tid + T * e;
+        if (j < outl) {
+            Lb[j] = lo_v[e];
+            if (j + 1 < outl) Hb[j + 1] = hi_v[e];
+        }
+    }
+    __syncthreads();
+    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int vv = 0; vv < R; vv++) {
+        const uint32_t k = k_first + vv * kRowLimbs;
+        rows.x0[vv] = k < outl ? Lb[k] : 0ull;
+        rows.x1[vv] = k + 1 < outl ? Lb[k + 1] : 0ull;
+        rows.y0[vv] = k < outl ? Hb[k] : 0ull;
+        rows.y1[vv] = k + 1 < outl ? Hb[k + 1] : 0ull;
+        rows.s0[vv] = (k == 0);
+        rows.s1[vv] = false;
+    }
+    block_add_rows<R>(rows, lds_gp);
+}
+
+// rows (as left by ntt_recombine_rows) += y[0, n) mod B^n, y in global
+// memory or LDS. Every thread of the block must call it.
+template <int R>
+__device__ __forceinline__ void ntt_rows_add(Rows<R>& rows, const uint64_t* y, uint32_t n, uint32_t* lds_gp) {
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int vv = 0; vv < R; vv++) {
+        const uint32_t k = k_first + vv * kRowLimbs;
+        rows.y0[vv] = k < n ? y[k] : 0ull;
+        rows.y1[vv] = k + 1 < n ? y[k + 1] : 0ull;
+        rows.s0[vv] = (k == 0);
+        rows.s1[vv] = false;
+    }
+    block_add_rows<R>(rows, lds_gp);
+}
+
+// Limbs [0, n) of rows to dst (global memory or LDS).
+template <int R>
+__device__ __forceinline__ void ntt_store_rows(const Rows<R>& rows, uint64_t* dst, uint32_t n) {
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
+#pragma unroll
+    for (int vv = 0; vv < R; vv++) {
+        const uint32_t k = k_first + vv * kRowLimbs;
+        if (k < n) dst[k] = rows.x0[vv];
+        if (k + 1 < n) dst[k + 1] = rows.x1[vv];
+    }
 }
 
 template <int LG, int V, int MODE>
@@ -260,62 +347,96 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
         return;
     }
 
-    // ---- exact recombination of the 16-bit-spaced coefficients: limb j is
-    // c[4j] + c[4j+1] 2^16 + c[4j+2] 2^32 + c[4j+3] 2^48 (< 2^112), split into
-    // L (low 64 bits, at limb j) and H (high bits, at limb j + 1), then one
-    // carry-propagating row addition L + H.
-    constexpr uint32_t LPT = V / 4;  // limbs per thread: 2m <= L / 4 = LPT * T
-    const uint32_t outl = 2 * m;
-    uint64_t lo_v[LPT], hi_v[LPT];
-#pragma unroll
-    for (uint32_t e = 0; e < LPT; e++) {
-        const uint32_t j = tid + T * e;
-        uint64_t lo = 0, hi = 0;
-        if (j < outl) {
-            const uint64_t c0 = x[ntt_pad(4 * j)], c1 = x[ntt_pad(4 * j + 1)], c2 = x[ntt_pad(4 * j + 2)],
-                           c3 = x[ntt_pad(4 * j + 3)];
-            unsigned __int128 acc = (unsigned __int128)c0 + ((unsigned __int128)c1 << 16) +
-                                    ((unsigned __int128)c2 << 32) + ((unsigned __int128)c3 << 48);
-            lo = (uint64_t)acc;
-            hi = (uint64_t)(acc >> 64);
-        }
-        lo_v[e] = lo;
-        hi_v[e] = hi;
-    }
-    __syncthreads();
-    uint64_t* Lb = x;
-    uint64_t* Hb = x + outl;
-    if (tid == 0) Hb[0] = 0;
-#pragma unroll
-    for (uint32_t e = 0; e < LPT; e++) {
-        const uint32_t j = tid + T * e;
-        if (j < outl) {
-            Lb[j] = lo_v[e];
-            if (j + 1 < outl) Hb[j + 1] = hi_v[e];
-        }
-    }
-    __syncthreads();
-    constexpr int R = V / 8;  // rows per wave: (T / 64) * R * 128 = L / 4 limbs
-    const uint32_t k_first = (tid / kWave) * R * kRowLimbs + 2 * lane_id();
+    Rows<V / 8> rows;
+    ntt_recombine_rows<LG, V>(x, 2 * m, rows, lds_gp);
+    ntt_store_rows<V / 8>(rows, r + (size_t)blockIdx.x * rl, rl);  // rl = 2m (full) or m (low half)
+}
+
+// The reference's polyFttKer (cuda/ker-fft-mul.cu.h:456-509) in one kernel:
+// r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance (one per workgroup),
+// the steps mod B^m in the reference's order, with eight L-point
+// transforms (L >= 8m) and at most two transformed operands in registers:
+//   P = NTT(a) / L;  a^2 + b     -> s1 in LDS (after the transform area)
+//   Q = NTT(b) / L;  ab          -> r (held there until the end)
+//                    b^2 + b     -> s2 in the transform area
+//   Q = NTT(s2) / L, P = NTT(s1) / L;  s1 s2 + ab -> r
+// Every pointwise product is of two transforms of inputs scaled by 1 / L,
+// times L (a shift by LG bits), so the unscaled inverse returns the exact
+// product coefficients; the low m limbs of each product need nothing above
+// them (carries only go up). No scratch memory: intermediates stay in LDS
+// and the instance's own m output limbs.
+template <int LG, int V>
+__global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, const uint64_t* __restrict__ a,
+                                                                  const uint64_t* __restrict__ b,
+                                                                  uint64_t* __restrict__ r,
+                                                                  const uint64_t* __restrict__ twf,
+                                                                  const uint64_t* __restrict__ twi, uint64_t inv_len) {
+    using Pl = NttPlan<LG, V>;
+    constexpr uint32_t L = 1u << LG;
+    constexpr int R = V / 8;
+    extern __shared__ __align__(16) uint64_t x[];  // transform area (L + L / 32 words), then s1 (m words)
+    __shared__ uint32_t lds_gp[16];
+    uint64_t* s1 = x + (L + L / 32);
+    const uint64_t* ga = a + (size_t)blockIdx.x * m;
+    const uint64_t* gb = b + (size_t)blockIdx.x * m;
+    uint64_t* gr = r + (size_t)blockIdx.x * m;
+    uint64_t P[V], Q[V];
     Rows<R> rows;
+
+    // ---- s1 = a^2 + b
+    ntt_fwd<LG, V, 0, true>(ga, m, x, P, twf, inv_len);
 #pragma unroll
-    for (int vv = 0; vv < R; vv++) {
-        const uint32_t k = k_first + vv * kRowLimbs;
-        rows.x0[vv] = k < outl ? Lb[k] : 0ull;
-        rows.x1[vv] = k + 1 < outl ? Lb[k + 1] : 0ull;
-        rows.y0[vv] = k < outl ? Hb[k] : 0ull;
-        rows.y1[vv] = k + 1 < outl ? Hb[k + 1] : 0ull;
-        rows.s0[vv] = (k == 0);
-        rows.s1[vv] = false;
-    }
-    block_add_rows<R>(rows, lds_gp);
-    uint64_t* gr = r + (size_t)blockIdx.x * rl;  // rl = 2m (full) or m (low half)
+    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mul(P[k], P[k]), LG);
+    __syncthreads();
+    NttInv<LG, V, Pl::NP - 1>::run(x, Q, opaque_u32(threadIdx.x), twi);
+    ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
+    ntt_rows_add<R>(rows, gb, m, lds_gp);
+    ntt_store_rows<R>(rows, s1, m);
+    __syncthreads();
+
+    // ---- ab (kept in r)
+    ntt_fwd<LG, V, 0, true>(gb, m, x, Q, twf, inv_len);
 #pragma unroll
-    for (int vv = 0; vv < R; vv++) {
-        const uint32_t k = k_first + vv * kRowLimbs;
-        if (k < rl) gr[k] = rows.x0[vv];
-        if (k + 1 < rl) gr[k + 1] = rows.x1[vv];
-    }
+    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
+    __syncthreads();
+    NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
+    ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
+    ntt_store_rows<R>(rows, gr, m);
+    __syncthreads();
+
+    // ---- s2 = b^2 + b, into the transform area
+#pragma unroll
+    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mul(Q[k], Q[k]), LG);
+    NttInv<LG, V, Pl::NP - 1>::run(x, Q, opaque_u32(threadIdx.x), twi);
+    ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
+    ntt_rows_add<R>(rows, gb, m, lds_gp);
+    ntt_store_rows<R>(rows, x, m);
+    __syncthreads();
+
+    // ---- r = s1 s2 + ab
+    ntt_fwd<LG, V, 0, true, true>(x, m, x, Q, twf, inv_len);
+    __syncthreads();  // every thread is done reading LDS for s2's last pass
+    ntt_fwd<LG, V, 0, true>(s1, m, x, P, twf, inv_len);
+#pragma unroll
+    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
+    __syncthreads();
+    NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
+    ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
+    ntt_rows_add<R>(rows, gr, m, lds_gp);
+    ntt_store_rows<R>(rows, gr, m);
+}
+
+template <int LG, int V>
+static hipError_t launch_poly_ntt_lg_v(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                                       const NttTables& t, hipStream_t s) {
+    constexpr uint32_t L = 1u << LG;
+    static_assert(L / V >= 64 && L / V <= 1024, "one to sixteen waves per block");
+    const size_t lds = (size_t)(L + L / 32 + m) * 8;
+    const void* fn = (const void*)poly_ntt_kernel<LG, V>;
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    poly_ntt_kernel<LG, V><<<n, L / V, lds, s>>>(m, a, b, r, t.d_fwd, t.d_inv, t.inv_len);
+    return hipGetLastError();
 }
 
 template <int LG, int V, int MODE>
@@ -433,24 +554,27 @@ hipError_t launch_ntt_conv(uint32_t n, const uint64_t* a, const uint64_t* b, uin
     return hipGetLastError();
 }
 
+uint32_t poly_ntt_lg(uint32_t m) {
+    const uint32_t lg = ntt_lg_for_limbs(m);
+    return lg < 10 ? 10 : lg;  // the register-pass kernel's shortest transform
+}
+
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m with NTT products (reference
 // polyFttKer, cuda/ker-fft-mul.cu.h:456-509: the same sequence of three
-// additions and four products as polyKerQ, every step mod B^m).
+// additions and four products as polyKerQ, every step mod B^m): one kernel
+// (poly_ntt_kernel) on tables of 2^poly_ntt_lg(m) points.
 hipError_t launch_poly_ntt(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
-                           uint64_t* scratch, const NttTables& t, hipStream_t s) {
-    const size_t nm = (size_t)n * m;
-    uint64_t* t1 = scratch;
-    uint64_t* t2 = scratch + nm;
-    uint64_t* t3 = scratch + 2 * nm;
-    uint64_t* t4 = scratch + 3 * nm;
-    hipError_t e;
-    if ((e = launch_mul_ntt(n, m, a, a, t1, t, s, true)) != hipSuccess) return e;       // t1 = a^2
-    if ((e = launch_add(n, m, t1, b, t2, nullptr, false, s)) != hipSuccess) return e;  // t2 = a^2 + b
-    if ((e = launch_mul_ntt(n, m, b, b, t3, t, s, true)) != hipSuccess) return e;      // t3 = b^2
-    if ((e = launch_add(n, m, t3, b, t4, nullptr, false, s)) != hipSuccess) return e;  // t4 = b^2 + b
-    if ((e = launch_mul_ntt(n, m, t2, t4, t1, t, s, true)) != hipSuccess) return e;    // t5 = t2 * t4
-    if ((e = launch_mul_ntt(n, m, a, b, t3, t, s, true)) != hipSuccess) return e;      // t6 = a * b
-    return launch_add(n, m, t1, t3, r, nullptr, false, s);                              // r = t5 + t6
+                           const NttTables& t, hipStream_t s) {
+    if (t.lg != poly_ntt_lg(m) || t.lg > kNttMaxLg || 8ull * m > (1ull << t.lg)) return hipErrorInvalidValue;
+    const bool v32 = ntt_vals(t.lg) == 32;
+    switch (t.lg) {
+        case 10: return launch_poly_ntt_lg_v<10, 16>(n, m, a, b, r, t, s);
+        case 11: return v32 ? launch_poly_ntt_lg_v<11, 32>(n, m, a, b, r, t, s) : launch_poly_ntt_lg_v<11, 16>(n, m, a, b, r, t, s);
+        case 12: return v32 ? launch_poly_ntt_lg_v<12, 32>(n, m, a, b, r, t, s) : launch_poly_ntt_lg_v<12, 16>(n, m, a, b, r, t, s);
+        case 13: return v32 ? launch_poly_ntt_lg_v<13, 32>(n, m, a, b, r, t, s) : launch_poly_ntt_lg_v<13, 16>(n, m, a, b, r, t, s);
+        case 14: return v32 ? launch_poly_ntt_lg_v<14, 32>(n, m, a, b, r, t, s) : launch_poly_ntt_lg_v<14, 16>(n, m, a, b, r, t, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace midint

@@ -386,17 +386,25 @@ def test_poly_fused_layouts(mi, ctx, torch_dev, orc, m, wpi, ipb):
     assert np.array_equal(host(dr), orc.poly(a, b, m)), (m, wpi, ipb)
 
 
-@pytest.mark.parametrize("m", [1, 3, 64, 100, 1024, 2048])
-def test_poly_ntt(ctx, torch_dev, orc, m):
-    """NTT composite (polyFttKer): same closed form as the classical one."""
+POLY_NTT_SIZES = [1, 2, 3, 64, 100, 127, 128, 129, 255, 256, 511, 512, 1000, 1024, 2047, 2048]
+
+
+@pytest.mark.parametrize("vals", [16, 32])
+@pytest.mark.parametrize("m", POLY_NTT_SIZES)
+def test_poly_ntt(mi, ctx, torch_dev, orc, m, vals):
+    """NTT composite (polyFttKer, one kernel): same closed form as the
+    classical one, with 16 and 32 values per thread (32 needs transforms of
+    2^11 points or more, 256 limbs and up)."""
     torch = torch_dev
     n = 3 if m >= 1024 else 9
     a, b = operands(m * 11 + 5, n, m)
     dr = empty(torch, n * m)
-    ctx.poly_ntt(dev(torch, a), dev(torch, b), dr, m)
-    torch.cuda.synchronize()
-    want = orc.poly(a, b, m)
-    assert np.array_equal(host(dr), want)
+    L = 1 << max(10, (8 * m - 1).bit_length())
+    with tuned(mi, L, {"ntt.vals": vals}):
+        ctx.poly_ntt(dev(torch, a), dev(torch, b), dr, m)
+        torch.cuda.synchronize()
+    got, want = host(dr), orc.poly(a, b, m)
+    assert np.array_equal(got, want), (m, vals, int(np.flatnonzero(got != want)[0]))
     if m <= midint_max_mul(ctx):
         dc = empty(torch, n * m)
         ctx.poly(dev(torch, a), dev(torch, b), dc, m)
