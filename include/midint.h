@@ -325,6 +325,18 @@ midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, 
                                   const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
 midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
                                       const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec);
+/* The composites each reference driver times after its base operation,
+ * with the same allocate / dry run / `runs` timed launches / copy-back flow
+ * (m limbs out):
+ *   midint_gpu_add6_host     <- a6pb10Ker in gpuAdd (cuda/main.cu:121-157)
+ *   midint_gpu_poly_host     <- polyKerQ in gpuMultiply (cuda/main.cu:317-351)
+ *   midint_gpu_poly_fft_host <- polyFttKer in gpuMulFFT (cuda/main.cu:526-570) */
+midint_status midint_gpu_add6_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                   uint64_t* h_r, uint32_t runs, double* usec);
+midint_status midint_gpu_poly_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                   uint64_t* h_r, uint32_t runs, double* usec);
+midint_status midint_gpu_poly_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                       uint64_t* h_r, uint32_t runs, double* usec);
 /* The same through the chosen prime (midint_mul_ntt_prime): the reference's
  * gpuMulFFT<FftPrime32 / FftPrime64, m> (cuda/main.cu:415-579, selected in
  * cuda/main.cu:765-766), exact. */

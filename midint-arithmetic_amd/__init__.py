@@ -70,6 +70,9 @@ _SIGS = {
     "midint_mul_ntt_prime": (_int, [_vp, _u32, _u32, _u32, _vp, _vp, _vp, _vp]),
     "midint_ntt_prepare": (_int, [_vp, _u32]),
     "midint_gpu_add_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_add6_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_poly_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
+    "midint_gpu_poly_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_host": (_int, [_int, _u32, _u32, _int, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_fft_host": (_int, [_int, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_gpu_mul_fft_prime_host": (_int, [_int, _u32, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),

@@ -541,7 +541,7 @@ midint_status midint_mul_ntt_prime(midint_ctx* ctx, uint32_t prime, uint32_t n, 
 }
 
 // ----------------------------------------------------------- host drop-ins
-enum class HostOp { Add, MulLo, MulFull, MulNtt };
+enum class HostOp { Add, MulLo, MulFull, MulNtt, Add6, Poly, PolyNtt };
 
 static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, const uint64_t* h_a,
                               const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec,
@@ -552,7 +552,7 @@ static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, con
     midint_status st = midint_init(&ctx, device);
     if (st != MIDINT_OK) return st;
     const size_t in_bytes = (size_t)n * m * sizeof(uint64_t);
-    const size_t out_limbs = (op == HostOp::Add || op == HostOp::MulLo) ? (size_t)n * m : (size_t)2 * n * m;
+    const size_t out_limbs = (op == HostOp::MulFull || op == HostOp::MulNtt) ? (size_t)2 * n * m : (size_t)n * m;
     uint64_t *d_a = nullptr, *d_b = nullptr, *d_r = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipError_t he = hipMalloc((void**)&d_a, in_bytes);
@@ -570,6 +570,9 @@ static midint_status run_host(int device, HostOp op, uint32_t n, uint32_t m, con
             case HostOp::MulLo: return midint_mul_lo(ctx, n, m, d_a, d_b, d_r, nullptr);
             case HostOp::MulFull: return midint_mul_full(ctx, n, m, d_a, d_b, d_r, nullptr);
             case HostOp::MulNtt: return midint_mul_ntt_prime(ctx, prime, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::Add6: return midint_add6(ctx, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::Poly: return midint_poly(ctx, n, m, d_a, d_b, d_r, nullptr);
+            case HostOp::PolyNtt: return midint_poly_ntt(ctx, n, m, d_a, d_b, d_r, nullptr);
         }
         return MIDINT_EINVAL;
     };
@@ -607,6 +610,21 @@ midint_status midint_gpu_mul_host(int device, uint32_t n, uint32_t m, int full, 
 midint_status midint_gpu_mul_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a,
                                       const uint64_t* h_b, uint64_t* h_r, uint32_t runs, double* usec) {
     return run_host(device, HostOp::MulNtt, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_add6_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                   uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, HostOp::Add6, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_poly_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                   uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, HostOp::Poly, n, m, h_a, h_b, h_r, runs, usec);
+}
+
+midint_status midint_gpu_poly_fft_host(int device, uint32_t n, uint32_t m, const uint64_t* h_a, const uint64_t* h_b,
+                                       uint64_t* h_r, uint32_t runs, double* usec) {
+    return run_host(device, HostOp::PolyNtt, n, m, h_a, h_b, h_r, runs, usec);
 }
 
 midint_status midint_gpu_mul_fft_prime_host(int device, uint32_t prime, uint32_t n, uint32_t m, const uint64_t* h_a,

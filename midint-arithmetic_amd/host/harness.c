@@ -7,6 +7,12 @@
  *   runFFTMuls    -> run_fft_muls    (cuda/main.cu:711-746), validated here;
  *                    Goldilocks, then FftPrime32 as cuda/main.cu:766 runs it
  *
+ * After each base operation the composite its reference driver times comes
+ * too, validated here as well: six additions after each addition
+ * (a6pb10Ker, cuda/main.cu:121-157), the polynomial after each classical
+ * product (polyKerQ, :317-351) and after each Goldilocks NTT product
+ * (polyFttKer, :526-570).
+ *
  * Inputs come from the reference's mkRandArrays<32,32> (cuda/main.cu:18-33):
  * at the start of each sweep, ourMkRandom (cuda/helper.h:99-114) fills the
  * operand arrays a, then b, from ONE glibc rand() stream that is never
@@ -67,7 +73,43 @@ static int validate_exact(const uint64_t* want, const uint64_t* got, size_t coun
     return 1;
 }
 
-enum { OP_ADD, OP_MUL_LO, OP_MUL_FFT };
+enum { OP_ADD, OP_MUL_LO, OP_MUL_FFT, OP_ADD6, OP_POLY, OP_POLY_FFT };
+
+/* r = a + b mod B^m, r may alias a or b. */
+static void ref_add(uint64_t* r, const uint64_t* a, const uint64_t* b, uint32_t m) {
+    (void)g_add_n((unsigned long*)r, (const unsigned long*)a, (const unsigned long*)b, (long)m);
+}
+
+/* r = a * b mod B^m through GMP's full product (t: 2m limbs). */
+static void ref_mul_lo(uint64_t* r, const uint64_t* a, const uint64_t* b, uint32_t m, uint64_t* t) {
+    g_mul((unsigned long*)t, (const unsigned long*)a, (long)m, (const unsigned long*)b, (long)m);
+    memcpy(r, t, (size_t)m * 8);
+}
+
+/* The reference's six-add chain (cuda/ker-addition.cu.h:301-324), every
+ * step mod B^m: 6a + 10b. t: 2m limbs. */
+static void ref_add6(uint64_t* r, const uint64_t* a, const uint64_t* b, uint32_t m, uint64_t* t) {
+    uint64_t *t1 = t, *t3 = t + m;
+    ref_add(t1, a, b, m);   /* a + b      */
+    ref_add(t1, t1, t1, m); /* 2a + 2b    */
+    ref_add(t3, t1, b, m);  /* 2a + 3b    */
+    ref_add(t1, t3, t3, m); /* 4a + 6b    */
+    ref_add(t1, t1, t3, m); /* 6a + 9b    */
+    ref_add(r, t1, b, m);   /* 6a + 10b   */
+}
+
+/* The reference's polynomial (cuda/ker-classic-mul.cu.h:533-559), every step
+ * mod B^m: (a^2 + b)(b^2 + b) + ab. t: 5m limbs. */
+static void ref_poly(uint64_t* r, const uint64_t* a, const uint64_t* b, uint32_t m, uint64_t* t) {
+    uint64_t *s1 = t, *s2 = t + m, *full = t + 2 * m, *ab = t + 4 * m;
+    ref_mul_lo(s1, a, a, m, full);
+    ref_add(s1, s1, b, m);
+    ref_mul_lo(s2, b, b, m, full);
+    ref_add(s2, s2, b, m);
+    ref_mul_lo(ab, a, b, m, full);
+    ref_mul_lo(r, s1, s2, m, full);
+    ref_add(r, r, ab, m);
+}
 
 /* The reference's ourMkRandom<32,32> over `words` u32 words (every word
  * random): two rand() calls per word, continuing the process's glibc stream. */
@@ -108,7 +150,7 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, cons
     const uint32_t m = m_u32 / 2;                    /* u64 limbs */
     const uint32_t n = (uint32_t)(total_work / m_u32);
     const size_t in_limbs = (size_t)n * m;
-    const size_t out_limbs = (op == OP_MUL_FFT) ? 2 * in_limbs : in_limbs;
+    const size_t out_limbs = (op == OP_MUL_FFT) ? 2 * in_limbs : in_limbs;  /* composites: m limbs */
     const uint64_t* a = (const uint64_t*)wa; /* a limb is two little-endian u32 words */
     const uint64_t* b = (const uint64_t*)wb;
     uint64_t* r = malloc(out_limbs * 8);
@@ -121,11 +163,18 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, cons
 
     double usec = 0.0;
     midint_status st;
-    const uint32_t runs = (op == OP_ADD) ? 50 : 25; /* GPU_RUNS_ADD / GPU_RUNS_MUL (cuda/main.cu:11-12) */
+    /* GPU_RUNS_ADD / GPU_RUNS_MUL (cuda/main.cu:11-12) */
+    const uint32_t runs = (op == OP_ADD || op == OP_ADD6) ? 50 : 25;
     if (op == OP_ADD)
         st = midint_gpu_add_host(device, n, m, a, b, r, runs, &usec);
+    else if (op == OP_ADD6)
+        st = midint_gpu_add6_host(device, n, m, a, b, r, runs, &usec);
     else if (op == OP_MUL_LO)
         st = midint_gpu_mul_host(device, n, m, 0, a, b, r, runs, &usec);
+    else if (op == OP_POLY)
+        st = midint_gpu_poly_host(device, n, m, a, b, r, runs, &usec);
+    else if (op == OP_POLY_FFT)
+        st = midint_gpu_poly_fft_host(device, n, m, a, b, r, runs, &usec);
     else
         st = midint_gpu_mul_fft_prime_host(device, prime, n, m, a, b, r, runs, &usec);
     if (st != MIDINT_OK) {
@@ -134,16 +183,24 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, cons
         return 0;
     }
     const unsigned bits = m_u32 * 32;
-    const char* name = op == OP_ADD      ? "add"
-                       : op == OP_MUL_LO ? "mul_lo"
+    const char* name = op == OP_ADD          ? "add"
+                       : op == OP_ADD6       ? "add6"
+                       : op == OP_MUL_LO     ? "mul_lo"
+                       : op == OP_POLY       ? "poly"
+                       : op == OP_POLY_FFT   ? "poly_ntt"
                        : prime == MIDINT_PRIME_P32 ? "mul_ntt_p32"
                                                    : "mul_ntt";
     double metric;
-    if (op == OP_ADD) {
+    if (op == OP_ADD || op == OP_ADD6) {
         metric = 3.0 * in_limbs * 8 / (usec * 1000.0);
-        printf("Our Addition of %u-bit Big-Numbers (base u64) runs %u instances in: %.0f microsecs, "
+        printf("Our %s of %u-bit Big-Numbers (base u64) runs %u instances in: %.0f microsecs, "
                "GB/sec: %.2f, Mil-Instances/sec: %.2f\n",
-               bits, n, usec, metric, n / usec);
+               op == OP_ADD ? "Addition" : "SIX Additions", bits, n, usec, metric, n / usec);
+    } else if (op == OP_POLY || op == OP_POLY_FFT) {
+        metric = 4.0 * n * ops_of_mul(m) / (usec * 1000.0);
+        printf("Our %s of %u-bits Big-Numbers (in base = 64 bits) runs %u instances in: %.0f microsecs, "
+               "Gu32ops/sec: %.2f, Mil-Instances/sec: %.2f\n",
+               op == OP_POLY ? "Polynomial" : "Poly-FFT Computation", bits, n, usec, metric, n / usec);
     } else {
         metric = (double)n * ops_of_mul(m) / (usec * 1000.0);
         printf("%s Multiplication of %u-bits Big-Numbers (in base = 64 bits) runs %u instances in: %.0f "
@@ -152,26 +209,32 @@ static int run_one(int op, int device, uint64_t total_work, uint32_t m_u32, cons
     }
     int ok = -1;
     if (gmp) {
-        if (op == OP_ADD) {
-            for (uint32_t i = 0; i < n; i++)
-                g_add_n((unsigned long*)(want + (size_t)i * m), (const unsigned long*)(a + (size_t)i * m),
-                        (const unsigned long*)(b + (size_t)i * m), (long)m);
-        } else {
-            uint64_t* full = malloc(2 * (size_t)m * 8);
-            for (uint32_t i = 0; i < n; i++) {
-                g_mul((unsigned long*)full, (const unsigned long*)(a + (size_t)i * m), (long)m,
-                      (const unsigned long*)(b + (size_t)i * m), (long)m);
-                if (op == OP_MUL_LO)
-                    memcpy(want + (size_t)i * m, full, (size_t)m * 8);
-                else
-                    memcpy(want + (size_t)i * 2 * m, full, 2 * (size_t)m * 8);
-            }
-            free(full);
+        uint64_t* t = malloc(5 * (size_t)m * 8);
+        if (!t) {
+            printf("host allocation failed\n");
+            free(r), free(want);
+            return 0;
         }
+        for (uint32_t i = 0; i < n; i++) {
+            const uint64_t* x = a + (size_t)i * m;
+            const uint64_t* y = b + (size_t)i * m;
+            uint64_t* w = want + (size_t)i * (op == OP_MUL_FFT ? 2 * m : m);
+            switch (op) {
+                case OP_ADD: ref_add(w, x, y, m); break;
+                case OP_ADD6: ref_add6(w, x, y, m, t); break;
+                case OP_MUL_LO: ref_mul_lo(w, x, y, m, t); break;
+                case OP_POLY:
+                case OP_POLY_FFT: ref_poly(w, x, y, m, t); break;
+                default:
+                    g_mul((unsigned long*)w, (const unsigned long*)x, (long)m, (const unsigned long*)y, (long)m);
+                    break;
+            }
+        }
+        free(t);
         ok = validate_exact(want, r, out_limbs);
     }
     printf("{\"op\": \"%s\", \"bits\": %u, \"instances\": %u, \"usec\": %.3f, \"%s\": %.3f, \"valid\": %s}\n",
-           name, bits, n, usec, op == OP_ADD ? "GB_per_s" : "Gu32ops_per_s", metric,
+           name, bits, n, usec, (op == OP_ADD || op == OP_ADD6) ? "GB_per_s" : "Gu32ops_per_s", metric,
            ok < 0 ? "null" : ok ? "true" : "false");
     fflush(stdout);
     free(r), free(want);
@@ -214,11 +277,17 @@ int main(int argc, char** argv) {
     uint32_t *a = NULL, *b = NULL;
     /* runAdditions: mkRandArrays<32,32>(total_work/32) -> total_work words each */
     if (!mk_rand_arrays(total_work, total_work, &a, &b)) return 1;
-    for (size_t i = 0; i < nsz; i++) ok &= run_one(OP_ADD, device, total_work, sizes[i], a, b, gmp, 0);
+    for (size_t i = 0; i < nsz; i++) {
+        ok &= run_one(OP_ADD, device, total_work, sizes[i], a, b, gmp, 0);
+        ok &= run_one(OP_ADD6, device, total_work, sizes[i], a, b, gmp, 0);
+    }
     free(a), free(b);
     /* runNaiveMuls: the same, continuing the stream */
     if (!mk_rand_arrays(total_work, total_work, &a, &b)) return 1;
-    for (size_t i = 0; i < nsz; i++) ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], a, b, gmp, 0);
+    for (size_t i = 0; i < nsz; i++) {
+        ok &= run_one(OP_MUL_LO, device, total_work, sizes[i], a, b, gmp, 0);
+        ok &= run_one(OP_POLY, device, total_work, sizes[i], a, b, gmp, 0);
+    }
     free(a), free(b);
     /* runFFTMuls<FftPrime32>: mkRandArrays<32,32>(total_work/16) draws
      * 2*total_work words per operand; the FFT sizes run total_work/16 words.
@@ -228,8 +297,10 @@ int main(int argc, char** argv) {
     /* p32 products take 8-bit coefficients: up to 2048 u32 words (1024 limbs) */
     for (size_t i = 1; i < nsz; i++)
         ok &= run_one(OP_MUL_FFT, device, fft_work, sizes[i], a, b, gmp, MIDINT_PRIME_P32);
-    for (size_t i = 0; i < nsz; i++)
+    for (size_t i = 0; i < nsz; i++) {
         ok &= run_one(OP_MUL_FFT, device, fft_work, sizes[i], a, b, gmp, MIDINT_PRIME_GOLDILOCKS);
+        ok &= run_one(OP_POLY_FFT, device, fft_work, sizes[i], a, b, gmp, 0);
+    }
     free(a), free(b);
     return ok ? 0 : 1;
 }
