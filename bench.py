@@ -372,8 +372,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", choices=("all", "core"), default="all",
-                    help="core: only the three fixed-size workloads (mul, NTT, add), so that a profiler run "
-                         "sees each kernel from one workload")
+                    help="core: only the fixed-size workloads (mul, NTT, add with its ceiling and add6, poly, "
+                         "poly_ntt), so that a profiler run sees each kernel from one workload")
     ap.add_argument("--calibrate", action="store_true",
                     help="also run a 256 MiB int64 negation (2 x 256 MiB of HBM traffic per launch) x5, "
                          "the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -450,8 +450,8 @@ def main():
 
     composites = ragged = serving = None
     rag = make_ragged(mi.gen_limbs) if (args.only == "all" or not args.no_cpu) else None
+    composites = bench_composites(torch, mi, ctx, args, (da, db, n, m), (dan, dbn, nn, mn))
     if args.only == "all":
-        composites = bench_composites(torch, mi, ctx, args, (da, db, n, m), (dan, dbn, nn, mn))
         ragged = bench_ragged(torch, mi, ctx, args, rag)
         serving = bench_serving(mi, ctx)
     del dan, dbn
