@@ -76,4 +76,19 @@ __device__ __forceinline__ void block_add_rows_carry(Rows<R>& t, uint32_t* lds_g
     }
 }
 
+// x <- x + y per segment with carries inside the wave only: for windows
+// where every wave's R rows hold whole segments (no barrier, no exchange).
+template <int R>
+__device__ __forceinline__ void wave_add_rows(Rows<R>& t) {
+    RowState st[R];
+#pragma unroll
+    for (int v = 0; v < R; v++) st[v] = row_build(t.x0[v], t.y0[v], t.x1[v], t.y1[v], t.s0[v], t.s1[v]);
+    uint32_t c = 0;
+#pragma unroll
+    for (int v = 0; v < R; v++) {
+        const uint64_t C = row_chain(st[v], &c);
+        row_apply(st[v], C, &t.x0[v], &t.x1[v]);
+    }
+}
+
 }  // namespace midint

@@ -44,6 +44,11 @@ hipError_t launch_mul(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t*
 bool mul_mfma_supported(uint32_t m, bool full);
 hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                            hipStream_t s, Strides st = {});
+// One product per wave on the 16 x 16 x 64 int8 instruction (mul_wave16.hip),
+// compiled for 64- and 128-limb operands (same contract as launch_mul).
+bool mul_wave16_supported(uint32_t m, bool full);
+hipError_t launch_mul_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                             hipStream_t s, Strides st = {});
 // Ragged multiplications on the matrix cores: one product (n_a >= n_b limbs,
 // full n_a + n_b limb result) per workgroup of 1, 2, 4 or 8 waves (default
 // kMfRgWaves).

@@ -416,8 +416,11 @@ static hipError_t launch_mul_t(const MulLayout& L, uint32_t n, uint32_t m, const
 template <bool FULL>
 static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                                         const Strides& st, hipStream_t s) {
-    if (tune_get_class("mul.mfma", m, 0) != 0 && mul_mfma_supported(m, FULL))
-        return launch_mul_mfma(n, m, a, b, r, FULL, s, st);
+    if (tune_get_class("mul.mfma", m, 0) != 0) {
+        if (tune_get_class("mul.wave16", m, 1) != 0 && mul_wave16_supported(m, FULL))
+            return launch_mul_wave16(n, m, a, b, r, FULL, s, st);
+        if (mul_mfma_supported(m, FULL)) return launch_mul_mfma(n, m, a, b, r, FULL, s, st);
+    }
     if (tune_get_class("mul.digits", m, 0) != 0) {
         DigitLayout D;
         const bool rp = tune_get_class("mul.rpark", m, 1) != 0;

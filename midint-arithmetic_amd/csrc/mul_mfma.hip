@@ -56,16 +56,14 @@
 #include <algorithm>
 
 #include "add_core.hpp"
+#include "mfma_digits.hpp"
 #include "midint_internal.h"
 
 namespace midint {
 
-typedef int mf_v4i __attribute__((ext_vector_type(4)));
 typedef int mf_v16i __attribute__((ext_vector_type(16)));
 
 constexpr int kMfTiles = 2;                          // column tiles (accumulator sets) per wave pass
-constexpr uint64_t kMfHalf = 0x8080808080808080ull;  // 128 in every byte
-constexpr uint32_t kMfBias = 1u << 21;               // B
 
 // Bytes of the a region of ipb instances (a zero block, then per instance
 // P digit blocks and a zero block) and of one instance's br region (P + 3
@@ -94,16 +92,6 @@ __device__ __forceinline__ uint32_t mf_med3(uint32_t x, uint32_t lo, uint32_t hi
     asm("v_med3_u32 %0, %1, %2, %3" : "=v"(r) : "v"(x), "v"(lo), "v"(hi));
     return r;
 }
-
-// 1, 2^8, 2^16, 2^24 in SGPRs the compiler cannot see through (built from
-// a kernel argument), so that c * 2^k + G stays one v_mad_i64_i32 instead
-// of a 64-bit shift and add.
-struct MfPow {
-    uint32_t p0, p8, p16, p24;
-};
-
-// The bias of one G_q: 255 B in each of its four columns.
-constexpr uint64_t kMfFoldBias = (uint64_t)(255u * kMfBias) * 0x01010101ull;
 
 // A finished tile's 16 signed column sums of the lane (the accumulators
 // start at zero) as the words of sum_q G_q 2^(64q), G_q = sum_t (c_{4q+t} +
@@ -260,21 +248,6 @@ __device__ __forceinline__ void mf_pass(const MfLane& L, uint32_t c0, uint32_t w
     for (int k = 0; k < NTL; k++) mf_store_block(gw[k], pw8[k], 32 * (c0 + k * wpi) + L.n, L.kg, N, L.E, L.O);
 }
 
-// x <- x + y per segment with carries inside the wave only: for windows
-// where every wave's R rows hold whole segments (no barrier, no exchange).
-template <int R>
-__device__ __forceinline__ void wave_add_rows(Rows<R>& t) {
-    RowState st[R];
-#pragma unroll
-    for (int v = 0; v < R; v++) st[v] = row_build(t.x0[v], t.y0[v], t.x1[v], t.y1[v], t.s0[v], t.s1[v]);
-    uint32_t c = 0;
-#pragma unroll
-    for (int v = 0; v < R; v++) {
-        const uint64_t C = row_chain(st[v], &c);
-        row_apply(st[v], C, &t.x0[v], &t.x1[v]);
-    }
-}
-
 // Sizes of the products of one workgroup (wave-uniform). a is the block
 // operand, b the Toeplitz operand (for unbalanced products the shorter one:
 // a tile's d range is at most dmax + 1 steps).
@@ -421,11 +394,7 @@ __device__ __forceinline__ void mf_block(const MfGeom& g, uint32_t ninst, const 
         L.Abase = BR + (ybase >> 2);
         L.E = Eb + (size_t)s * g.PLp;
         L.O = Ob + (size_t)s * g.PLp;
-        MfPow pw;
-        pw.p0 = pow8 >> 8;
-        pw.p8 = pow8;
-        pw.p16 = pow8 * pow8;
-        pw.p24 = pw.p16 * pow8;
+        const MfPow pw = mf_pow(pow8);
         for (uint32_t c0 = wi; c0 < g.NT; c0 += kMfTiles * g.wpi) {
             if (c0 + g.wpi < g.NT)
                 mf_pass<2>(L, c0, g.wpi, g.Pa, g.dmax, g.N, pw);

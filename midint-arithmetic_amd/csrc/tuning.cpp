@@ -6,6 +6,8 @@
 //   mul.mfma      1: int8 matrix-core classical kernel (balanced 8-bit digits)
 //   mul.mfma_wpi  waves per instance of that kernel (0: a quarter of the column tiles)
 //   mul.mfma_ipb  instances per workgroup of that kernel (0: 4 / waves per instance)
+//   mul.wave16    1 (default): with mul.mfma, 64- and 128-limb operands run one product
+//                 per wave on the 16 x 16 x 64 instruction (mul_wave16.hip)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)
@@ -43,7 +45,7 @@ namespace {
 std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
-        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
+        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
         {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
     };
     return t;

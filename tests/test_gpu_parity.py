@@ -208,7 +208,8 @@ MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256,
 # at Q = 4, 8, 16 columns per lane group, the first group parked in LDS or
 # in registers, with and without operand prefetch; 28-bit digits with one
 # mad per product; balanced 8-bit digits on the int8 matrix cores (automatic
-# and one wave per instance).
+# and one wave per instance, and the one-product-per-wave 16 x 16 x 64
+# kernel where it is compiled: 64 and 128 limbs).
 MUL_CFGS = [
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 1},
@@ -222,8 +223,9 @@ MUL_CFGS = [
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 1, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 1},
-    {"mul.mfma": 1, "mul.mfma_wpi": 0},
-    {"mul.mfma": 1, "mul.mfma_wpi": 1},
+    {"mul.mfma": 1, "mul.mfma_wpi": 0, "mul.wave16": 0},
+    {"mul.mfma": 1, "mul.mfma_wpi": 1, "mul.wave16": 0},
+    {"mul.mfma": 1, "mul.wave16": 1},
 ]
 
 
@@ -270,7 +272,7 @@ def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, cfg):
 MFMA_PATTERN_SIZES = [1, 3, 4, 5, 64, 100, 128, 129, 256, 777]
 
 
-@pytest.mark.parametrize("wpi", [0, 1, 2])
+@pytest.mark.parametrize("wpi", [0, 1, 2, "w16"])
 @pytest.mark.parametrize("m", MFMA_PATTERN_SIZES)
 def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     """The matrix-core kernel writes operands in balanced base-256 digits
@@ -278,7 +280,7 @@ def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     whole operand, 0x80 and 0xFF bytes sit at the digit range's ends, and
     mixed limbs break chains at random places."""
     torch = torch_dev
-    rng = np.random.default_rng(m * 7 + wpi)
+    rng = np.random.default_rng(m * 7 + (3 if wpi == "w16" else wpi))
     k7f = np.uint64(0x7F7F7F7F7F7F7F7F)
     k80 = np.uint64(0x8080808080808080)
     pool = np.array([k7f, k80, MASK, 0, np.uint64(0x7F7F7F7F7F7F7F80)], dtype=np.uint64)
@@ -293,7 +295,8 @@ def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     a = np.concatenate([r[0] for r in rows]).astype(np.uint64)
     b = np.concatenate([r[1] for r in rows]).astype(np.uint64)
     n = len(rows)
-    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi}):
+    cfg = {"mul.mfma": 1, "mul.wave16": 1} if wpi == "w16" else {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.wave16": 0}
+    with tuned(mi, m, cfg):
         da, db = dev(torch, a), dev(torch, b)
         rf = empty(torch, 2 * n * m)
         rl = empty(torch, n * m)
@@ -321,7 +324,7 @@ def test_mul_mfma_layout_sweep(mi, ctx, torch_dev, orc, seed):
     ipb = int(rng.choice([0, 1, 2, 3, 4, 5, 8]))
     wpi = int(rng.choice([0, 1, 2, 4]))
     a, b = operands(seed * 31 + m, n, m)
-    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb}):
+    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb, "mul.wave16": 0}):
         da, db = dev(torch, a), dev(torch, b)
         rf = empty(torch, 2 * n * m)
         rl = empty(torch, n * m)
@@ -332,6 +335,41 @@ def test_mul_mfma_layout_sweep(mi, ctx, torch_dev, orc, seed):
     want = orc.mul_full(a, b, m)
     assert np.array_equal(got, want), (m, n, ipb, wpi, int(np.flatnonzero(got != want)[0]))
     assert np.array_equal(host(rl), orc.mul_lo(a, b, m)), (m, n, ipb, wpi)
+
+
+@pytest.mark.parametrize("n", [1, 3, 4, 5, 9, 67])
+@pytest.mark.parametrize("m", [64, 128])
+def test_mul_wave16_batches(mi, ctx, torch_dev, orc, m, n):
+    """The one-product-per-wave kernel: batch counts that leave waves of the
+    last workgroup idle, instance strides above the operand and product
+    lengths (odd strides: 8-byte alignment only, the gaps untouched), and the
+    same products from the 32 x 32 x 32 kernel."""
+    torch = torch_dev
+    a, b = operands(m * 1000 + n, n, m)
+    sa, sb = m + 3, m + 1
+    A = np.full(n * sa, 0x3C3C3C3C3C3C3C3C, dtype=np.uint64)
+    B = np.full(n * sb, 0x3C3C3C3C3C3C3C3C, dtype=np.uint64)
+    for i in range(n):
+        A[i * sa:i * sa + m] = a[i * m:(i + 1) * m]
+        B[i * sb:i * sb + m] = b[i * m:(i + 1) * m]
+    gap = np.uint64(0x5A5A5A5A5A5A5A5A)
+    for op, pl, want in (("mul_full", 2 * m, orc.mul_full(a, b, m)), ("mul_lo", m, orc.mul_lo(a, b, m))):
+        sr = pl + 5
+        with tuned(mi, m, {"mul.mfma": 1, "mul.wave16": 1}):
+            dr = dev(torch, np.full(n * sr, gap, dtype=np.uint64))
+            ctx.strided(op, n, m, dev(torch, A), sa, dev(torch, B), sb, dr, sr)
+            dc = empty(torch, n * pl)
+            getattr(ctx, op)(dev(torch, a), dev(torch, b), dc, m)
+            torch.cuda.synchronize()
+        got = host(dr).reshape(n, sr)
+        assert np.array_equal(got[:, :pl].reshape(-1), want), (op, m, n)
+        assert np.all(got[:, pl:] == gap), (op, m, n)
+        assert np.array_equal(host(dc), want), (op, m, n)
+        with tuned(mi, m, {"mul.mfma": 1, "mul.wave16": 0}):
+            do = empty(torch, n * pl)
+            getattr(ctx, op)(dev(torch, a), dev(torch, b), do, m)
+            torch.cuda.synchronize()
+        assert np.array_equal(host(do), want), (op, m, n)
 
 
 def _check_mul(ctx, torch_dev, orc, m):
