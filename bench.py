@@ -86,6 +86,27 @@ def mfma_work(m, wpi=0):
     return 2 * D * D, 2 * 32768 * mfmas
 
 
+def wave16_work(m, full=True, delta=48):
+    """(useful, issued) int8 ops of one m x m-limb product on the
+    one-product-per-wave kernel: one v_mfma_i32_16x16x64_i8 (2 x 16384 ops)
+    per tile of 16 output blocks of 16 digits and per step k that has a
+    nonzero term (a window of 64 digits of a starting at 16(I - 4k) - delta
+    against digits of b, as the kernel's compile-time plan counts them)."""
+    D = 8 * m + 1
+    nblk = m if full else m // 2
+    mfmas = 0
+    for t in range(nblk // 16):
+        for k in range(-8, D // 64 + 9):
+            for n in range(16):
+                I = 16 * t + n
+                L = 16 * I - 64 * k - delta
+                ilo, ihi = max(L, 0), min(L + 63, D - 1)
+                if ilo <= ihi and ilo <= 16 * I + 15 and ihi + D - 1 >= 16 * I:
+                    mfmas += 1
+                    break
+    return 2 * D * D, 2 * 16384 * mfmas
+
+
 def gu32ops_of_mul(m):
     """The reference's numAd32OpsOfMultInst (cuda/helper.h:36-43) for an
     operand of m u64 limbs: 300 n log2 n with n = 2m u32 words."""
@@ -422,15 +443,28 @@ def main():
                      "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "peak_source": "midint_probe_valu on this GPU (issue rate of v_mad_u64_u32)"}
     if on_mfma:
-        useful, issued = mfma_work(m, tuned_value(mi, "mul.mfma_wpi", m, 0))
+        wave16 = m in (64, 128) and tuned_value(mi, "mul.wave16", m, 1) != 0
+        useful, issued = wave16_work(m) if wave16 else mfma_work(m, tuned_value(mi, "mul.mfma_wpi", m, 0))
         tops = n * useful / (ms * 1e-3) / 1e12
         roofline = {"bound": "mfma", "achieved": tops, "peak": INT8_PEAK_TOPS, "unit": "TOP/s",
                     "frac": tops / INT8_PEAK_TOPS, "traffic": measured_traffic("mul_full_16384x128"),
                     "work": f"int8 ops per launch = {n} x 2 x {8 * m + 1}^2 (balanced base-256 digit products "
-                            f"of the {64 * m}-bit operands); MFMA-issued ops incl. tile padding: "
+                            f"of the {64 * m}-bit operands); "
+                            f"{'v_mfma_i32_16x16x64_i8, one product per wave' if wave16 else 'v_mfma_i32_32x32x32_i8'}; "
+                            f"MFMA-issued ops incl. tile padding: "
                             f"{n * issued / (ms * 1e-3) / 1e12:.0f} TOP/s ({issued / useful:.2f}x useful)",
                     "traffic_source": "profiles/traffic.json (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                     "peak_source": "MI355X published dense int8 MFMA rate"}
+        burst, sustained = mi.probe_mfma(0)
+        issued_tops = n * issued / (ms * 1e-3) / 1e12
+        roofline["mfma_ceiling"] = {
+            "burst_TOPs": burst / 1e12, "sustained_TOPs": sustained / 1e12,
+            "issued_TOPs": issued_tops, "issued_frac_of_sustained": issued_tops * 1e12 / sustained,
+            "what": "midint_probe_mfma: dense v_mfma_i32_16x16x64_i8 chains with changing random operands on "
+                    "every SIMD, no memory traffic (burst: short launches after idle; sustained: after 0.2 s "
+                    "of back-to-back load)"}
+        log(f"mfma ceiling (probe): burst {burst / 1e12:.0f} TOP/s, sustained {sustained / 1e12:.0f} TOP/s; "
+            f"the kernel issues {issued_tops:.0f} TOP/s ({issued_tops * 1e12 / sustained:.2f} of sustained)")
     else:
         roofline = valu_roofline
 

@@ -369,6 +369,13 @@ midint_status midint_probe_valu(int device, double* mad64_per_s, double* add32_p
  * (v_mad_u64_u32 with carry-out + v_addc_co_u32 per product) on register
  * operands: the ceiling of the multiplication loop without LDS traffic. */
 midint_status midint_probe_mac(int device, double* pp_per_s);
+/* int8 matrix-core ceiling: dense v_mfma_i32_16x16x64_i8 chains (operands
+ * changing every step) on every SIMD, 2 x 16 x 16 x 64 ops per instruction.
+ * `burst`: best of three short launches after idle; `sustained`: the last
+ * quarter of 400 back-to-back launches (about 0.2 s of load, when the
+ * clocks have settled under the power limit). Synchronous. No reference
+ * counterpart. */
+midint_status midint_probe_mfma(int device, double* burst_ops_per_s, double* sustained_ops_per_s);
 
 /* HBM streaming ceiling of the addition's access pattern: d_r = d_a + d_b
  * per u64 limb, no carries, in the add kernel's launch shape (blocks of 4

@@ -78,6 +78,7 @@ _SIGS = {
     "midint_gpu_mul_fft_prime_host": (_int, [_int, _u32, _u32, _u32, _vp, _vp, _vp, _u32, _dblp]),
     "midint_probe_valu": (_int, [_int, _dblp, _dblp]),
     "midint_probe_mac": (_int, [_int, _dblp]),
+    "midint_probe_mfma": (_int, [_int, _dblp, _dblp]),
     "midint_max_limbs_ragged": (_u32, []),
     "midint_ragged_plan_create": (_int, [_vp, _vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)]),
     "midint_ragged_launch": (_int, [_vp, _vp]),
@@ -499,6 +500,14 @@ def probe_mac(device=0):
     v = ctypes.c_double(0.0)
     check(lib().midint_probe_mac(device, ctypes.byref(v)), "midint_probe_mac")
     return v.value
+
+
+def probe_mfma(device=0):
+    """(burst, sustained) int8 ops per second of dense 16x16x64 MFMA chains."""
+    burst = ctypes.c_double(0.0)
+    sus = ctypes.c_double(0.0)
+    check(lib().midint_probe_mfma(device, ctypes.byref(burst), ctypes.byref(sus)), "midint_probe_mfma")
+    return burst.value, sus.value
 
 
 def gpu_add_host(a, b, m, runs=0, device=0):

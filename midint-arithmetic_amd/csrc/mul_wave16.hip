@@ -235,15 +235,25 @@ __global__ __launch_bounds__(64 * kW16Waves) void mul_wave16_kernel(uint32_t n_i
         mf_v4i acc[G::T];
 #pragma unroll
         for (int t = 0; t < G::T; t++) acc[t] = mf_v4i{0, 0, 0, 0};
+        // B fragments two steps ahead of their MFMAs (a ring of three): the
+        // LDS latency of a fragment hides behind two steps of MFMAs (left
+        // to itself the compiler loads each fragment just before its MFMAs
+        // and waits for it)
+        mf_v4i bq[3];
+        bq[0] = *reinterpret_cast<const mf_v4i*>(pb);
+        bq[1] = *reinterpret_cast<const mf_v4i*>(pb + 64);
 #pragma unroll
         for (int u = P.umin; u <= P.umax; u++) {
-            const mf_v4i bu = *reinterpret_cast<const mf_v4i*>(pb + 64 * (u - P.umin));
+            const int i = u - P.umin;
+            if (u + 2 <= P.umax) bq[(i + 2) % 3] = *reinterpret_cast<const mf_v4i*>(pb + 64 * (i + 2));
+            __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int t = 0; t < G::T; t++) {
                 const int k = 4 * t - u;
                 if (k >= P.kmin[t] && k <= P.kmax[t])
-                    acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[k], bu, acc[t], 0, 0, 0);
+                    acc[t] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[k], bq[i % 3], acc[t], 0, 0, 0);
             }
+            __builtin_amdgcn_sched_barrier(0);
         }
 
         // ---- column sums to words V_w

@@ -95,6 +95,32 @@ __global__ __launch_bounds__(256) void probe_stream_add(uint64_t limbs, const ui
     }
 }
 
+typedef int probe_v4i __attribute__((ext_vector_type(4)));
+
+// int8 matrix-core issue throughput: 8 independent accumulator chains of
+// v_mfma_i32_16x16x64_i8 per wave; the B operand changes every step (one
+// xor, one add) so the operands toggle like real digit data.
+__global__ __launch_bounds__(256) void probe_mfma16(uint32_t iters, uint32_t seed, int32_t* out) {
+    const uint32_t t = threadIdx.x + 977u * blockIdx.x;
+    probe_v4i av = {(int)(seed * 2654435761u ^ t), (int)(seed + 0x9E3779B9u * t), (int)(t * 40503u + seed),
+                    (int)(~seed ^ (t << 7))};
+    probe_v4i bv = {(int)(t * 2246822519u), (int)(seed ^ 0x85EBCA6Bu), (int)(t + 0xC2B2AE35u), (int)(seed * t)};
+    const int kx = (int)(seed * 0x27D4EB2Fu + t);
+    probe_v4i acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) acc[c] = probe_v4i{c, 0, 0, 0};
+    for (uint32_t i = 0; i < iters; i++) {
+#pragma unroll
+        for (int c = 0; c < 8; c++) acc[c] = __builtin_amdgcn_mfma_i32_16x16x64_i8(av, bv, acc[c], 0, 0, 0);
+        bv.x ^= kx;
+        bv.z += kx;
+    }
+    int s = 0;
+#pragma unroll
+    for (int c = 0; c < 8; c++) s ^= acc[c].x ^ acc[c].y ^ acc[c].z ^ acc[c].w;
+    if (s == 0x1234567) out[0] = s;  // keeps the chains live
+}
+
 }  // namespace midint
 
 using namespace midint;
@@ -182,5 +208,51 @@ extern "C" midint_status midint_probe_valu(int device, double* mad64_per_s, doub
     if (e != hipSuccess) return MIDINT_EHIP;
     *mad64_per_s = rates[0];
     *add32_per_s = rates[1];
+    return MIDINT_OK;
+}
+
+extern "C" midint_status midint_probe_mfma(int device, double* burst_ops_per_s, double* sustained_ops_per_s) {
+    if (!burst_ops_per_s || !sustained_ops_per_s) return MIDINT_EINVAL;
+    if (hipSetDevice(device) != hipSuccess) return MIDINT_EHIP;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess)
+        return MIDINT_EHIP;
+    int32_t* d_out = nullptr;
+    if (hipMalloc((void**)&d_out, 64) != hipSuccess) return MIDINT_ENOMEM;
+    hipEvent_t e0, e1;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    // 4 waves per SIMD, about 0.5 ms per launch at the published rate
+    const uint32_t blocks = (uint32_t)cus * 4, threads = 256, iters = 2048;
+    const double ops = (double)blocks * (threads / 64) * iters * 8 * 2.0 * 16 * 16 * 64;
+    double best = 0, sustained = 0;
+    for (int rep = 0; rep < 3; rep++) {  // short runs after idle: the burst rate
+        (void)hipDeviceSynchronize();
+        (void)hipEventRecord(e0, nullptr);
+        probe_mfma16<<<blocks, threads>>>(iters, 4242u + rep, d_out);
+        (void)hipEventRecord(e1, nullptr);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms > 0.f && ops / (ms * 1e-3) > best) best = ops / (ms * 1e-3);
+    }
+    // 400 back-to-back launches (about 0.2 s): the rate under sustained load
+    constexpr int kLaunches = 400, kTail = 100;
+    for (int i = 0; i < kLaunches; i++) {
+        if (i == kLaunches - kTail) (void)hipEventRecord(e0, nullptr);
+        probe_mfma16<<<blocks, threads>>>(iters, 77u + i, d_out);
+    }
+    (void)hipEventRecord(e1, nullptr);
+    (void)hipEventSynchronize(e1);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (ms > 0.f) sustained = ops * kTail / (ms * 1e-3);
+    hipError_t e = hipGetLastError();
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(d_out);
+    if (e != hipSuccess) return MIDINT_EHIP;
+    *burst_ops_per_s = best;
+    *sustained_ops_per_s = sustained;
     return MIDINT_OK;
 }
