@@ -49,6 +49,9 @@ hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint
 bool mul_wave16_supported(uint32_t m, bool full);
 hipError_t launch_mul_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                              hipStream_t s, Strides st = {});
+// polyKerQ with one instance per wave on the same machinery (64 and 128 limbs).
+hipError_t launch_poly_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                              hipStream_t s, Strides st = {});
 // Ragged multiplications on the matrix cores: one product (n_a >= n_b limbs,
 // full n_a + n_b limb result) per workgroup of 1, 2, 4 or 8 waves (default
 // kMfRgWaves).

@@ -474,7 +474,11 @@ bool poly_uses_fused(uint32_t m) { return tune_get_class("poly.fused", m, 1) != 
 // caller-provided scratch of 4*n*m limbs.
 hipError_t launch_poly(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                        uint64_t* scratch, hipStream_t s, Strides st) {
-    if (poly_uses_fused(m)) return launch_poly_fused(n, m, a, b, r, s, st);
+    if (poly_uses_fused(m)) {
+        if (tune_get_class("mul.wave16", m, 1) != 0 && mul_wave16_supported(m, false))
+            return launch_poly_wave16(n, m, a, b, r, s, st);
+        return launch_poly_fused(n, m, a, b, r, s, st);
+    }
     const size_t nm = (size_t)n * m;
     uint64_t* t1 = scratch;
     uint64_t* t2 = scratch + nm;

@@ -41,6 +41,10 @@
 // V_{2L+1} 2^32 < 2^88: its low 64 bits, plus the bits above 64 of W_{L-1},
 // one pass of the row adder over the wave's rows.
 //
+// The same machinery runs the reference's polyKerQ composite (four low-half
+// products and three additions per instance, poly_wave16_kernel) with the
+// intermediates in registers.
+//
 // No barrier: every wave owns its LDS region (digits of a, reversed digits
 // of b, the words V) and one product at a time; LDS operations of a wave
 // execute in order.
@@ -138,92 +142,89 @@ struct W16 {
 };
 
 constexpr uint32_t kW16Waves = 4;  // waves (independent products) per workgroup
+// Workgroups per CU the register allocation must allow: 4 waves per SIMD
+// (at most 128 VGPRs; with fewer waves the SIMDs idle on LDS latency)
+constexpr uint32_t kW16MinBlocks = 4;
 
 }  // namespace
 
+// One wave's product machinery: its LDS region (digits of a, reversed digits
+// of b, the words V) and the lane constants of the fragments.
 template <int M, bool FULL>
-__global__ __launch_bounds__(64 * kW16Waves) void mul_wave16_kernel(uint32_t n_inst, uint32_t pow8,
-                                                                    const uint64_t* __restrict__ a, uint64_t sa,
-                                                                    const uint64_t* __restrict__ b, uint64_t sb,
-                                                                    uint64_t* __restrict__ r, uint64_t sr) {
+struct W16Wave {
     using G = W16<M, FULL>;
-    constexpr W16Plan P = G::P;
-    extern __shared__ __align__(16) uint64_t lds64[];
-    const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint8_t* base = reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * G::WAVE_BYTES;
-    uint8_t* areg = base;
-    uint8_t* breg = base + G::A_BYTES;
-    uint8_t* vreg = base + G::V_OFF;
+    static constexpr W16Plan P = G::P;
+    uint8_t* areg;
+    uint8_t* breg;
+    uint8_t* vreg;
+    const uint8_t* pa;  // A fragment of step k: dwords from pa + 64 (khi - k)
+    const uint8_t* pb;  // B fragment u: 16 bytes at pb + 64 (u - umin)
+    uint8_t* pvw;       // word w = 64 t + 4 n + g of tile t at pvw + 512 t
+    uint32_t lane, sh;
+    MfPow pw;
 
-    // zeros around the digits and before the words (written once: the
-    // products only overwrite the digit bytes and the words)
-    for (uint32_t x = 16 * lane; x < (uint32_t)G::V_OFF; x += 1024)
-        *reinterpret_cast<uint4*>(base + x) = make_uint4(0u, 0u, 0u, 0u);
+    // The region at `base`: zeros around the digits and before the words
+    // (written once: the products only overwrite digit bytes and words).
+    __device__ __forceinline__ void init(uint8_t* base, uint32_t pow8) {
+        lane = threadIdx.x & 63;
+        areg = base;
+        breg = base + G::A_BYTES;
+        vreg = base + G::V_OFF;
+        for (uint32_t x = 16 * lane; x < (uint32_t)G::V_OFF; x += 1024)
+            *reinterpret_cast<uint4*>(base + x) = make_uint4(0u, 0u, 0u, 0u);
+        const uint32_t ln = lane & 15, lg = lane >> 4;
+        const uint32_t yb = (uint32_t)(G::X0 - 64 * P.khi - kW16Delta) - ln + 16 * lg;
+        sh = 8 * (yb & 3u);
+        pa = breg + (yb & ~3u);
+        // AOFF + 16 (n + g) + 64 u - DL = 16 (n + g) + 64 (u - umin)
+        pb = areg + 16 * (ln + lg);
+        pvw = vreg + 8 * (4 * ln + lg);
+        pw = mf_pow(pow8);
+    }
 
-    const uint32_t ln = lane & 15, lg = lane >> 4;
-    // A fragment of step k: dwords from byte (yb & ~3) + 64 (khi - k) of br
-    const uint32_t yb = (uint32_t)(G::X0 - 64 * P.khi - kW16Delta) - ln + 16 * lg;
-    const uint32_t sh = 8 * (yb & 3u);
-    const uint8_t* pa = breg + (yb & ~3u);
-    // B fragment u: 16 bytes at AOFF + 16 (n + g) + 64 u - DL = 16 (n + g) + 64 (u - umin)
-    const uint8_t* pb = areg + 16 * (ln + lg);
-    // word w = 64 t + 4 n + g of tile t
-    uint8_t* pvw = vreg + 8 * (4 * ln + lg);
-    const MfPow pw = mf_pow(pow8);
-
-    const uint64_t stride = (uint64_t)gridDim.x * kW16Waves;
-    for (uint64_t inst = (uint64_t)blockIdx.x * kW16Waves + wv; inst < n_inst; inst += stride) {
-        // ---- digits: x = limbs + 0x8080...80 row by row (the carry between
-        // rows in a scalar), digit bytes x ^ 0x80; a in order, b reversed
-        // with byte-swapped limbs; the carry out is digit 8M
-        {
-            const uint64_t* ga = a + inst * sa;
-            const uint64_t* gb = b + inst * sb;
-            uint64_t xa[G::ROWS_IN][2], xb[G::ROWS_IN][2];
+    // Digits of the operands held as rows (lane: limbs 128 ro + 2 lane, +1;
+    // limbs at or above M are ignored): x = limbs + 0x8080...80 row by row
+    // (the carry between rows in a scalar), digit bytes x ^ 0x80; a in
+    // order, b reversed with byte-swapped limbs; the carry out is digit 8M.
+    __device__ __forceinline__ void stage(const uint64_t (&xa)[G::ROWS_IN][2], const uint64_t (&xb)[G::ROWS_IN][2]) {
+        uint32_t ca = 0, cb = 0;
 #pragma unroll
-            for (int ro = 0; ro < G::ROWS_IN; ro++) {
-                const uint32_t j0 = 128 * ro + 2 * lane;
-                const bool v = j0 < (uint32_t)M;  // M even: both limbs or none
-                xa[ro][0] = v ? ga[j0] : 0ull;
-                xa[ro][1] = v ? ga[j0 + 1] : 0ull;
-                xb[ro][0] = v ? gb[j0] : 0ull;
-                xb[ro][1] = v ? gb[j0 + 1] : 0ull;
+        for (int ro = 0; ro < G::ROWS_IN; ro++) {
+            const uint32_t j0 = 128 * ro + 2 * lane;
+            const bool in = j0 < (uint32_t)M;  // M even: both limbs or none
+            const uint64_t h = in ? kMfHalf : 0ull;
+            uint64_t w[2][2];
+#pragma unroll
+            for (int o = 0; o < 2; o++) {
+                // (limbs at or above M only exist when M is not a multiple of 128)
+                const uint64_t x0 = (M % 128 == 0 || in) ? (o ? xb[ro][0] : xa[ro][0]) : 0ull;
+                const uint64_t x1 = (M % 128 == 0 || in) ? (o ? xb[ro][1] : xa[ro][1]) : 0ull;
+                const RowState st = row_build(x0, h, x1, h, false, false);
+                const uint64_t C = row_chain(st, o ? &cb : &ca);
+                row_apply(st, C, &w[o][0], &w[o][1]);
+                w[o][0] ^= h;
+                w[o][1] ^= h;
             }
-            uint32_t ca = 0, cb = 0;
-#pragma unroll
-            for (int ro = 0; ro < G::ROWS_IN; ro++) {
-                const uint32_t j0 = 128 * ro + 2 * lane;
-                const uint64_t h = j0 < (uint32_t)M ? kMfHalf : 0ull;
-                uint64_t w[2][2];
-#pragma unroll
-                for (int o = 0; o < 2; o++) {
-                    const RowState st = o ? row_build(xb[ro][0], h, xb[ro][1], h, false, false)
-                                          : row_build(xa[ro][0], h, xa[ro][1], h, false, false);
-                    const uint64_t C = row_chain(st, o ? &cb : &ca);
-                    row_apply(st, C, &w[o][0], &w[o][1]);
-                    w[o][0] ^= h;
-                    w[o][1] ^= h;
-                }
-                if (j0 < (uint32_t)M) {
-                    *reinterpret_cast<ulonglong2*>(areg + G::AOFF + 8 * j0) = make_ulonglong2(w[0][0], w[0][1]);
-                    *reinterpret_cast<ulonglong2*>(breg + G::X0 - 8 * j0 - 15) =
-                        make_ulonglong2(__builtin_bswap64(w[1][1]), __builtin_bswap64(w[1][0]));
-                } else if (j0 == (uint32_t)M) {
-                    // the limb after the operand holds its carry (rows end
-                    // inside a row when M is not a multiple of 128)
-                    *reinterpret_cast<uint64_t*>(areg + G::AOFF + 8 * M) = w[0][0];
-                    *reinterpret_cast<uint64_t*>(breg + G::X0 - 8 * M - 7) = __builtin_bswap64(w[1][0]);
-                }
-            }
-            if (M % 128 == 0 && lane == 0) {
-                *reinterpret_cast<uint64_t*>(areg + G::AOFF + 8 * M) = (uint64_t)ca;
-                *reinterpret_cast<uint64_t*>(breg + G::X0 - 8 * M - 7) = __builtin_bswap64((uint64_t)cb);
+            if (in) {
+                *reinterpret_cast<ulonglong2*>(areg + G::AOFF + 8 * j0) = make_ulonglong2(w[0][0], w[0][1]);
+                *reinterpret_cast<ulonglong2*>(breg + G::X0 - 8 * j0 - 15) =
+                    make_ulonglong2(__builtin_bswap64(w[1][1]), __builtin_bswap64(w[1][0]));
+            } else if (j0 == (uint32_t)M) {
+                // the limb after the operand holds its carry (the rows end
+                // inside a row when M is not a multiple of 128)
+                *reinterpret_cast<uint64_t*>(areg + G::AOFF + 8 * M) = w[0][0];
+                *reinterpret_cast<uint64_t*>(breg + G::X0 - 8 * M - 7) = __builtin_bswap64(w[1][0]);
             }
         }
+        if (M % 128 == 0 && lane == 0) {
+            *reinterpret_cast<uint64_t*>(areg + G::AOFF + 8 * M) = (uint64_t)ca;
+            *reinterpret_cast<uint64_t*>(breg + G::X0 - 8 * M - 7) = __builtin_bswap64((uint64_t)cb);
+        }
         asm volatile("" ::: "memory");
+    }
 
-        // ---- matrix-core phase
+    // The matrix-core phase of the staged operands and the words V_w.
+    __device__ __forceinline__ void product() {
         mf_v4i A[P.khi + 1];
 #pragma unroll
         for (int k = 0; k <= P.khi; k++) {
@@ -255,11 +256,11 @@ __global__ __launch_bounds__(64 * kW16Waves) void mul_wave16_kernel(uint32_t n_i
             }
             __builtin_amdgcn_sched_barrier(0);
         }
-
-        // ---- column sums to words V_w
+        // column sums to words V_w (the one more B of column 0 is added in
+        // finish: a lane-dependent constant here stays live across the loop)
 #pragma unroll
         for (int t = 0; t < G::T; t++) {
-            int64_t V = (int64_t)(kMfFoldBias + (t == 0 && lane == 0 ? (uint64_t)kMfBias : 0ull));
+            int64_t V = (int64_t)kMfFoldBias;
             V += (int64_t)acc[t][0] * (int32_t)pw.p0;
             V += (int64_t)acc[t][1] * (int32_t)pw.p8;
             V += (int64_t)acc[t][2] * (int32_t)pw.p16;
@@ -267,15 +268,19 @@ __global__ __launch_bounds__(64 * kW16Waves) void mul_wave16_kernel(uint32_t n_i
             *reinterpret_cast<int64_t*>(pvw + 512 * t) = V;
         }
         asm volatile("" ::: "memory");
+    }
 
-        // ---- limbs: x_L = W_L mod 2^64, y_L = W_{L-1} >> 64, x + y
-        Rows<G::ROWS_OUT> rows;
+    // Limbs of the product from the words: x_L = W_L mod 2^64, y_L = W_{L-1}
+    // >> 64 (W_L = V_{2L} + V_{2L+1} 2^32), x + y, as rows (x0, x1; limbs at
+    // or above PL are not defined).
+    __device__ __forceinline__ void finish(Rows<G::ROWS_OUT>& rows) {
 #pragma unroll
         for (int ro = 0; ro < G::ROWS_OUT; ro++) {
             const uint8_t* pv = vreg + 32 * (64 * ro + lane);  // word 4 (64 ro + lane)
             const ulonglong2 v01 = *reinterpret_cast<const ulonglong2*>(pv - 16);
-            const ulonglong2 v23 = *reinterpret_cast<const ulonglong2*>(pv);
+            ulonglong2 v23 = *reinterpret_cast<const ulonglong2*>(pv);
             const ulonglong2 v45 = *reinterpret_cast<const ulonglong2*>(pv + 16);
+            if (ro == 0 && lane == 0) v23.x += kMfBias;  // V_0: column 0 takes 256 B
             const uint64_t sp = v01.y + (v01.x >> 32);
             const uint64_t s0 = v23.y + (v23.x >> 32);
             const uint64_t s1 = v45.y + (v45.x >> 32);
@@ -287,13 +292,121 @@ __global__ __launch_bounds__(64 * kW16Waves) void mul_wave16_kernel(uint32_t n_i
             rows.s1[ro] = false;
         }
         wave_add_rows<G::ROWS_OUT>(rows);
+    }
+};
+
+// Limbs 128 ro + 2 lane, +1 of an M-limb operand (zeros at or above M).
+template <int M, int ROWS>
+__device__ __forceinline__ void w16_load(const uint64_t* __restrict__ g, uint64_t (&x)[ROWS][2]) {
+    const uint32_t lane = threadIdx.x & 63;
+#pragma unroll
+    for (int ro = 0; ro < ROWS; ro++) {
+        const uint32_t j0 = 128 * ro + 2 * lane;
+        const bool v = j0 < (uint32_t)M;  // M even: both limbs or none
+        x[ro][0] = v ? g[j0] : 0ull;
+        x[ro][1] = v ? g[j0 + 1] : 0ull;
+    }
+}
+
+template <int M, bool FULL>
+__global__ __launch_bounds__(64 * kW16Waves, kW16MinBlocks) void mul_wave16_kernel(uint32_t n_inst, uint32_t pow8,
+                                                                    const uint64_t* __restrict__ a, uint64_t sa,
+                                                                    const uint64_t* __restrict__ b, uint64_t sb,
+                                                                    uint64_t* __restrict__ r, uint64_t sr) {
+    using G = W16<M, FULL>;
+    extern __shared__ __align__(16) uint64_t lds64[];
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    W16Wave<M, FULL> w;
+    w.init(reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * G::WAVE_BYTES, pow8);
+    const uint64_t stride = (uint64_t)gridDim.x * kW16Waves;
+    for (uint64_t inst = (uint64_t)blockIdx.x * kW16Waves + wv; inst < n_inst; inst += stride) {
+        uint64_t xa[G::ROWS_IN][2], xb[G::ROWS_IN][2];
+        w16_load<M, G::ROWS_IN>(a + inst * sa, xa);
+        w16_load<M, G::ROWS_IN>(b + inst * sb, xb);
+        w.stage(xa, xb);
+        w.product();
+        Rows<G::ROWS_OUT> rows;
+        w.finish(rows);
         uint64_t* gr = r + inst * sr;
 #pragma unroll
         for (int ro = 0; ro < G::ROWS_OUT; ro++) {
-            const uint32_t j0 = 128 * ro + 2 * lane;
+            const uint32_t j0 = 128 * ro + 2 * w.lane;
             if (j0 < (uint32_t)G::PL) {
                 gr[j0] = rows.x0[ro];
                 gr[j0 + 1] = rows.x1[ro];
+            }
+        }
+        asm volatile("" ::: "memory");
+    }
+}
+
+// The reference's polyKerQ (cuda/ker-classic-mul.cu.h:500-575), one instance
+// per wave: r = ((a^2 + b)(b^2 + b) + ab) mod B^M, every step mod B^M in the
+// reference's order. The four low-half products run through the wave's LDS
+// region one after another; the operands, the intermediates and the sums
+// stay in registers as rows.
+template <int M>
+__global__ __launch_bounds__(64 * kW16Waves, kW16MinBlocks) void poly_wave16_kernel(uint32_t n_inst, uint32_t pow8,
+                                                                     const uint64_t* __restrict__ a, uint64_t sa,
+                                                                     const uint64_t* __restrict__ b, uint64_t sb,
+                                                                     uint64_t* __restrict__ r, uint64_t sr) {
+    using G = W16<M, false>;
+    static_assert(G::ROWS_IN == G::ROWS_OUT, "low-half products: rows in = rows out");
+    constexpr int R = G::ROWS_IN;
+    extern __shared__ __align__(16) uint64_t lds64[];
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    W16Wave<M, false> w;
+    w.init(reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * G::WAVE_BYTES, pow8);
+    // x + y mod B^M of two row sets (limbs at or above M are ignored by every
+    // consumer, so the carry past limb M - 1 does not matter)
+    auto add_rows = [](const uint64_t (&x)[R][2], const uint64_t (&y)[R][2], uint64_t (&z)[R][2]) {
+        Rows<R> t;
+#pragma unroll
+        for (int ro = 0; ro < R; ro++) {
+            t.x0[ro] = x[ro][0];
+            t.x1[ro] = x[ro][1];
+            t.y0[ro] = y[ro][0];
+            t.y1[ro] = y[ro][1];
+            t.s0[ro] = false;
+            t.s1[ro] = false;
+        }
+        wave_add_rows<R>(t);
+#pragma unroll
+        for (int ro = 0; ro < R; ro++) {
+            z[ro][0] = t.x0[ro];
+            z[ro][1] = t.x1[ro];
+        }
+    };
+    auto mul = [&](const uint64_t (&x)[R][2], const uint64_t (&y)[R][2], uint64_t (&z)[R][2]) {
+        w.stage(x, y);
+        w.product();
+        Rows<R> t;
+        w.finish(t);
+#pragma unroll
+        for (int ro = 0; ro < R; ro++) {
+            z[ro][0] = t.x0[ro];
+            z[ro][1] = t.x1[ro];
+        }
+    };
+    const uint64_t stride = (uint64_t)gridDim.x * kW16Waves;
+    for (uint64_t inst = (uint64_t)blockIdx.x * kW16Waves + wv; inst < n_inst; inst += stride) {
+        uint64_t xa[R][2], xb[R][2], t1[R][2], t2[R][2], t3[R][2];
+        w16_load<M, R>(a + inst * sa, xa);
+        w16_load<M, R>(b + inst * sb, xb);
+        mul(xa, xa, t1);       // a^2
+        add_rows(t1, xb, t1);  // a^2 + b
+        mul(xb, xb, t2);       // b^2
+        add_rows(t2, xb, t2);  // b^2 + b
+        mul(t1, t2, t1);       // (a^2 + b)(b^2 + b)
+        mul(xa, xb, t3);       // a b
+        add_rows(t1, t3, t1);  // + a b
+        uint64_t* gr = r + inst * sr;
+#pragma unroll
+        for (int ro = 0; ro < R; ro++) {
+            const uint32_t j0 = 128 * ro + 2 * w.lane;
+            if (j0 < (uint32_t)M) {
+                gr[j0] = t1[ro][0];
+                gr[j0 + 1] = t1[ro][1];
             }
         }
         asm volatile("" ::: "memory");
@@ -318,6 +431,17 @@ hipError_t launch_w16_t(uint32_t n, const uint64_t* a, const uint64_t* b, uint64
     return hipGetLastError();
 }
 
+template <int M>
+hipError_t launch_poly_w16_t(uint32_t n, const uint64_t* a, const uint64_t* b, uint64_t* r, const Strides& st,
+                             hipStream_t s) {
+    using G = W16<M, false>;
+    const size_t lds = (size_t)kW16Waves * G::WAVE_BYTES;
+    const uint32_t blocks = (uint32_t)(((uint64_t)n + kW16Waves - 1) / kW16Waves);
+    poly_wave16_kernel<M><<<blocks, 64 * kW16Waves, lds, s>>>(n, 256u, a, st.a ? st.a : M, b, st.b ? st.b : M,
+                                                              r, st.r ? st.r : M);
+    return hipGetLastError();
+}
+
 }  // namespace
 
 bool mul_wave16_supported(uint32_t m, bool full) {
@@ -332,6 +456,16 @@ hipError_t launch_mul_wave16(uint32_t n, uint32_t m, const uint64_t* a, const ui
         case 64: return full ? launch_w16_t<64, true>(n, a, b, r, st, s) : launch_w16_t<64, false>(n, a, b, r, st, s);
         case 128:
             return full ? launch_w16_t<128, true>(n, a, b, r, st, s) : launch_w16_t<128, false>(n, a, b, r, st, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_poly_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
+                              hipStream_t s, Strides st) {
+    if (n == 0) return mul_wave16_supported(m, false) ? hipSuccess : hipErrorInvalidValue;
+    switch (m) {
+        case 64: return launch_poly_w16_t<64>(n, a, b, r, st, s);
+        case 128: return launch_poly_w16_t<128>(n, a, b, r, st, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -7,7 +7,8 @@
 //   mul.mfma_wpi  waves per instance of that kernel (0: a quarter of the column tiles)
 //   mul.mfma_ipb  instances per workgroup of that kernel (0: 4 / waves per instance)
 //   mul.wave16    1 (default): with mul.mfma, 64- and 128-limb operands run one product
-//                 per wave on the 16 x 16 x 64 instruction (mul_wave16.hip)
+//                 per wave on the 16 x 16 x 64 instruction (mul_wave16.hip); with
+//                 poly.fused, so does the poly composite (one instance per wave)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)

@@ -391,16 +391,18 @@ def _check_mul(ctx, torch_dev, orc, m):
 POLY_SIZES = [1, 2, 4, 16, 33, 64, 127, 128, 129, 300, 512, 1024, 2048]
 
 
-@pytest.mark.parametrize("fused", [1, 0], ids=["fused", "chain"])
+@pytest.mark.parametrize("fused", [2, 1, 0], ids=["wave16", "fused", "chain"])
 @pytest.mark.parametrize("m", POLY_SIZES)
 def test_poly(mi, ctx, torch_dev, orc, m, fused):
-    """polyKerQ: the one-kernel matrix-core version and the launch chain,
+    """polyKerQ: one instance per wave on the 16 x 16 x 64 kernel (64 and
+    128 limbs), the one-kernel 32 x 32 x 32 version and the launch chain,
     batch counts that leave the last workgroup partial."""
     torch = torch_dev
+    cfg = {"poly.fused": min(fused, 1), "mul.wave16": 1 if fused == 2 else 0}
     for n in ((13, 1) if m < 1024 else (5, 1)):
         a, b = operands(m * 3 + 1 + n, n, m)
         dr = empty(torch, n * m)
-        with tuned(mi, m, {"poly.fused": fused}):
+        with tuned(mi, m, cfg):
             ctx.poly(dev(torch, a), dev(torch, b), dr, m)
             torch.cuda.synchronize()
         got, want = host(dr)[: n * m], orc.poly(a, b, m)
@@ -418,7 +420,7 @@ def test_poly_fused_layouts(mi, ctx, torch_dev, orc, m, wpi, ipb):
     n = 7
     a, b = operands(m * 5 + wpi + 3 * ipb, n, m)
     dr = empty(torch, n * m)
-    with tuned(mi, m, {"poly.fused": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb}):
+    with tuned(mi, m, {"poly.fused": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb, "mul.wave16": 0}):
         ctx.poly(dev(torch, a), dev(torch, b), dr, m)
         torch.cuda.synchronize()
     assert np.array_equal(host(dr), orc.poly(a, b, m)), (m, wpi, ipb)
