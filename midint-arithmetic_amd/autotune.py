@@ -258,11 +258,21 @@ def main(argv=None):
                           "mul.threads": [256]})
     mul_cands += grid_of({"mul.mfma": [0], "mul.digits": [1], "mul.q": [8], "mul.rpark": [0, 1],
                           "mul.prefetch": [0, 1], "mul.threads": [256]})
-    # int8 matrix-core kernel: waves per instance (0 = its own choice)
-    mul_cands += grid_of({"mul.mfma": [1], "mul.mfma_wpi": [0, 1, 2, 4, 8, 16], "mul.digits": [0], "mul.q": [8],
-                          "mul.rpark": [1], "mul.prefetch": [0], "mul.threads": [256]})
+    # int8 matrix-core kernels (every candidate sets every key of the three,
+    # so none inherits a previous candidate's class key): 32 x 32 x 32 tiles
+    # with their waves per instance, 16 x 16 x 64 tiles one product per
+    # workgroup with their waves per product, one product per wave (64 and
+    # 128 limbs; elsewhere it falls through to the 16 x 16 x 64 kernel);
+    # 0 = the kernel's own choice
+    mx = {"mul.mfma": [1], "mul.digits": [0], "mul.q": [8], "mul.rpark": [1], "mul.prefetch": [0],
+          "mul.threads": [256]}
+    mul_cands += grid_of({**mx, "mul.wave16": [0], "mul.m16": [0], "mul.mfma_wpi": [0, 1, 2, 4, 8, 16],
+                          "mul.m16_waves": [0]})
+    mul_cands += grid_of({**mx, "mul.wave16": [0], "mul.m16": [1], "mul.mfma_wpi": [0],
+                          "mul.m16_waves": [0, 1, 2, 4, 8, 16]})
+    mul_cands += grid_of({**mx, "mul.wave16": [1], "mul.m16": [1], "mul.mfma_wpi": [0], "mul.m16_waves": [0]})
     mul_default = {"mul.mfma": 0, "mul.mfma_wpi": 0, "mul.digits": 0, "mul.q": 8, "mul.rpark": 1, "mul.prefetch": 0,
-                   "mul.threads": 256}
+                   "mul.threads": 256, "mul.wave16": 1, "mul.m16": 1, "mul.m16_waves": 0}
     for cls in mul_classes:
         # about 2^30 partial products per launch, never fewer than 512 instances
         n = min(1 << 20, max(512, (1 << 30) // (2 * cls) ** 2))

@@ -49,6 +49,11 @@ hipError_t launch_mul_mfma(uint32_t n, uint32_t m, const uint64_t* a, const uint
 bool mul_wave16_supported(uint32_t m, bool full);
 hipError_t launch_mul_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
                              hipStream_t s, Strides st = {});
+// Any size up to the LDS limit on the 16 x 16 x 64 instruction, one product per
+// workgroup, the tile plan at run time (mul_mfma16.hip).
+bool mul_mfma16_supported(uint32_t m, bool full);
+hipError_t launch_mul_mfma16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r, bool full,
+                             hipStream_t s, Strides st = {});
 // polyKerQ with one instance per wave on the same machinery (64 and 128 limbs).
 hipError_t launch_poly_wave16(uint32_t n, uint32_t m, const uint64_t* a, const uint64_t* b, uint64_t* r,
                               hipStream_t s, Strides st = {});

@@ -419,6 +419,8 @@ static hipError_t launch_mul_full_or_lo(uint32_t n, uint32_t m, const uint64_t* 
     if (tune_get_class("mul.mfma", m, 0) != 0) {
         if (tune_get_class("mul.wave16", m, 1) != 0 && mul_wave16_supported(m, FULL))
             return launch_mul_wave16(n, m, a, b, r, FULL, s, st);
+        if (tune_get_class("mul.m16", m, 1) != 0 && mul_mfma16_supported(m, FULL))
+            return launch_mul_mfma16(n, m, a, b, r, FULL, s, st);
         if (mul_mfma_supported(m, FULL)) return launch_mul_mfma(n, m, a, b, r, FULL, s, st);
     }
     if (tune_get_class("mul.digits", m, 0) != 0) {

@@ -9,6 +9,9 @@
 //   mul.wave16    1 (default): with mul.mfma, 64- and 128-limb operands run one product
 //                 per wave on the 16 x 16 x 64 instruction (mul_wave16.hip); with
 //                 poly.fused, so does the poly composite (one instance per wave)
+//   mul.m16       1 (default): with mul.mfma, other sizes run one product per workgroup on
+//                 the 16 x 16 x 64 instruction (mul_mfma16.hip); 0: 32 x 32 x 32 tiles
+//   mul.m16_waves waves per product of that kernel (0: half the 4-tile passes, at most 16)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)
@@ -46,7 +49,7 @@ namespace {
 std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
-        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
+        {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"mul.m16", 1}, {"mul.m16_waves", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
         {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
     };
     return t;

@@ -208,8 +208,10 @@ MUL_SIZES = [1, 2, 3, 4, 5, 8, 16, 17, 31, 32, 64, 100, 127, 128, 129, 200, 256,
 # at Q = 4, 8, 16 columns per lane group, the first group parked in LDS or
 # in registers, with and without operand prefetch; 28-bit digits with one
 # mad per product; balanced 8-bit digits on the int8 matrix cores (automatic
-# and one wave per instance, and the one-product-per-wave 16 x 16 x 64
-# kernel where it is compiled: 64 and 128 limbs).
+# and one wave per instance; 16 x 16 x 64 tiles with one product per
+# workgroup at 1, 3 or the default number of waves; and the
+# one-product-per-wave 16 x 16 x 64 kernel where it is compiled: 64 and 128
+# limbs).
 MUL_CFGS = [
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 0, "mul.q": 4, "mul.rpark": 0, "mul.prefetch": 1},
@@ -223,8 +225,11 @@ MUL_CFGS = [
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 1, "mul.prefetch": 0},
     {"mul.mfma": 0, "mul.digits": 1, "mul.rpark": 0, "mul.prefetch": 1},
-    {"mul.mfma": 1, "mul.mfma_wpi": 0, "mul.wave16": 0},
-    {"mul.mfma": 1, "mul.mfma_wpi": 1, "mul.wave16": 0},
+    {"mul.mfma": 1, "mul.mfma_wpi": 0, "mul.wave16": 0, "mul.m16": 0},
+    {"mul.mfma": 1, "mul.mfma_wpi": 1, "mul.wave16": 0, "mul.m16": 0},
+    {"mul.mfma": 1, "mul.wave16": 0, "mul.m16": 1, "mul.m16_waves": 0},
+    {"mul.mfma": 1, "mul.wave16": 0, "mul.m16": 1, "mul.m16_waves": 1},
+    {"mul.mfma": 1, "mul.wave16": 0, "mul.m16": 1, "mul.m16_waves": 3},
     {"mul.mfma": 1, "mul.wave16": 1},
 ]
 
@@ -272,7 +277,7 @@ def test_mul_full_and_lo(mi, ctx, torch_dev, orc, m, cfg):
 MFMA_PATTERN_SIZES = [1, 3, 4, 5, 64, 100, 128, 129, 256, 777]
 
 
-@pytest.mark.parametrize("wpi", [0, 1, 2, "w16"])
+@pytest.mark.parametrize("wpi", [0, 1, 2, "w16", "m16"])
 @pytest.mark.parametrize("m", MFMA_PATTERN_SIZES)
 def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     """The matrix-core kernel writes operands in balanced base-256 digits
@@ -280,7 +285,7 @@ def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     whole operand, 0x80 and 0xFF bytes sit at the digit range's ends, and
     mixed limbs break chains at random places."""
     torch = torch_dev
-    rng = np.random.default_rng(m * 7 + (3 if wpi == "w16" else wpi))
+    rng = np.random.default_rng(m * 7 + {"w16": 3, "m16": 4}.get(wpi, wpi))
     k7f = np.uint64(0x7F7F7F7F7F7F7F7F)
     k80 = np.uint64(0x8080808080808080)
     pool = np.array([k7f, k80, MASK, 0, np.uint64(0x7F7F7F7F7F7F7F80)], dtype=np.uint64)
@@ -295,7 +300,9 @@ def test_mul_mfma_digit_patterns(mi, ctx, torch_dev, orc, m, wpi):
     a = np.concatenate([r[0] for r in rows]).astype(np.uint64)
     b = np.concatenate([r[1] for r in rows]).astype(np.uint64)
     n = len(rows)
-    cfg = {"mul.mfma": 1, "mul.wave16": 1} if wpi == "w16" else {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.wave16": 0}
+    cfg = ({"mul.mfma": 1, "mul.wave16": 1} if wpi == "w16" else
+           {"mul.mfma": 1, "mul.wave16": 0, "mul.m16": 1} if wpi == "m16" else
+           {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.wave16": 0, "mul.m16": 0})
     with tuned(mi, m, cfg):
         da, db = dev(torch, a), dev(torch, b)
         rf = empty(torch, 2 * n * m)
@@ -324,7 +331,7 @@ def test_mul_mfma_layout_sweep(mi, ctx, torch_dev, orc, seed):
     ipb = int(rng.choice([0, 1, 2, 3, 4, 5, 8]))
     wpi = int(rng.choice([0, 1, 2, 4]))
     a, b = operands(seed * 31 + m, n, m)
-    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb, "mul.wave16": 0}):
+    with tuned(mi, m, {"mul.mfma": 1, "mul.mfma_wpi": wpi, "mul.mfma_ipb": ipb, "mul.wave16": 0, "mul.m16": 0}):
         da, db = dev(torch, a), dev(torch, b)
         rf = empty(torch, 2 * n * m)
         rl = empty(torch, n * m)
@@ -370,6 +377,31 @@ def test_mul_wave16_batches(mi, ctx, torch_dev, orc, m, n):
             getattr(ctx, op)(dev(torch, a), dev(torch, b), do, m)
             torch.cuda.synchronize()
         assert np.array_equal(host(do), want), (op, m, n)
+
+
+@pytest.mark.parametrize("seed", range(24))
+def test_mul_mfma16_sweep(mi, ctx, torch_dev, orc, seed):
+    """16 x 16 x 64 tiles with the tile plan at run time: random sizes
+    (odd, straddling rows and tiles, up to 2048 limbs), batch counts and
+    waves per product, full and low-half products, against the oracle and
+    the 32 x 32 x 32 kernel."""
+    torch = torch_dev
+    rng = np.random.default_rng(0x16F + seed)
+    m = int(rng.choice([int(rng.integers(1, 64)), int(rng.integers(64, 300)), int(rng.integers(300, 2049))]))
+    n = int(rng.integers(1, 7))
+    waves = int(rng.choice([0, 1, 2, 3, 5, 8, 16]))
+    a, b = operands(seed * 37 + m, n, m)
+    want_f, want_l = orc.mul_full(a, b, m), orc.mul_lo(a, b, m)
+    with tuned(mi, m, {"mul.mfma": 1, "mul.wave16": 0, "mul.m16": 1, "mul.m16_waves": waves}):
+        da, db = dev(torch, a), dev(torch, b)
+        rf = empty(torch, 2 * n * m)
+        rl = empty(torch, n * m)
+        ctx.mul_full(da, db, rf, m)
+        ctx.mul_lo(da, db, rl, m)
+        torch.cuda.synchronize()
+    got = host(rf)
+    assert np.array_equal(got, want_f), (m, n, waves, int(np.flatnonzero(got != want_f)[0]))
+    assert np.array_equal(host(rl), want_l), (m, n, waves)
 
 
 def _check_mul(ctx, torch_dev, orc, m):
