@@ -132,28 +132,34 @@ __device__ __forceinline__ void m16_block(const M16Geom& g, const uint64_t* __re
         *reinterpret_cast<uint4*>(lds + x) = make_uint4(0u, 0u, 0u, 0u);
     __syncthreads();
 
-    // ---- digits: wave 0 writes a, wave 1 (wave 0 if alone) b; rows in
-    // order with the carry between rows in a scalar; the carry out of
-    // x = limbs + 0x8080...80 is digit 8n (the limb after the operand)
+    // ---- digits: x = limbs + 0x8080...80 as one block-wide row addition per
+    // operand (wave w holds rows w R .. w R + R - 1, every row's limbs load at
+    // once, the carry crosses the waves through LDS), digit bytes x ^ 0x80;
+    // the carry out is digit 8n: limb n of x (0 + 0 + carry) when it lies in
+    // the rows, else the addition's carry out
     for (uint32_t o = 0; o < 2; o++) {
-        if (wv != (waves > 1 ? o : 0u)) continue;
         const uint32_t n = o ? g.nb : g.na;
         const uint64_t* src = o ? b : a;
-        uint32_t cin = 0;
-        const uint32_t nrow = (n + kRowLimbs - 1) / kRowLimbs;
-        for (uint32_t row = 0; row < nrow; row++) {
-            const uint32_t j0 = row * kRowLimbs + 2 * lane;
-            const bool v0 = j0 < n, v1 = j0 + 1 < n;
-            const uint64_t x0 = v0 ? src[j0] : 0ull, x1 = v1 ? src[j0 + 1] : 0ull;
-            const uint64_t h0 = v0 ? kMfHalf : 0ull, h1 = v1 ? kMfHalf : 0ull;
-            const RowState st = row_build(x0, h0, x1, h1, false, false);
-            const uint64_t C = row_chain(st, &cin);
-            uint64_t w0, w1;
-            row_apply(st, C, &w0, &w1);
-            w0 ^= h0;
-            w1 ^= h1;
-            // limbs j0, j0 + 1 (j0 < n; the limb at n, when inside the row,
-            // is the carry: x = y = 0 there plus the incoming carry)
+        Rows<R> t;
+        uint64_t h0[R], h1[R];
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t j0 = (wv * R + v) * kRowLimbs + 2 * lane;
+            t.x0[v] = j0 < n ? src[j0] : 0ull;
+            t.x1[v] = j0 + 1 < n ? src[j0 + 1] : 0ull;
+            h0[v] = j0 < n ? kMfHalf : 0ull;
+            h1[v] = j0 + 1 < n ? kMfHalf : 0ull;
+            t.y0[v] = h0[v];
+            t.y1[v] = h1[v];
+            t.s0[v] = false;
+            t.s1[v] = false;
+        }
+        uint32_t c = 0;
+        block_add_rows_carry<R>(t, lds_gp, &c);
+#pragma unroll
+        for (int v = 0; v < R; v++) {
+            const uint32_t j0 = (wv * R + v) * kRowLimbs + 2 * lane;
+            const uint64_t w0 = t.x0[v] ^ h0[v], w1 = t.x1[v] ^ h1[v];
             if (o == 0) {
                 if (j0 <= n) *reinterpret_cast<uint64_t*>(areg + g.aoff + 8 * j0) = w0;
                 if (j0 + 1 <= n) *reinterpret_cast<uint64_t*>(areg + g.aoff + 8 * (j0 + 1)) = w1;
@@ -162,11 +168,11 @@ __device__ __forceinline__ void m16_block(const M16Geom& g, const uint64_t* __re
                 if (j0 + 1 <= n) *reinterpret_cast<uint64_t*>(breg + g.x0 - 8 * (j0 + 1) - 7) = __builtin_bswap64(w1);
             }
         }
-        if (n % kRowLimbs == 0 && lane == 0) {  // the rows end at limb n: the carry goes to limb n
+        if (n == waves * R * kRowLimbs && tid == 0) {
             if (o == 0)
-                *reinterpret_cast<uint64_t*>(areg + g.aoff + 8 * n) = (uint64_t)cin;
+                *reinterpret_cast<uint64_t*>(areg + g.aoff + 8 * n) = (uint64_t)c;
             else
-                *reinterpret_cast<uint64_t*>(breg + g.x0 - 8 * n - 7) = __builtin_bswap64((uint64_t)cin);
+                *reinterpret_cast<uint64_t*>(breg + g.x0 - 8 * n - 7) = __builtin_bswap64((uint64_t)c);
         }
     }
     __syncthreads();
@@ -332,6 +338,20 @@ __global__ __launch_bounds__(1024) void mul_mfma16_kernel(uint32_t n_inst, uint3
     m16_block<R>(g, a + inst * sa, b + inst * sb, r + inst * sr, pow8, reinterpret_cast<uint8_t*>(lds64), lds_gp);
 }
 
+// Ragged batches: one product per workgroup (items[blockIdx.x], n_a >= n_b),
+// all waves on its passes. The shorter operand gives the windows and the
+// longer one the Toeplitz fragments: a tile's steps then advance with its
+// columns, so the four tiles of a pass share one short run of steps (for
+// 1024 x 32 limbs 2.15x the useful digit products issued instead of 4.36x).
+template <int R>
+__global__ __launch_bounds__(1024) void mul_mfma16_rg_kernel(const MfRgItem* __restrict__ items, uint32_t pow8) {
+    extern __shared__ __align__(16) uint64_t lds64[];
+    __shared__ uint32_t lds_gp[16];
+    const MfRgItem it = items[blockIdx.x];
+    const M16Geom g = m16_geom(it.n_b, it.n_a, it.n_a + it.n_b);
+    m16_block<R>(g, it.b, it.a, it.r, pow8, reinterpret_cast<uint8_t*>(lds64), lds_gp);
+}
+
 namespace {
 
 struct M16Layout {
@@ -384,6 +404,26 @@ hipError_t launch_mul_mfma16(uint32_t n, uint32_t m, const uint64_t* a, const ui
     if (!m16_layout(m, full, &L)) return hipErrorInvalidValue;
     if (n == 0) return hipSuccess;
     return L.R == 4 ? launch_m16_t<4>(L, n, m, a, b, r, st, s) : launch_m16_t<8>(L, n, m, a, b, r, st, s);
+}
+
+// The geometry's LDS bytes grow with each of n_a, n_b and the product
+// length (the a region with the last tile's windows and n_a, the br region
+// with n_b, the words with the length), so the largest item of a launch
+// sizes it.
+bool mfma16_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves) {
+    if (max_na == 0 || max_nb == 0 || max_na > kMulMaxLimbs || max_nb > max_na || max_pl > max_na + max_nb)
+        return false;
+    if (waves == 0 || waves > 16) return false;
+    uint32_t R = 4;
+    while ((uint64_t)waves * R * kRowLimbs < max_pl) R *= 2;
+    if (R > 8) return false;
+    const M16Geom g = m16_geom(max_nb, max_na, max_pl);  // windows: the shorter operand
+    if (g.bytes > 160 * 1024 - 256) return false;
+    out->lds = g.bytes;
+    out->threads = 64 * waves;
+    out->pow8 = 256u;
+    out->func = R == 4 ? (const void*)mul_mfma16_rg_kernel<4> : (const void*)mul_mfma16_rg_kernel<8>;
+    return true;
 }
 
 }  // namespace midint

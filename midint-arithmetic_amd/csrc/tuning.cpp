@@ -11,7 +11,7 @@
 //                 poly.fused, so does the poly composite (one instance per wave)
 //   mul.m16       1 (default): with mul.mfma, other sizes run one product per workgroup on
 //                 the 16 x 16 x 64 instruction (mul_mfma16.hip); 0: 32 x 32 x 32 tiles
-//   mul.m16_waves waves per product of that kernel (0: half the 4-tile passes, at most 16)
+//   mul.m16_waves waves per product of that kernel (0: from the number of 4-tile passes)
 //   mul.prefetch  1: double-buffer the operand LDS reads
 //   mul.digits    1: 28-bit-digit classical kernel (one mad per product)
 //   mul.threads   threads per block of the word kernel (64..256)
@@ -32,6 +32,9 @@
 //   ragged.mfma_min ragged multiplications whose shorter operand has at least
 //                 this many limbs run on the matrix-core kernel (0: none)
 //   ragged.overlap  1: the persistent kernel runs beside the matrix-core node
+//   ragged.m16    1: the matrix-core nodes of ragged batches and of the serving loop
+//                 use 16 x 16 x 64 tiles ("ragged.m16@C", C the class of the longer
+//                 operand); 0 (default, faster on the 50000-op batch): 32 x 32 x 32
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -50,7 +53,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"mul.m16", 1}, {"mul.m16_waves", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0},
     };
     return t;
 }

@@ -123,20 +123,25 @@ def test_ragged_full_workload(mi, ctx, torch_dev, orc):
                 assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])
 
 
+@pytest.mark.parametrize("m16", [0, 1], ids=["tiles32", "tiles16"])
 @pytest.mark.parametrize("mfma_min", [0, 1, 16])
-def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min):
+def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min, m16):
     """Multiplications whose shorter operand has at least ragged.mfma_min
     limbs run on the matrix-core graph node (one product per workgroup, the
     longer operand as the block operand), the rest on the persistent
     kernel: 0 sends none there, 1 sends every one (1 x 1 and 1 x 1024
-    products included)."""
+    products included). The node on 32 x 32 x 32 tiles and on 16 x 16 x 64
+    tiles (ragged.m16, the shorter operand as the windows)."""
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=400, seed=31, lo_bits=6, hi_bits=16)
     old = mi.tune_get("ragged.mfma_min")
+    old16 = mi.tune_get("ragged.m16")
     mi.tune_set("ragged.mfma_min", mfma_min)
+    mi.tune_set("ragged.m16", m16)
     try:
         got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
     finally:
         mi.tune_set("ragged.mfma_min", old)
+        mi.tune_set("ragged.m16", old16)
     want = ragged_reference(orc, ops, a, b, rl)
     assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
 
