@@ -280,6 +280,42 @@ def bench_composites(torch, mi, ctx, args, d2, d3):
     return out
 
 
+def bench_mul_sweep(torch, mi, ctx, args):
+    """Classical full products at 256 .. 2048 limbs (2^21 limbs per operand
+    batch, random limbs): the tuned kernel of each size and, for comparison,
+    the 32 x 32 x 32 matrix-core kernel on the same operands (results checked
+    equal)."""
+    out = {}
+    rng = np.random.default_rng(0x5EE9)
+    for m in (256, 512, 1024, 2048):
+        n = (1 << 21) // m
+        a = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+        b = rng.integers(0, 2**64, size=n * m, dtype=np.uint64)
+        da, db = to_dev(torch, a), to_dev(torch, b)
+        dr = torch.empty(2 * n * m, dtype=torch.int64, device="cuda:0")
+        kernel = ("16x16x64 one product per workgroup" if tuned_value(mi, "mul.m16", m, 1) != 0
+                  else "32x32x32")
+        steps = max(5, args.steps // 10)
+        ms = timed(torch, lambda: ctx.mul_full(da, db, dr, m), steps, 3)
+        tuned = dr.clone()
+        key = f"mul.m16@{m}"
+        try:
+            old = mi.tune_get(key)
+        except Exception:
+            old = None
+        mi.tune_set(key, 0)
+        ms_old = timed(torch, lambda: ctx.mul_full(da, db, dr, m), steps, 3)
+        same = bool(torch.equal(dr, tuned))
+        mi.tune_set(key, old if old is not None else mi.tune_get("mul.m16"))
+        D = 8 * m + 1
+        out[f"mul_{n}x{m}"] = {"us_per_batch": ms * 1e3, "muls_per_s": n / (ms * 1e-3), "kernel": kernel,
+                               "useful_TOPs": n * 2 * D * D / (ms * 1e-3) / 1e12,
+                               "us_per_batch_32x32x32": ms_old * 1e3, "results_equal": same}
+        log(f"mul_full {n} x {m}: {ms * 1e3:.1f} us ({kernel}); 32x32x32 kernel {ms_old * 1e3:.1f} us; equal {same}")
+        del da, db, dr, tuned
+    return out
+
+
 def bench_ragged(torch, mi, ctx, args, rag):
     """Ragged batch (50000 ops): one persistent hipGraph launch vs plain
     fixed-size launches, one per (op, power-of-two size class) and one per
@@ -485,6 +521,7 @@ def main():
     composites = ragged = serving = None
     rag = make_ragged(mi.gen_limbs) if (args.only == "all" or not args.no_cpu) else None
     composites = bench_composites(torch, mi, ctx, args, (da, db, n, m), (dan, dbn, nn, mn))
+    mul_sweep = bench_mul_sweep(torch, mi, ctx, args)
     if args.only == "all":
         ragged = bench_ragged(torch, mi, ctx, args, rag)
         serving = bench_serving(mi, ctx)
@@ -549,6 +586,7 @@ def main():
                                            "traffic": measured_traffic("mul_ntt_1024x2048")}},
             "mul_valu_equivalent": valu_roofline,
             "composites": composites,
+            "mul_sweep": mul_sweep,
             "ragged_50000": ragged,
             "probe": {"mad_u64_u32_per_s": mad_rate, "valu32_ops_per_s": add_rate},
             "serving": serving,
