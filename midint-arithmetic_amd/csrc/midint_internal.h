@@ -79,10 +79,13 @@ struct MfRgConfig {
     size_t lds;
     uint32_t pow8;
 };
+// 32 x 32 x 32 tiles (mfma32_rg_config) or 16 x 16 x 64 tiles
+// (mfma16_rg_config, mul_mfma16.hip); mfma_rg_config (the serving loop's
+// choice) takes the second when the tuning key ragged.m16 is 1. Ragged
+// plans choose per product (ragged_plan.cpp).
 bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out,
                     uint32_t waves = kMfRgWaves);
-// The same items on the 16 x 16 x 64 instruction (mul_mfma16.hip);
-// mfma_rg_config returns this one when the tuning key ragged.m16 is 1.
+bool mfma32_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves);
 bool mfma16_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves);
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ): the launch
 // chain over 4*n*m limbs of scratch, and the fused one-kernel version

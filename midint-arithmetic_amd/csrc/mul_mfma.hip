@@ -748,10 +748,14 @@ static const void* mfma_rg_func(uint32_t R) {
 }
 
 bool mfma_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves) {
+    if (tune_get_class("ragged.m16", max_na, 0) == 1 && mfma16_rg_config(max_na, max_nb, max_pl, out, waves))
+        return true;
+    return mfma32_rg_config(max_na, max_nb, max_pl, out, waves);
+}
+
+bool mfma32_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgConfig* out, uint32_t waves) {
     if (max_na == 0 || max_nb == 0 || max_na > kMulMaxLimbs || max_nb > max_na || max_pl > max_na + max_nb)
         return false;
-    if (tune_get_class("ragged.m16", max_na, 0) != 0 && mfma16_rg_config(max_na, max_nb, max_pl, out, waves))
-        return true;
     if (waves != 1 && waves != 2 && waves != 4 && waves != 8) return false;
     uint32_t R = 1;
     while (waves * R * kRowLimbs < max_pl) R *= 2;

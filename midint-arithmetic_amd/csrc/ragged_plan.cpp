@@ -2,11 +2,15 @@
 // hipGraph replayed by midint_ragged_launch. The graph is a counter reset
 // node followed by
 //  * the multiplications whose shorter operand has at least ragged.mfma_min
-//    limbs on the matrix cores, one product per workgroup of
-//    ragged.mfma_waves@C waves (1, 2, 4 or 8) for size class C (a power of
-//    two; the class of an op is that of its longer operand, (C/2, C] limbs):
-//    one node per distinct wave count, largest products first, LDS sized for
-//    the node's largest product;
+//    limbs on the matrix cores, one product per workgroup: 32 x 32 x 32
+//    tiles with ragged.mfma_waves@C waves (1, 2, 4 or 8) for size class C (a
+//    power of two; the class of an op is that of its longer operand, (C/2, C]
+//    limbs), or 16 x 16 x 64 tiles with ragged.m16_waves@C waves for the
+//    products ragged.m16 sends there (0, the default: none; 1: all; 2: those
+//    whose shorter operand has at least 1 / ragged.m16_ratio of the longer
+//    one's limbs, where those tiles issue less work); one node per (tiles,
+//    wave count), largest products first, LDS sized for the node's largest
+//    product;
 //  * one persistent kernel per (threads, Q) pair that the size classes of
 //    the other ops select (ragged.threads@C: 256 or 512; ragged.q@C: 4 or 8
 //    columns per lane group), each with its own work counter;
@@ -237,28 +241,52 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     int cus = 0;
     hipError_t e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, p->device);
 
-    // ---- matrix-core nodes, one per wave count. A class whose products do
-    // not fit its wave count's rows (one wave holds at most 1024 output
-    // limbs) runs with the default count.
-    std::vector<MfRgItem> mf_by_waves[4];  // index: log2(waves)
+    // ---- matrix-core nodes, one per (tiles, wave count). A class whose
+    // products do not fit its wave count's rows (one wave holds at most 1024
+    // output limbs) runs with the default count.
+    auto config = [](int k16, uint32_t na, uint32_t nb, uint32_t pl, MfRgConfig* out, uint32_t w) {
+        return k16 ? mfma16_rg_config(na, nb, pl, out, w) : mfma32_rg_config(na, nb, pl, out, w);
+    };
+    std::vector<MfRgItem> mf_by_waves[2][4];  // index: 16 x 16 x 64 tiles, log2(waves)
     for (int c = 0; c < 32; c++) {
         if (mf_by_class[c].empty()) continue;
-        uint32_t na = 0, nb = 0, pl = 0;
+        const int64_t m16_mode = tune_get_class("ragged.m16", 1u << c, 0);
+        const int64_t m16_ratio = std::max<int64_t>(1, tune_get_class("ragged.m16_ratio", 1u << c, 2));
+        std::vector<MfRgItem> part[2];
         for (const MfRgItem& it : mf_by_class[c]) {
-            na = std::max(na, it.n_a);
-            nb = std::max(nb, it.n_b);
-            pl = std::max(pl, it.n_a + it.n_b);
+            const bool k16 = m16_mode == 1 || (m16_mode == 2 && (uint64_t)it.n_b * m16_ratio >= it.n_a);
+            part[k16 ? 1 : 0].push_back(it);
         }
-        const int64_t w = tune_get_class("ragged.mfma_waves", 1u << c, kMfRgWaves);
-        MfRgConfig probe;
-        const uint32_t waves = (w == 1 || w == 2 || w == 4 || w == 8) && mfma_rg_config(na, nb, pl, &probe, (uint32_t)w)
-                                   ? (uint32_t)w
-                                   : kMfRgWaves;
-        const int lw = waves == 1 ? 0 : waves == 2 ? 1 : waves == 8 ? 3 : 2;
-        mf_by_waves[lw].insert(mf_by_waves[lw].end(), mf_by_class[c].begin(), mf_by_class[c].end());
+        for (int k16 = 0; k16 < 2; k16++) {
+            if (part[k16].empty()) continue;
+            uint32_t na = 0, nb = 0, pl = 0;
+            for (const MfRgItem& it : part[k16]) {
+                na = std::max(na, it.n_a);
+                nb = std::max(nb, it.n_b);
+                pl = std::max(pl, it.n_a + it.n_b);
+            }
+            auto pick_waves = [&](int kk) {
+                const int64_t w = kk ? tune_get_class("ragged.m16_waves", 1u << c, kMfRgWaves)
+                                     : tune_get_class("ragged.mfma_waves", 1u << c, kMfRgWaves);
+                MfRgConfig probe;
+                return (w == 1 || w == 2 || w == 4 || w == 8) && config(kk, na, nb, pl, &probe, (uint32_t)w)
+                           ? (uint32_t)w
+                           : kMfRgWaves;
+            };
+            int kk = k16;
+            uint32_t waves = pick_waves(kk);
+            MfRgConfig probe;
+            if (kk && !config(kk, na, nb, pl, &probe, waves)) {  // 16 x 16 x 64 tiles do not fit
+                kk = 0;
+                waves = pick_waves(kk);
+            }
+            const int lw = waves == 1 ? 0 : waves == 2 ? 1 : waves == 8 ? 3 : 2;
+            mf_by_waves[kk][lw].insert(mf_by_waves[kk][lw].end(), part[k16].begin(), part[k16].end());
+        }
     }
-    for (int lw = 3; e == hipSuccess && lw >= 0; lw--) {
-        std::vector<MfRgItem>& v = mf_by_waves[lw];
+    for (int g = 7; e == hipSuccess && g >= 0; g--) {
+        const int k16 = g & 1, lw = g >> 1;
+        std::vector<MfRgItem>& v = mf_by_waves[k16][lw];
         if (v.empty()) continue;
         std::stable_sort(v.begin(), v.end(), [](const MfRgItem& x, const MfRgItem& y) {
             return (uint64_t)x.n_a * x.n_b > (uint64_t)y.n_a * y.n_b;
@@ -271,7 +299,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         }
         MfLaunch l;
         l.waves = 1u << lw;
-        if (!mfma_rg_config(na, nb, pl, &l.cfg, l.waves)) {
+        if (!config(k16, na, nb, pl, &l.cfg, l.waves)) {
             free_plan(p);
             return MIDINT_EUNSUPPORTED;
         }

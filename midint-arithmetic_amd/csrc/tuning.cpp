@@ -32,9 +32,12 @@
 //   ragged.mfma_min ragged multiplications whose shorter operand has at least
 //                 this many limbs run on the matrix-core kernel (0: none)
 //   ragged.overlap  1: the persistent kernel runs beside the matrix-core node
-//   ragged.m16    1: the matrix-core nodes of ragged batches and of the serving loop
-//                 use 16 x 16 x 64 tiles ("ragged.m16@C", C the class of the longer
-//                 operand); 0 (default, faster on the 50000-op batch): 32 x 32 x 32
+//   ragged.m16    which ragged products run on 16 x 16 x 64 tiles, per size class: 0
+//                 (default, fastest on the 50000-op batch) none, 1 all (and the serving
+//                 loop's matrix-core node), 2 those whose shorter operand has at least
+//                 1 / ragged.m16_ratio (default 2) of the longer one's limbs; the rest
+//                 on 32 x 32 x 32 tiles
+//   ragged.m16_waves waves per workgroup of the 16 x 16 x 64 nodes, per size class (default 4)
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -53,7 +56,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"mul.m16", 1}, {"mul.m16_waves", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0}, {"ragged.m16_ratio", 2}, {"ragged.m16_waves", 4},
     };
     return t;
 }

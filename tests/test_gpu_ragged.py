@@ -123,15 +123,17 @@ def test_ragged_full_workload(mi, ctx, torch_dev, orc):
                 assert np.array_equal(res[j, :rlen], want[orr:orr + rlen])
 
 
-@pytest.mark.parametrize("m16", [0, 1], ids=["tiles32", "tiles16"])
+@pytest.mark.parametrize("m16", [0, 1, 2], ids=["tiles32", "tiles16", "balanced16"])
 @pytest.mark.parametrize("mfma_min", [0, 1, 16])
 def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min, m16):
     """Multiplications whose shorter operand has at least ragged.mfma_min
     limbs run on the matrix-core graph node (one product per workgroup, the
     longer operand as the block operand), the rest on the persistent
     kernel: 0 sends none there, 1 sends every one (1 x 1 and 1 x 1024
-    products included). The node on 32 x 32 x 32 tiles and on 16 x 16 x 64
-    tiles (ragged.m16, the shorter operand as the windows)."""
+    products included). The node on 32 x 32 x 32 tiles, on 16 x 16 x 64
+    tiles (ragged.m16 = 1, the shorter operand as the windows), and split
+    between the two (ragged.m16 = 2: products whose shorter operand has at
+    least half the longer one's limbs on 16 x 16 x 64 tiles)."""
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=400, seed=31, lo_bits=6, hi_bits=16)
     old = mi.tune_get("ragged.mfma_min")
     old16 = mi.tune_get("ragged.m16")
@@ -169,7 +171,8 @@ def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
     the persistent kernels run beside the matrix-core nodes or after them,
     and per size class the persistent kernel's threads and columns per lane
     group (Q) and the matrix-core node's waves per workgroup -- or one thread
-    count for every class through the plan's own tuning argument. Two
+    count for every class through the plan's own tuning argument, and which
+    matrix-core products run on 16 x 16 x 64 tiles with how many waves. Two
     replays of the graph, each equal to the oracle."""
     rng = np.random.default_rng(0xA11 + seed)
     lo = int(rng.integers(6, 11))
@@ -181,6 +184,9 @@ def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
         cfg[f"ragged.q@{c}"] = int(rng.choice([4, 8]))
         cfg[f"ragged.threads@{c}"] = int(rng.choice([256, 512]))
         cfg[f"ragged.mfma_waves@{c}"] = int(rng.choice([1, 2, 4, 8]))
+        cfg[f"ragged.m16@{c}"] = int(rng.choice([0, 1, 2]))
+        cfg[f"ragged.m16_ratio@{c}"] = int(rng.choice([1, 2, 4]))
+        cfg[f"ragged.m16_waves@{c}"] = int(rng.choice([1, 2, 4, 8]))
     threads = int(rng.choice([0, 256, 512]))
     saved = _set_keys(mi, cfg)
     try:
@@ -211,9 +217,11 @@ def test_ragged_uniform_class_keys(mi, ctx, torch_dev, orc, q, threads, waves):
         assert np.array_equal(got, want), (q, threads, waves, mf_min, int(np.flatnonzero(got != want)[0]))
 
 
-def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc):
-    """Every multiplication on the matrix-core node: unbalanced and tiny
-    shapes, all-ones and 0x7F-byte operands (the longest digit carry chain)."""
+@pytest.mark.parametrize("m16", [0, 1], ids=["tiles32", "tiles16"])
+def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc, m16):
+    """Every multiplication on the matrix-core node (32 x 32 x 32 or
+    16 x 16 x 64 tiles): unbalanced and tiny shapes, all-ones and
+    0x7F-byte operands (the longest digit carry chain)."""
     K7F = np.uint64(0x7F7F7F7F7F7F7F7F)
     MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
     shapes = [(1, 1), (1, 1024), (1024, 1), (1024, 1024), (3, 5), (1023, 1021), (4, 4), (511, 513), (128, 129),
@@ -232,10 +240,9 @@ def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc):
         ob += nb
         orr += na + nb
     a, b = np.concatenate(a_parts), np.concatenate(b_parts)
-    old = mi.tune_get("ragged.mfma_min")
-    mi.tune_set("ragged.mfma_min", 1)
+    saved = _set_keys(mi, {"ragged.mfma_min": 1, "ragged.m16": m16})
     try:
         got = run_plan(mi, ctx, torch_dev, ops, a, b, orr)
     finally:
-        mi.tune_set("ragged.mfma_min", old)
+        _set_keys(mi, saved)
     assert np.array_equal(got, ragged_reference(orc, ops, a, b, orr))
