@@ -1,0 +1,293 @@
+// Ragged operations one per wave: products whose shorter operand has at most
+// 128 limbs on the int8 matrix cores' 16 x 16 x 64 instruction, and full
+// additions. No reference counterpart: the reference serves mixed sizes with
+// one compile-time specialisation and one launch per size
+// (runAdditions / runNaiveMuls, cuda/main.cu:661-709); its column-sum
+// schoolbook is cuda/ker-classic-mul.cu.h:299-463 and its carry scan
+// cuda/ker-addition.cu.h:141-194.
+//
+// Products (mul_rgw_kernel). The digits, tiling and fold are those of the
+// one-product-per-wave kernel of mul_wave16.hip (balanced base-256 digits,
+// D = 8n + 1 per operand, output block I = 16t + n of tile t, step k with
+// the Toeplitz fragment T_k of the shorter operand b and the 64-digit
+// windows of a starting at 16(n + g + 4u) - DL, u = 4t - k, DL = 48). The
+// sizes are run-time values here and the products lopsided, so:
+//
+//  * all K = floor((8 n_b + 15) / 64) + 1 fragments T_0 .. T_{K-1} stay in
+//    registers (K <= 17 for n_b <= 128; digits of b beyond its length read
+//    as zero, and T_k for k >= K would multiply zeros only); the kernel body
+//    is compiled for each K and the wave branches to its product's K;
+//  * the output is produced one row of 128 limbs (4 tiles, 1024 digits) at a
+//    time. Row R needs the windows of u = 16R - K + 1 .. 16R + 12, i.e. a's
+//    digits [1024R - 64K + 16, 1024R + 1024): row R of a and at most 1072
+//    digits before it. a's digits are staged row by row into a ring of four
+//    1024-byte slots (digit i at byte i mod 4096; every window read is
+//    16-byte aligned, so none wraps), the staging carry (a + 0x80..80) in a
+//    scalar from row to row; the slots start as zeros, which are the digits
+//    below 0;
+//  * row R's 4 K MFMAs take K + 12 window fragments from the ring (tile i at
+//    fragment j uses k = 4i + K - 1 - j), the tiles are folded into the
+//    row's 256 words V, and the row is normalised at once: limb L of the
+//    product is (W_L mod 2^64) + (W_{L-1} >> 64), W_L = V_{2L} + V_{2L+1}
+//    2^32, added by the wave's row adder with the carry from row R - 1 in a
+//    scalar (the last two words of row R - 1 are kept in front of the row's
+//    words). The column bias telescopes as in mul_wave16.hip: 255 B in every
+//    column of the rows computed, one B more in column 0, B 2^(64 P) = 0
+//    modulo the output length for any P >= n_a + n_b limbs.
+//
+// Per wave 7280 bytes of LDS: the ring (4096), b's digits reversed (1120),
+// the words V of one row after a 16-byte prefix (2064). Each wave owns its
+// region and one op at a time, so waves never synchronise.
+//
+// Additions (add_rgw_kernel): one full sum (max(n_a, n_b) + 1 limbs) per
+// wave, rows of 128 limbs with the carry from row to row in a scalar, the
+// next row's loads in flight while a row is added.
+#include "add_core.hpp"
+#include "mfma_digits.hpp"
+#include "midint_internal.h"
+
+namespace midint {
+
+namespace {
+
+constexpr int kRgwDL = 48;               // DL
+constexpr uint32_t kRgwRing = 4096;      // a's digits: four rows of 1024
+constexpr int32_t kRgwX0 = 1087;         // b's digit j at byte X0 - j of its region (X0 = 15 mod 16)
+constexpr uint32_t kRgwBrBytes = 1120;   // covers bytes [X0 - 64 K + 1, X0 + 20) for K <= 17
+constexpr uint32_t kRgwVOff = kRgwRing + kRgwBrBytes + 16;
+constexpr uint32_t kRgwWaveBytes = kRgwVOff + 256 * 8;
+constexpr uint32_t kRgwWaves = 4;  // waves (independent ops) per workgroup
+constexpr int kRgwMaxK = 17;
+static_assert(kRgwX0 % 16 == 15 && kRgwX0 >= 64 * kRgwMaxK - 1, "b region origin");
+static_assert(kRgwBrBytes >= (uint32_t)kRgwX0 + 21 && kRgwBrBytes % 16 == 0, "b region size");
+
+// Limbs 128 ro + 2 lane, +1 of an n-limb operand (zeros at or above n).
+__device__ __forceinline__ void rgw_load_row(const uint64_t* __restrict__ g, uint32_t n, uint32_t ro, uint32_t lane,
+                                             uint64_t& x0, uint64_t& x1) {
+    const uint32_t j0 = 128 * ro + 2 * lane;
+    x0 = j0 < n ? g[j0] : 0ull;
+    x1 = j0 + 1 < n ? g[j0 + 1] : 0ull;
+}
+
+// Digits of one staged row: x = limbs + 0x80..80 (limbs below n only) with
+// the carry *c from the previous row, digit bytes x ^ 0x80; the carry out
+// of limb n - 1 lands in limb n as digit 8n (0 or 1), limbs above stay 0.
+__device__ __forceinline__ void rgw_digits(uint64_t x0, uint64_t x1, uint32_t n, uint32_t ro, uint32_t lane,
+                                           uint32_t* c, uint64_t& w0, uint64_t& w1) {
+    const uint32_t j0 = 128 * ro + 2 * lane;
+    const uint64_t h0 = j0 < n ? kMfHalf : 0ull, h1 = j0 + 1 < n ? kMfHalf : 0ull;
+    const RowState st = row_build(x0, h0, x1, h1, false, false);
+    const uint64_t C = row_chain(st, c);
+    row_apply(st, C, &w0, &w1);
+    w0 ^= h0;
+    w1 ^= h1;
+}
+
+template <int K>
+__device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, uint32_t lane, const MfPow& pw) {
+    uint8_t* ring = base;
+    uint8_t* breg = base + kRgwRing;
+    uint8_t* vreg = base + kRgwVOff;
+    const uint32_t na = it.n_a, nb = it.n_b, pl = na + nb;
+    const uint32_t ln = lane & 15, lg = lane >> 4;
+
+    // ---- zero the ring, b's region and the word prefix
+#pragma unroll
+    for (uint32_t x = 0; x < kRgwRing; x += 1024)
+        *reinterpret_cast<uint4*>(ring + x + 16 * lane) = make_uint4(0u, 0u, 0u, 0u);
+    for (uint32_t x = 16 * lane; x < kRgwBrBytes; x += 1024)
+        *reinterpret_cast<uint4*>(breg + x) = make_uint4(0u, 0u, 0u, 0u);
+    if (lane == 0) *reinterpret_cast<uint4*>(vreg - 16) = make_uint4(0u, 0u, 0u, 0u);
+    asm volatile("" ::: "memory");
+
+    // ---- b's digits reversed: digit j at byte X0 - j (n_b + 1 limbs: one
+    // or two rows)
+    {
+        uint32_t cb = 0;
+        const uint32_t rows_b = (nb + 128) / 128;
+        for (uint32_t ro = 0; ro < rows_b; ro++) {
+            uint64_t x0, x1, w0, w1;
+            rgw_load_row(it.b, nb, ro, lane, x0, x1);
+            rgw_digits(x0, x1, nb, ro, lane, &cb, w0, w1);
+            const uint32_t j0 = 128 * ro + 2 * lane;
+            if (j0 <= nb)
+                *reinterpret_cast<ulonglong2*>(breg + kRgwX0 - 8 * j0 - 15) =
+                    make_ulonglong2(__builtin_bswap64(w1), __builtin_bswap64(w0));
+        }
+    }
+    asm volatile("" ::: "memory");
+
+    // ---- the K Toeplitz fragments of b: lane (r, g) holds the 16 bytes
+    // b_{64k + DL + r - 16g - e}, e < 16, bytes [Y, Y + 16) with
+    // Y = X0 - 64k - DL - r + 16g (the funnel shift depends on the lane only)
+    mf_v4i A[K];
+    {
+        const uint32_t y0 = (uint32_t)(kRgwX0 - 64 * (K - 1) - kRgwDL) - ln + 16 * lg;
+        const uint32_t sh = 8 * (y0 & 3u);
+        const uint8_t* pa = breg + (y0 & ~3u);
+#pragma unroll
+        for (int k = 0; k < K; k++) {
+            const uint32_t* q = reinterpret_cast<const uint32_t*>(pa + 64 * (K - 1 - k));
+            const uint32_t q0 = q[0], q1 = q[1], q2 = q[2], q3 = q[3], q4 = q[4];
+            A[k] = mf_v4i{(int)__builtin_amdgcn_alignbit(q1, q0, sh), (int)__builtin_amdgcn_alignbit(q2, q1, sh),
+                          (int)__builtin_amdgcn_alignbit(q3, q2, sh), (int)__builtin_amdgcn_alignbit(q4, q3, sh)};
+        }
+    }
+
+    // ---- rows of the product
+    const uint32_t rows = (pl + 127) / 128;
+    uint32_t ca = 0, cn = 0;
+    uint64_t nx0, nx1;  // limbs of a's next row, loaded a row ahead
+    rgw_load_row(it.a, na, 0, lane, nx0, nx1);
+    uint8_t* pvw = vreg + 8 * (4 * ln + lg);  // word 64 i + 4 n + g of tile i at pvw + 512 i
+    const uint8_t* pv = vreg + 32 * lane;     // words 4 lane .. 4 lane + 3
+    for (uint32_t R = 0; R < rows; R++) {
+        // stage a's row R into slot R mod 4
+        {
+            uint64_t w0, w1;
+            rgw_digits(nx0, nx1, na, R, lane, &ca, w0, w1);
+            *reinterpret_cast<ulonglong2*>(ring + ((1024 * R + 16 * lane) & (kRgwRing - 1))) =
+                make_ulonglong2(w0, w1);
+        }
+        if (R + 1 < rows) rgw_load_row(it.a, na, R + 1, lane, nx0, nx1);
+        asm volatile("" ::: "memory");
+
+        // 4 K MFMAs over K + 12 window fragments, two fragments ahead
+        constexpr int NJ = K + 12;
+        const int32_t fb = (int32_t)(16 * (ln + lg)) - kRgwDL + 1024 * (int32_t)R - 64 * (K - 1);
+        auto frag = [&](int j) {
+            return *reinterpret_cast<const mf_v4i*>(ring + ((uint32_t)(fb + 64 * j) & (kRgwRing - 1)));
+        };
+        mf_v4i acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[i] = mf_v4i{0, 0, 0, 0};
+        mf_v4i bq[3];
+        bq[0] = frag(0);
+        bq[1] = frag(1);
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            if (j + 2 < NJ) bq[(j + 2) % 3] = frag(j + 2);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int k = 4 * i + K - 1 - j;
+                if (k >= 0 && k < K) acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[k], bq[j % 3], acc[i], 0, 0, 0);
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+
+        // fold the tiles' column sums into the row's words
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int64_t V = (int64_t)kMfFoldBias;
+            V += (int64_t)acc[i][0] * (int32_t)pw.p0;
+            V += (int64_t)acc[i][1] * (int32_t)pw.p8;
+            V += (int64_t)acc[i][2] * (int32_t)pw.p16;
+            V += (int64_t)acc[i][3] * (int32_t)pw.p24;
+            *reinterpret_cast<int64_t*>(pvw + 512 * i) = V;
+        }
+        asm volatile("" ::: "memory");
+
+        // limbs of row R: x_L = W_L mod 2^64, y_L = W_{L-1} >> 64, x + y
+        const ulonglong2 v01 = *reinterpret_cast<const ulonglong2*>(pv - 16);
+        ulonglong2 v23 = *reinterpret_cast<const ulonglong2*>(pv);
+        const ulonglong2 v45 = *reinterpret_cast<const ulonglong2*>(pv + 16);
+        asm volatile("" ::: "memory");
+        if (lane == 63) *reinterpret_cast<ulonglong2*>(vreg - 16) = v45;  // words 254, 255 for row R + 1
+        if (R == 0 && lane == 0) v23.x += kMfBias;                         // V_0: column 0 takes 256 B
+        const uint64_t sp = v01.y + (v01.x >> 32);
+        const uint64_t s0 = v23.y + (v23.x >> 32);
+        const uint64_t s1 = v45.y + (v45.x >> 32);
+        const RowState st = row_build((v23.x & 0xffffffffull) | (s0 << 32), sp >> 32,
+                                      (v45.x & 0xffffffffull) | (s1 << 32), s0 >> 32, false, false);
+        const uint64_t C = row_chain(st, &cn);
+        uint64_t z0, z1;
+        row_apply(st, C, &z0, &z1);
+        const uint32_t j0 = 128 * R + 2 * lane;
+        if (j0 < pl) it.r[j0] = z0;
+        if (j0 + 1 < pl) it.r[j0 + 1] = z1;
+        asm volatile("" ::: "memory");
+    }
+}
+
+}  // namespace
+
+// One product per wave: item blockIdx.x * 4 + wave (n_a >= n_b, n_b <= 128).
+// Operands and results may be only 8-byte aligned (packed ragged buffers):
+// global accesses are limb by limb.
+__global__ __launch_bounds__(64 * kRgwWaves) void mul_rgw_kernel(const MfRgItem* __restrict__ items, uint32_t n_items,
+                                                                  uint32_t pow8) {
+    extern __shared__ __align__(16) uint64_t lds64[];
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t idx = blockIdx.x * kRgwWaves + wv;
+    if (idx >= n_items) return;
+    const MfRgItem it = items[idx];
+    uint8_t* base = reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * kRgwWaveBytes;
+    const MfPow pw = mf_pow(pow8);
+    const uint32_t K = (8 * it.n_b + 15) / 64 + 1;
+    switch (K) {
+#define MIDINT_RGW_CASE(k) \
+    case k: rgw_product<k>(it, base, lane, pw); break;
+        MIDINT_RGW_CASE(1)
+        MIDINT_RGW_CASE(2)
+        MIDINT_RGW_CASE(3)
+        MIDINT_RGW_CASE(4)
+        MIDINT_RGW_CASE(5)
+        MIDINT_RGW_CASE(6)
+        MIDINT_RGW_CASE(7)
+        MIDINT_RGW_CASE(8)
+        MIDINT_RGW_CASE(9)
+        MIDINT_RGW_CASE(10)
+        MIDINT_RGW_CASE(11)
+        MIDINT_RGW_CASE(12)
+        MIDINT_RGW_CASE(13)
+        MIDINT_RGW_CASE(14)
+        MIDINT_RGW_CASE(15)
+        MIDINT_RGW_CASE(16)
+        MIDINT_RGW_CASE(17)
+#undef MIDINT_RGW_CASE
+        default: break;
+    }
+}
+
+// One full sum per wave: r = a + b, max(n_a, n_b) + 1 limbs.
+__global__ __launch_bounds__(64 * kRgwWaves) void add_rgw_kernel(const MfRgItem* __restrict__ items, uint32_t n_items) {
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t idx = blockIdx.x * kRgwWaves + wv;
+    if (idx >= n_items) return;
+    const MfRgItem it = items[idx];
+    const uint32_t na = it.n_a, nb = it.n_b;
+    const uint32_t out = (na > nb ? na : nb) + 1;
+    const uint32_t rows = (out + 127) / 128;
+    uint64_t x0, x1, y0, y1;
+    rgw_load_row(it.a, na, 0, lane, x0, x1);
+    rgw_load_row(it.b, nb, 0, lane, y0, y1);
+    uint32_t c = 0;
+    for (uint32_t R = 0; R < rows; R++) {
+        const RowState st = row_build(x0, y0, x1, y1, false, false);
+        if (R + 1 < rows) {
+            rgw_load_row(it.a, na, R + 1, lane, x0, x1);
+            rgw_load_row(it.b, nb, R + 1, lane, y0, y1);
+        }
+        const uint64_t C = row_chain(st, &c);
+        uint64_t z0, z1;
+        row_apply(st, C, &z0, &z1);
+        const uint32_t j0 = 128 * R + 2 * lane;
+        if (j0 < out) it.r[j0] = z0;
+        if (j0 + 1 < out) it.r[j0 + 1] = z1;
+    }
+}
+
+uint32_t rgw_max_short() { return 128; }
+
+size_t rgw_lds_bytes() { return (size_t)kRgwWaves * kRgwWaveBytes; }
+
+uint32_t rgw_waves() { return kRgwWaves; }
+
+const void* rgw_mul_kernel_ptr() { return (const void*)mul_rgw_kernel; }
+
+const void* rgw_add_kernel_ptr() { return (const void*)add_rgw_kernel; }
+
+}  // namespace midint

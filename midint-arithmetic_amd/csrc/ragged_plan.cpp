@@ -1,6 +1,12 @@
 // Host side of ragged batches: packing ops into work items, upload, and the
-// hipGraph replayed by midint_ragged_launch. The graph is a counter reset
-// node followed by
+// hipGraph replayed by midint_ragged_launch. With ragged.wave = 1 (the
+// default) the graph is
+//  * the additions, one per wave (add_rgw_kernel), and the multiplications
+//    whose shorter operand has at most 128 limbs, one per wave on 16 x 16 x 64
+//    tiles (mul_rgw_kernel), costliest first, each kind one node;
+//  * the other multiplications on the matrix-core nodes below;
+// all as parallel branches. With ragged.wave = 0 it is a counter reset node
+// followed by
 //  * the multiplications whose shorter operand has at least ragged.mfma_min
 //    limbs on the matrix cores, one product per workgroup: 32 x 32 x 32
 //    tiles with ragged.mfma_waves@C waves (1, 2, 4 or 8) for size class C (a
@@ -48,11 +54,21 @@ struct MfLaunch {
     MfRgConfig cfg = {};
 };
 
+// One node of one-op-per-wave items (products with a short operand, or
+// additions).
+struct WvLaunch {
+    bool mul = false;
+    MfRgItem* d_items = nullptr;
+    uint32_t n_items = 0;
+};
+
 struct midint_ragged_plan {
     int device = 0;
     uint32_t* d_counter = nullptr;  // one counter per persistent launch
     std::vector<RgLaunch> rg;
     std::vector<MfLaunch> mf;
+    std::vector<WvLaunch> wv;
+    uint32_t pow8 = 256;
     uint32_t n_counters = 0;
     const uint64_t* A = nullptr;
     const uint64_t* B = nullptr;
@@ -76,6 +92,8 @@ void free_plan(midint_ragged_plan* p) {
         if (l.d_ops) (void)hipFree(l.d_ops);
     }
     for (MfLaunch& l : p->mf)
+        if (l.d_items) (void)hipFree(l.d_items);
+    for (WvLaunch& l : p->wv)
         if (l.d_items) (void)hipFree(l.d_items);
     if (p->d_counter) (void)hipFree(p->d_counter);
     delete p;
@@ -205,15 +223,36 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     // limbs (0: none), grouped by the size class of the longer operand; the
     // other ops go to the persistent launch their class selects
     const int64_t mf_min = tune_get("ragged.mfma_min", 32);
+    // ragged.wave = 1 (default): additions and the products whose shorter
+    // operand has at most rgw_max_short() limbs run one op per wave
+    const bool by_wave = tune_get("ragged.wave", 1) != 0;
     std::vector<std::vector<MfRgItem>> mf_by_class(32);  // index: log2 of the size class
     std::vector<midint_op> rest_by_launch[4];  // index: (threads == 512) * 2 + (q == 8)
+    std::vector<MfRgItem> wv_mul, wv_add;
     for (size_t i = 0; i < n_ops; i++) {
         const midint_op& o = ops[i];
         if (o.reserved != 0 || (o.kind != MIDINT_OP_ADD && o.kind != MIDINT_OP_MUL)) return MIDINT_EINVAL;
         if (o.n_a == 0 || o.n_b == 0) return MIDINT_EINVAL;
         if (o.n_a > kRgMaxLimbs || o.n_b > kRgMaxLimbs) return MIDINT_EUNSUPPORTED;
         const uint32_t nmax = std::max(o.n_a, o.n_b);
-        if (o.kind == MIDINT_OP_MUL && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
+        if (by_wave && o.kind == MIDINT_OP_ADD) {
+            MfRgItem it = {};
+            it.a = d_a + o.off_a;
+            it.b = d_b + o.off_b;
+            it.r = d_r + o.off_r;
+            it.n_a = (uint32_t)o.n_a;
+            it.n_b = (uint32_t)o.n_b;
+            wv_add.push_back(it);
+        } else if (by_wave && o.kind == MIDINT_OP_MUL && std::min(o.n_a, o.n_b) <= rgw_max_short()) {
+            const bool sw = o.n_a < o.n_b;  // the longer operand is the kernel's a
+            MfRgItem it = {};
+            it.a = sw ? d_b + o.off_b : d_a + o.off_a;
+            it.b = sw ? d_a + o.off_a : d_b + o.off_b;
+            it.r = d_r + o.off_r;
+            it.n_a = (uint32_t)(sw ? o.n_b : o.n_a);
+            it.n_b = (uint32_t)(sw ? o.n_a : o.n_b);
+            wv_mul.push_back(it);
+        } else if (o.kind == MIDINT_OP_MUL && (by_wave || (mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min))) {
             const bool sw = o.n_a < o.n_b;  // the longer operand is the kernel's a
             const uint64_t* x = sw ? d_b + o.off_b : d_a + o.off_a;
             const uint64_t* y = sw ? d_a + o.off_a : d_b + o.off_b;
@@ -311,6 +350,27 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         p->mf.push_back(l);
     }
 
+    // ---- one-op-per-wave nodes: the costliest ops first (the workgroups are
+    // dispatched in order as the CUs free up)
+    std::stable_sort(wv_mul.begin(), wv_mul.end(), [](const MfRgItem& x, const MfRgItem& y) {
+        const uint64_t cx = (uint64_t)((x.n_a + x.n_b + 127) / 128) * ((8 * x.n_b + 15) / 64 + 1);
+        const uint64_t cy = (uint64_t)((y.n_a + y.n_b + 127) / 128) * ((8 * y.n_b + 15) / 64 + 1);
+        return cx > cy;
+    });
+    std::stable_sort(wv_add.begin(), wv_add.end(), [](const MfRgItem& x, const MfRgItem& y) {
+        return std::max(x.n_a, x.n_b) > std::max(y.n_a, y.n_b);
+    });
+    for (int k = 0; e == hipSuccess && k < 2; k++) {
+        std::vector<MfRgItem>& v = k == 0 ? wv_mul : wv_add;
+        if (v.empty()) continue;
+        WvLaunch l;
+        l.mul = k == 0;
+        l.n_items = (uint32_t)v.size();
+        e = hipMalloc((void**)&l.d_items, v.size() * sizeof(MfRgItem));
+        if (e == hipSuccess) e = hipMemcpy(l.d_items, v.data(), v.size() * sizeof(MfRgItem), hipMemcpyHostToDevice);
+        p->wv.push_back(l);
+    }
+
     // ---- persistent launches
     for (int k = 0; e == hipSuccess && k < 4; k++) {
         if (rest_by_launch[k].empty()) continue;
@@ -357,7 +417,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     // them)
     e = hipGraphCreate(&p->graph, 0);
     hipGraphNode_t n_reset = nullptr;
-    if (e == hipSuccess) {
+    if (e == hipSuccess && !p->rg.empty()) {
         void* rargs[] = {(void*)&p->d_counter, (void*)&p->n_counters};
         hipKernelNodeParams kp = {};
         kp.func = const_cast<void*>(ragged_reset_ptr());
@@ -380,7 +440,21 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         kp.kernelParams = margs;
         kp.extra = nullptr;
         hipGraphNode_t n = nullptr;
-        e = hipGraphAddKernelNode(&n, p->graph, &n_reset, 1, &kp);
+        e = hipGraphAddKernelNode(&n, p->graph, n_reset ? &n_reset : nullptr, n_reset ? 1 : 0, &kp);
+        mf_nodes.push_back(n);
+    }
+    for (WvLaunch& l : p->wv) {
+        if (e != hipSuccess) break;
+        void* wargs[] = {(void*)&l.d_items, (void*)&l.n_items, (void*)&p->pow8};
+        hipKernelNodeParams kp = {};
+        kp.func = const_cast<void*>(l.mul ? rgw_mul_kernel_ptr() : rgw_add_kernel_ptr());
+        kp.gridDim = dim3((l.n_items + rgw_waves() - 1) / rgw_waves());
+        kp.blockDim = dim3(64 * rgw_waves());
+        kp.sharedMemBytes = l.mul ? (unsigned)rgw_lds_bytes() : 0u;
+        kp.kernelParams = wargs;
+        kp.extra = nullptr;
+        hipGraphNode_t n = nullptr;
+        e = hipGraphAddKernelNode(&n, p->graph, n_reset ? &n_reset : nullptr, n_reset ? 1 : 0, &kp);
         mf_nodes.push_back(n);
     }
     // ragged.overlap = 1: the persistent kernels do not wait for the
@@ -412,7 +486,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
 
 midint_status midint_ragged_launch(midint_ragged_plan* p, void* stream) {
     if (!p) return MIDINT_EINVAL;
-    if (p->rg.empty() && p->mf.empty()) return MIDINT_OK;
+    if (p->rg.empty() && p->mf.empty() && p->wv.empty()) return MIDINT_OK;
     int cur = -1;
     if (hipGetDevice(&cur) != hipSuccess) return MIDINT_EHIP;
     if (cur != p->device && hipSetDevice(p->device) != hipSuccess) return MIDINT_EHIP;
@@ -429,6 +503,7 @@ midint_status midint_ragged_plan_stats(const midint_ragged_plan* p, uint64_t* n_
         lds = std::max<uint64_t>(lds, l.lds);
     }
     for (const MfLaunch& l : p->mf) items += l.n_items;
+    for (const WvLaunch& l : p->wv) items += l.n_items;
     if (n_items) *n_items = items;
     if (grid) *grid = g;
     if (lds_bytes) *lds_bytes = lds;

@@ -38,6 +38,10 @@
 //                 1 / ragged.m16_ratio (default 2) of the longer one's limbs; the rest
 //                 on 32 x 32 x 32 tiles
 //   ragged.m16_waves waves per workgroup of the 16 x 16 x 64 nodes, per size class (default 4)
+//   ragged.wave   1 (default): the additions, and the multiplications whose shorter
+//                 operand has at most 128 limbs, run one op per wave (16 x 16 x 64 tiles
+//                 for the products); the other multiplications on the matrix-core nodes,
+//                 no persistent kernel. 0: the persistent kernels and ragged.mfma_min
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -56,7 +60,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"mul.m16", 1}, {"mul.m16_waves", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0}, {"ragged.m16_ratio", 2}, {"ragged.m16_waves", 4},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0}, {"ragged.m16_ratio", 2}, {"ragged.m16_waves", 4}, {"ragged.wave", 1},
     };
     return t;
 }

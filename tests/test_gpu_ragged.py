@@ -42,10 +42,15 @@ def run_plan(mi, ctx, torch, ops, a, b, rl, threads=0, reps=1):
     return out
 
 
+@pytest.mark.parametrize("wave", [0, 1], ids=["persistent", "by_wave"])
 @pytest.mark.parametrize("threads", [256, 512])
-def test_ragged_small_vs_oracle(mi, ctx, torch_dev, orc, threads):
+def test_ragged_small_vs_oracle(mi, ctx, torch_dev, orc, threads, wave):
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=600, seed=11, lo_bits=6, hi_bits=14)
-    got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, threads=threads, reps=2)
+    saved = _set_keys(mi, {"ragged.wave": wave})
+    try:
+        got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, threads=threads, reps=2)
+    finally:
+        _set_keys(mi, saved)
     want = ragged_reference(orc, ops, a, b, rl)
     assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
 
@@ -133,17 +138,14 @@ def test_ragged_mfma_threshold(mi, ctx, torch_dev, orc, mfma_min, m16):
     products included). The node on 32 x 32 x 32 tiles, on 16 x 16 x 64
     tiles (ragged.m16 = 1, the shorter operand as the windows), and split
     between the two (ragged.m16 = 2: products whose shorter operand has at
-    least half the longer one's limbs on 16 x 16 x 64 tiles)."""
+    least half the longer one's limbs on 16 x 16 x 64 tiles). One op per
+    wave is off here."""
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=400, seed=31, lo_bits=6, hi_bits=16)
-    old = mi.tune_get("ragged.mfma_min")
-    old16 = mi.tune_get("ragged.m16")
-    mi.tune_set("ragged.mfma_min", mfma_min)
-    mi.tune_set("ragged.m16", m16)
+    saved = _set_keys(mi, {"ragged.mfma_min": mfma_min, "ragged.m16": m16, "ragged.wave": 0})
     try:
         got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
     finally:
-        mi.tune_set("ragged.mfma_min", old)
-        mi.tune_set("ragged.m16", old16)
+        _set_keys(mi, saved)
     want = ragged_reference(orc, ops, a, b, rl)
     assert np.array_equal(got, want), int(np.flatnonzero(got != want)[0])
 
@@ -167,7 +169,8 @@ def _set_keys(mi, cfg):
 @pytest.mark.parametrize("seed", range(12))
 def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
     """Random batches (op count, size range, add/mul mix) under random
-    settings of every ragged tuning key: the matrix-core threshold, whether
+    settings of every ragged tuning key: one op per wave for the additions
+    and the products with a short operand or not, the matrix-core threshold, whether
     the persistent kernels run beside the matrix-core nodes or after them,
     and per size class the persistent kernel's threads and columns per lane
     group (Q) and the matrix-core node's waves per workgroup -- or one thread
@@ -179,7 +182,8 @@ def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
     hi = int(rng.integers(lo + 1, 17))
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=int(rng.integers(50, 500)), seed=100 + seed,
                                 mul_frac=float(rng.choice([0.3, 0.7, 1.0])), lo_bits=lo, hi_bits=hi)
-    cfg = {"ragged.mfma_min": int(rng.choice([0, 1, 8, 32, 64])), "ragged.overlap": int(rng.integers(0, 2))}
+    cfg = {"ragged.mfma_min": int(rng.choice([0, 1, 8, 32, 64])), "ragged.overlap": int(rng.integers(0, 2)),
+           "ragged.wave": int(rng.integers(0, 2))}
     for c in RAGGED_CLASSES:
         cfg[f"ragged.q@{c}"] = int(rng.choice([4, 8]))
         cfg[f"ragged.threads@{c}"] = int(rng.choice([256, 512]))
@@ -200,11 +204,12 @@ def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
 @pytest.mark.parametrize("q,threads,waves", [(8, 256, 1), (8, 512, 2), (4, 512, 8), (8, 256, 8)])
 def test_ragged_uniform_class_keys(mi, ctx, torch_dev, orc, q, threads, waves):
     """Every class on one (Q, threads, waves) setting, with the matrix-core
-    threshold off (every product on the persistent kernel) and on."""
+    threshold off (every product on the persistent kernel) and on (one op per
+    wave off)."""
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=800, seed=31, lo_bits=6, hi_bits=16)
     want = ragged_reference(orc, ops, a, b, rl)
     for mf_min in (0, 32):
-        cfg = {"ragged.mfma_min": mf_min}
+        cfg = {"ragged.mfma_min": mf_min, "ragged.wave": 0}
         for c in RAGGED_CLASSES:
             cfg[f"ragged.q@{c}"] = q
             cfg[f"ragged.threads@{c}"] = threads
@@ -240,9 +245,62 @@ def test_ragged_mfma_edge_shapes(mi, ctx, torch_dev, orc, m16):
         ob += nb
         orr += na + nb
     a, b = np.concatenate(a_parts), np.concatenate(b_parts)
-    saved = _set_keys(mi, {"ragged.mfma_min": 1, "ragged.m16": m16})
+    saved = _set_keys(mi, {"ragged.mfma_min": 1, "ragged.m16": m16, "ragged.wave": 0})
     try:
         got = run_plan(mi, ctx, torch_dev, ops, a, b, orr)
     finally:
         _set_keys(mi, saved)
     assert np.array_equal(got, ragged_reference(orc, ops, a, b, orr))
+
+
+def _packed(shapes, kinds, fills, seed):
+    rng = np.random.default_rng(seed)
+    ops, a_parts, b_parts = [], [], []
+    oa = ob = orr = 0
+    for (na, nb), kind, fill in zip(shapes, kinds, fills):
+        x = np.full(na, fill, dtype=np.uint64) if fill is not None else rng.integers(0, 2**64, na, dtype=np.uint64)
+        y = np.full(nb, fill, dtype=np.uint64) if fill is not None else rng.integers(0, 2**64, nb, dtype=np.uint64)
+        ops.append((kind, na, nb, oa, ob, orr))
+        a_parts.append(x)
+        b_parts.append(y)
+        oa += na
+        ob += nb
+        orr += na + nb if kind == 1 else max(na, nb) + 1
+    return ops, np.concatenate(a_parts), np.concatenate(b_parts), orr
+
+
+def test_ragged_wave_every_short_size(mi, ctx, torch_dev, orc):
+    """One product per wave (ragged.wave = 1): every shorter-operand size
+    1..128 limbs (every fragment count K = 1..17), as either operand, against
+    longer operands of odd, even, row-multiple and maximal length (the rows
+    of a staged through the ring, the carry digit in the next row), with
+    all-ones, 0x7F-byte and random limbs; and additions of the same shapes."""
+    K7F = np.uint64(0x7F7F7F7F7F7F7F7F)
+    MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
+    longs = [1, 2, 127, 128, 129, 255, 256, 383, 512, 640, 1000, 1023, 1024]
+    shapes, kinds, fills = [], [], []
+    for ns in range(1, 129):
+        nl = max(ns, longs[ns % len(longs)])
+        shp = (nl, ns) if ns % 2 else (ns, nl)
+        for kind in (1, 1, 0):
+            shapes.append(shp)
+            kinds.append(kind)
+            fills.append((MASK, K7F, None)[len(fills) % 3])
+    for nl in longs:
+        for ns in (1, 64, 127, 128):
+            if ns <= nl:
+                for kind in (1, 0):
+                    shapes.append((nl, ns))
+                    kinds.append(kind)
+                    fills.append((None, MASK)[len(fills) % 2])
+    ops, a, b, rl = _packed(shapes, kinds, fills, 17)
+    saved = _set_keys(mi, {"ragged.wave": 1})
+    try:
+        got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
+    finally:
+        _set_keys(mi, saved)
+    want = ragged_reference(orc, ops, a, b, rl)
+    if not np.array_equal(got, want):
+        bad = int(np.flatnonzero(got != want)[0])
+        op = max(i for i, o in enumerate(ops) if o[5] <= bad)
+        raise AssertionError(f"first mismatch at limb {bad}, op {ops[op][:3]}")
