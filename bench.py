@@ -390,9 +390,11 @@ def bench_ragged(torch, mi, ctx, args, rag):
             "muls_per_s_plain": n_mul / (ms_plain * 1e-3), "plain_launches": launches_c,
             "plain_per_exact_size_ms": ms_exact, "plain_exact_launches": launches_e,
             "speedup_vs_exact": ms_exact / ms_rag, "plan": pstats,
-            "note": "one hipGraph launch (counter reset, matrix-core node and persistent kernel) vs one fixed-size "
-                    "launch per (op, power-of-two size class) and vs one launch per distinct (op, n_a, n_b), each "
-                    "on operands zero-padded to the launch size; eager launches from Python on one stream"}
+            "note": "one hipGraph launch (the additions and the products whose shorter operand has at most 128 "
+                    "limbs one op per wave, the larger products on matrix-core nodes, parallel branches) vs one "
+                    "fixed-size launch per (op, power-of-two size class) and vs one launch per distinct "
+                    "(op, n_a, n_b), each on operands zero-padded to the launch size; eager launches from Python "
+                    "on one stream"}
 
 
 def bench_serving(mi, ctx):

@@ -127,14 +127,18 @@ def tune_op(mi, torch, ctx, op, cls, cands, defaults, n, rng, log, sets=1, key_c
 
 def tune_ragged(mi, torch, ctx, log, classes=(1024, 512, 256, 128, 64, 32, 16, 8, 4)):
     """Ragged batch launch parameters on the 50000-op batch. First the
-    global keys: the matrix-core threshold (shorter operand limbs) and
-    whether the graph branches overlap; then, size class by size class
+    global keys: one op per wave for the additions and the products with a
+    short operand (ragged.wave = 1; their products on one kernel or two,
+    ragged.wave_split), or the persistent kernels with a
+    matrix-core threshold (shorter operand limbs) and whether the graph
+    branches overlap (ragged.wave = 0); then, size class by size class
     (coordinate descent, largest class first), the persistent kernel's
     threads and columns per lane group (ragged.threads@C, ragged.q@C) and
-    the matrix-core node's waves per workgroup (ragged.mfma_waves@C). Every
-    candidate's output must equal the first plan's before it is timed, and a
-    class setting is kept only if the batch gets at least 1 % faster.
-    Returns the tuning-file lines."""
+    the matrix-core node's tiles (ragged.m16@C: 32 x 32 x 32 or
+    16 x 16 x 64) and waves per workgroup (ragged.mfma_waves@C,
+    ragged.m16_waves@C). Every candidate's output must equal the first
+    plan's before it is timed, and a class setting is kept only if the batch
+    gets at least 1 % faster. Returns the tuning-file lines."""
     sys.path.insert(0, os.path.join(os.path.dirname(HERE), "tests"))
     from workloads import make_ragged
 
@@ -161,35 +165,43 @@ def tune_ragged(mi, torch, ctx, log, classes=(1024, 512, 256, 128, 64, 32, 16, 8
         return ms
 
     best, best_ms = {}, None
-    for mf_min in (0, 32, 64, 128):
-        for ov in (0, 1):
-            if mf_min == 0 and ov == 1:
-                continue
-            cand = {"ragged.mfma_min": mf_min, "ragged.overlap": ov}
-            ms = trial(cand)
-            if ms is None:
-                continue
-            log(f"  ragged {cand}: {ms:.3f} ms")
-            if best_ms is None or ms < best_ms:
-                best, best_ms = cand, ms
+    globals_ = [{"ragged.wave": 1, "ragged.wave_split": sp, "ragged.mfma_min": 32, "ragged.overlap": 1}
+                for sp in (0, 1)]
+    globals_ += [{"ragged.wave": 0, "ragged.wave_split": 0, "ragged.mfma_min": mf_min, "ragged.overlap": ov}
+                 for mf_min in (0, 32, 64, 128) for ov in (0, 1) if not (mf_min == 0 and ov == 1)]
+    for cand in globals_:
+        ms = trial(cand)
+        if ms is None:
+            continue
+        log(f"  ragged {cand}: {ms:.3f} ms")
+        if best_ms is None or ms < best_ms:
+            best, best_ms = cand, ms
     for k, v in best.items():
         mi.tune_set(k, v)
     lines = [f"{k} {v}" for k, v in best.items()]
 
     # classes present in the batch, by where their ops go
+    wave = best["ragged.wave"] == 1
     mf_min = best["ragged.mfma_min"]
     has_rg, has_mf = set(), set()
     for kind, na, nb, *_ in ops:
         c = 1
         while c < max(na, nb):
             c *= 2
-        (has_mf if kind == 1 and mf_min > 0 and min(na, nb) >= mf_min else has_rg).add(c)
+        if wave:
+            if kind == 1 and min(na, nb) > 128:
+                has_mf.add(c)
+        elif kind == 1 and mf_min > 0 and min(na, nb) >= mf_min:
+            has_mf.add(c)
+        else:
+            has_rg.add(c)
     for c in classes:
         cands = []
         if c in has_rg:
             cands += [{f"ragged.q@{c}": q, f"ragged.threads@{c}": th} for q in (4, 8) for th in (256, 512)]
         if c in has_mf:
-            cands += [{f"ragged.mfma_waves@{c}": w} for w in (1, 2, 4, 8)]
+            cands += [{f"ragged.m16@{c}": 0, f"ragged.mfma_waves@{c}": w} for w in (1, 2, 4, 8)]
+            cands += [{f"ragged.m16@{c}": 1, f"ragged.m16_waves@{c}": w} for w in (2, 4, 8)]
         for cand in cands:
             keep = {k: _get(mi, k) for k in cand}
             ms = trial(cand)

@@ -90,14 +90,17 @@ bool mfma16_rg_config(uint32_t max_na, uint32_t max_nb, uint32_t max_pl, MfRgCon
 // One ragged op per wave (mul_rgw_kernel, add_rgw_kernel in mul_rgwave.hip):
 // products whose shorter operand (the item's b, n_a >= n_b) has at most
 // rgw_max_short() limbs, on 16 x 16 x 64 tiles, and full additions (a, b in
-// the op's order). Kernel arguments (const MfRgItem* items, uint32_t n_items,
+// the op's order). Product kernels: part 0 any n_b up to rgw_max_short(),
+// part 1 n_b <= rgw_small_max_short() (fewer registers and LDS), part 2 the
+// n_b above that. Kernel arguments (const MfRgItem* items, uint32_t n_items,
 // uint32_t pow8) for products, (items, n_items) for additions; grid
-// ceil(n_items / rgw_waves()) workgroups of 64 * rgw_waves() threads, product
-// workgroups with rgw_lds_bytes() of LDS.
+// ceil(n_items / rgw_waves()) workgroups of 64 * rgw_waves() threads,
+// product workgroups with rgw_lds_bytes(part) of LDS.
 uint32_t rgw_max_short();
-size_t rgw_lds_bytes();
+uint32_t rgw_small_max_short();
+size_t rgw_lds_bytes(int part);
 uint32_t rgw_waves();
-const void* rgw_mul_kernel_ptr();
+const void* rgw_mul_kernel_ptr(int part);
 const void* rgw_add_kernel_ptr();
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m (reference polyKerQ): the launch
 // chain over 4*n*m limbs of scratch, and the fused one-kernel version

@@ -50,16 +50,25 @@ namespace midint {
 
 namespace {
 
-constexpr int kRgwDL = 48;               // DL
-constexpr uint32_t kRgwRing = 4096;      // a's digits: four rows of 1024
-constexpr int32_t kRgwX0 = 1087;         // b's digit j at byte X0 - j of its region (X0 = 15 mod 16)
-constexpr uint32_t kRgwBrBytes = 1120;   // covers bytes [X0 - 64 K + 1, X0 + 20) for K <= 17
-constexpr uint32_t kRgwVOff = kRgwRing + kRgwBrBytes + 16;
-constexpr uint32_t kRgwWaveBytes = kRgwVOff + 256 * 8;
+constexpr int kRgwDL = 48;        // DL
 constexpr uint32_t kRgwWaves = 4;  // waves (independent ops) per workgroup
-constexpr int kRgwMaxK = 17;
-static_assert(kRgwX0 % 16 == 15 && kRgwX0 >= 64 * kRgwMaxK - 1, "b region origin");
-static_assert(kRgwBrBytes >= (uint32_t)kRgwX0 + 21 && kRgwBrBytes % 16 == 0, "b region size");
+constexpr int kRgwSmallK = 9;      // the small-fragment kernel: K <= 9 (n_b <= 70)
+
+// Per-wave LDS layout of the kernel for fragment counts K <= KMAX: the ring
+// of a's digit rows (row R's windows reach back at most 64 KMAX - 16
+// digits: two slots for KMAX <= 9, four up to 17), b's digits reversed
+// (digit j at byte X0 - j, X0 = 15 mod 16, bytes [X0 - 64 KMAX + 1, X0 + 20)
+// read), the words V of one row after a 16-byte prefix.
+template <int KMAX>
+struct RgwCfg {
+    static constexpr uint32_t RING = KMAX <= 9 ? 2048 : 4096;
+    static constexpr int32_t X0 = 64 * KMAX - 1;
+    static constexpr uint32_t BR = ((uint32_t)X0 + 21 + 15) / 16 * 16;
+    static constexpr uint32_t VOFF = RING + BR + 16;
+    static constexpr uint32_t WAVE_BYTES = VOFF + 256 * 8;
+    static_assert(X0 % 16 == 15 && X0 >= 64 * KMAX - 1, "b region origin");
+    static_assert(64 * KMAX - 16 <= (int)RING - 1024, "ring look-back");
+};
 
 // Limbs 128 ro + 2 lane, +1 of an n-limb operand (zeros at or above n).
 __device__ __forceinline__ void rgw_load_row(const uint64_t* __restrict__ g, uint32_t n, uint32_t ro, uint32_t lane,
@@ -83,19 +92,27 @@ __device__ __forceinline__ void rgw_digits(uint64_t x0, uint64_t x1, uint32_t n,
     w1 ^= h1;
 }
 
-template <int K>
+template <int K, int KMAX>
 __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, uint32_t lane, const MfPow& pw) {
+    using G = RgwCfg<KMAX>;
+    constexpr uint32_t kRing = G::RING;
+    constexpr int32_t kX0 = G::X0;
     uint8_t* ring = base;
-    uint8_t* breg = base + kRgwRing;
-    uint8_t* vreg = base + kRgwVOff;
+    uint8_t* breg = base + kRing;
+    uint8_t* vreg = base + G::VOFF;
     const uint32_t na = it.n_a, nb = it.n_b, pl = na + nb;
     const uint32_t ln = lane & 15, lg = lane >> 4;
 
+    // the first rows of both operands in flight before anything else
+    uint64_t bx0, bx1, nx0, nx1;  // b's row 0; a's next row (loaded a row ahead)
+    rgw_load_row(it.b, nb, 0, lane, bx0, bx1);
+    rgw_load_row(it.a, na, 0, lane, nx0, nx1);
+
     // ---- zero the ring, b's region and the word prefix
 #pragma unroll
-    for (uint32_t x = 0; x < kRgwRing; x += 1024)
+    for (uint32_t x = 0; x < kRing; x += 1024)
         *reinterpret_cast<uint4*>(ring + x + 16 * lane) = make_uint4(0u, 0u, 0u, 0u);
-    for (uint32_t x = 16 * lane; x < kRgwBrBytes; x += 1024)
+    for (uint32_t x = 16 * lane; x < G::BR; x += 1024)
         *reinterpret_cast<uint4*>(breg + x) = make_uint4(0u, 0u, 0u, 0u);
     if (lane == 0) *reinterpret_cast<uint4*>(vreg - 16) = make_uint4(0u, 0u, 0u, 0u);
     asm volatile("" ::: "memory");
@@ -104,14 +121,13 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     // or two rows)
     {
         uint32_t cb = 0;
-        const uint32_t rows_b = (nb + 128) / 128;
+        const uint32_t rows_b = (nb + 128) / 128;  // row 1 (n_b = 128) holds the carry digit only
         for (uint32_t ro = 0; ro < rows_b; ro++) {
-            uint64_t x0, x1, w0, w1;
-            rgw_load_row(it.b, nb, ro, lane, x0, x1);
-            rgw_digits(x0, x1, nb, ro, lane, &cb, w0, w1);
+            uint64_t w0, w1;
+            rgw_digits(ro ? 0ull : bx0, ro ? 0ull : bx1, nb, ro, lane, &cb, w0, w1);
             const uint32_t j0 = 128 * ro + 2 * lane;
             if (j0 <= nb)
-                *reinterpret_cast<ulonglong2*>(breg + kRgwX0 - 8 * j0 - 15) =
+                *reinterpret_cast<ulonglong2*>(breg + kX0 - 8 * j0 - 15) =
                     make_ulonglong2(__builtin_bswap64(w1), __builtin_bswap64(w0));
         }
     }
@@ -122,7 +138,7 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     // Y = X0 - 64k - DL - r + 16g (the funnel shift depends on the lane only)
     mf_v4i A[K];
     {
-        const uint32_t y0 = (uint32_t)(kRgwX0 - 64 * (K - 1) - kRgwDL) - ln + 16 * lg;
+        const uint32_t y0 = (uint32_t)(kX0 - 64 * (K - 1) - kRgwDL) - ln + 16 * lg;
         const uint32_t sh = 8 * (y0 & 3u);
         const uint8_t* pa = breg + (y0 & ~3u);
 #pragma unroll
@@ -137,8 +153,6 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     // ---- rows of the product
     const uint32_t rows = (pl + 127) / 128;
     uint32_t ca = 0, cn = 0;
-    uint64_t nx0, nx1;  // limbs of a's next row, loaded a row ahead
-    rgw_load_row(it.a, na, 0, lane, nx0, nx1);
     uint8_t* pvw = vreg + 8 * (4 * ln + lg);  // word 64 i + 4 n + g of tile i at pvw + 512 i
     const uint8_t* pv = vreg + 32 * lane;     // words 4 lane .. 4 lane + 3
     for (uint32_t R = 0; R < rows; R++) {
@@ -146,7 +160,7 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
         {
             uint64_t w0, w1;
             rgw_digits(nx0, nx1, na, R, lane, &ca, w0, w1);
-            *reinterpret_cast<ulonglong2*>(ring + ((1024 * R + 16 * lane) & (kRgwRing - 1))) =
+            *reinterpret_cast<ulonglong2*>(ring + ((1024 * R + 16 * lane) & (kRing - 1))) =
                 make_ulonglong2(w0, w1);
         }
         if (R + 1 < rows) rgw_load_row(it.a, na, R + 1, lane, nx0, nx1);
@@ -156,7 +170,7 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
         constexpr int NJ = K + 12;
         const int32_t fb = (int32_t)(16 * (ln + lg)) - kRgwDL + 1024 * (int32_t)R - 64 * (K - 1);
         auto frag = [&](int j) {
-            return *reinterpret_cast<const mf_v4i*>(ring + ((uint32_t)(fb + 64 * j) & (kRgwRing - 1)));
+            return *reinterpret_cast<const mf_v4i*>(ring + ((uint32_t)(fb + 64 * j) & (kRing - 1)));
         };
         mf_v4i acc[4];
 #pragma unroll
@@ -212,9 +226,11 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
 
 }  // namespace
 
-// One product per wave: item blockIdx.x * 4 + wave (n_a >= n_b, n_b <= 128).
-// Operands and results may be only 8-byte aligned (packed ragged buffers):
-// global accesses are limb by limb.
+// One product per wave: item blockIdx.x * 4 + wave (n_a >= n_b, fragment
+// count K = floor((8 n_b + 15) / 64) + 1 in [KMIN, KMAX]; KMAX <= 17, i.e.
+// n_b <= 128). Operands and results may be only 8-byte aligned (packed
+// ragged buffers): global accesses are limb by limb.
+template <int KMIN, int KMAX>
 __global__ __launch_bounds__(64 * kRgwWaves) void mul_rgw_kernel(const MfRgItem* __restrict__ items, uint32_t n_items,
                                                                   uint32_t pow8) {
     extern __shared__ __align__(16) uint64_t lds64[];
@@ -223,12 +239,14 @@ __global__ __launch_bounds__(64 * kRgwWaves) void mul_rgw_kernel(const MfRgItem*
     const uint32_t idx = blockIdx.x * kRgwWaves + wv;
     if (idx >= n_items) return;
     const MfRgItem it = items[idx];
-    uint8_t* base = reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * kRgwWaveBytes;
+    uint8_t* base = reinterpret_cast<uint8_t*>(lds64) + (size_t)wv * RgwCfg<KMAX>::WAVE_BYTES;
     const MfPow pw = mf_pow(pow8);
-    const uint32_t K = (8 * it.n_b + 15) / 64 + 1;
+    const int K = (int)((8 * it.n_b + 15) / 64 + 1);
+#define MIDINT_RGW_CASE(k)                                                           \
+    case k:                                                                          \
+        if constexpr (k >= KMIN && k <= KMAX) rgw_product<k, KMAX>(it, base, lane, pw); \
+        break;
     switch (K) {
-#define MIDINT_RGW_CASE(k) \
-    case k: rgw_product<k>(it, base, lane, pw); break;
         MIDINT_RGW_CASE(1)
         MIDINT_RGW_CASE(2)
         MIDINT_RGW_CASE(3)
@@ -246,9 +264,9 @@ __global__ __launch_bounds__(64 * kRgwWaves) void mul_rgw_kernel(const MfRgItem*
         MIDINT_RGW_CASE(15)
         MIDINT_RGW_CASE(16)
         MIDINT_RGW_CASE(17)
-#undef MIDINT_RGW_CASE
         default: break;
     }
+#undef MIDINT_RGW_CASE
 }
 
 // One full sum per wave: r = a + b, max(n_a, n_b) + 1 limbs.
@@ -282,11 +300,19 @@ __global__ __launch_bounds__(64 * kRgwWaves) void add_rgw_kernel(const MfRgItem*
 
 uint32_t rgw_max_short() { return 128; }
 
-size_t rgw_lds_bytes() { return (size_t)kRgwWaves * kRgwWaveBytes; }
+uint32_t rgw_small_max_short() { return (64 * kRgwSmallK - 15 - 1) / 8; }
+
+size_t rgw_lds_bytes(int part) {
+    return (size_t)kRgwWaves * (part == 1 ? RgwCfg<kRgwSmallK>::WAVE_BYTES : RgwCfg<17>::WAVE_BYTES);
+}
 
 uint32_t rgw_waves() { return kRgwWaves; }
 
-const void* rgw_mul_kernel_ptr() { return (const void*)mul_rgw_kernel; }
+const void* rgw_mul_kernel_ptr(int part) {
+    if (part == 1) return (const void*)mul_rgw_kernel<1, kRgwSmallK>;
+    if (part == 2) return (const void*)mul_rgw_kernel<kRgwSmallK + 1, 17>;
+    return (const void*)mul_rgw_kernel<1, 17>;
+}
 
 const void* rgw_add_kernel_ptr() { return (const void*)add_rgw_kernel; }
 

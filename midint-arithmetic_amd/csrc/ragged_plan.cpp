@@ -3,7 +3,8 @@
 // default) the graph is
 //  * the additions, one per wave (add_rgw_kernel), and the multiplications
 //    whose shorter operand has at most 128 limbs, one per wave on 16 x 16 x 64
-//    tiles (mul_rgw_kernel), costliest first, each kind one node;
+//    tiles (mul_rgw_kernel; ragged.wave_split = 1: one node up to 70 limbs,
+//    one above), costliest first;
 //  * the other multiplications on the matrix-core nodes below;
 // all as parallel branches. With ragged.wave = 0 it is a counter reset node
 // followed by
@@ -58,6 +59,7 @@ struct MfLaunch {
 // additions).
 struct WvLaunch {
     bool mul = false;
+    int part = 0;  // products: rgw_mul_kernel_ptr part
     MfRgItem* d_items = nullptr;
     uint32_t n_items = 0;
 };
@@ -226,9 +228,12 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
     // ragged.wave = 1 (default): additions and the products whose shorter
     // operand has at most rgw_max_short() limbs run one op per wave
     const bool by_wave = tune_get("ragged.wave", 1) != 0;
+    // ragged.wave_split = 1: the products up to rgw_small_max_short() limbs on
+    // a kernel of their own (a second node)
+    const bool wv_split = tune_get("ragged.wave_split", 0) != 0;
     std::vector<std::vector<MfRgItem>> mf_by_class(32);  // index: log2 of the size class
     std::vector<midint_op> rest_by_launch[4];  // index: (threads == 512) * 2 + (q == 8)
-    std::vector<MfRgItem> wv_mul, wv_add;
+    std::vector<MfRgItem> wv_mul, wv_mul_small, wv_add;
     for (size_t i = 0; i < n_ops; i++) {
         const midint_op& o = ops[i];
         if (o.reserved != 0 || (o.kind != MIDINT_OP_ADD && o.kind != MIDINT_OP_MUL)) return MIDINT_EINVAL;
@@ -251,7 +256,7 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
             it.r = d_r + o.off_r;
             it.n_a = (uint32_t)(sw ? o.n_b : o.n_a);
             it.n_b = (uint32_t)(sw ? o.n_a : o.n_b);
-            wv_mul.push_back(it);
+            (wv_split && it.n_b <= rgw_small_max_short() ? wv_mul_small : wv_mul).push_back(it);
         } else if (o.kind == MIDINT_OP_MUL && (by_wave || (mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min))) {
             const bool sw = o.n_a < o.n_b;  // the longer operand is the kernel's a
             const uint64_t* x = sw ? d_b + o.off_b : d_a + o.off_a;
@@ -352,19 +357,22 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
 
     // ---- one-op-per-wave nodes: the costliest ops first (the workgroups are
     // dispatched in order as the CUs free up)
-    std::stable_sort(wv_mul.begin(), wv_mul.end(), [](const MfRgItem& x, const MfRgItem& y) {
+    auto by_cost = [](const MfRgItem& x, const MfRgItem& y) {
         const uint64_t cx = (uint64_t)((x.n_a + x.n_b + 127) / 128) * ((8 * x.n_b + 15) / 64 + 1);
         const uint64_t cy = (uint64_t)((y.n_a + y.n_b + 127) / 128) * ((8 * y.n_b + 15) / 64 + 1);
         return cx > cy;
-    });
+    };
+    std::stable_sort(wv_mul.begin(), wv_mul.end(), by_cost);
+    std::stable_sort(wv_mul_small.begin(), wv_mul_small.end(), by_cost);
     std::stable_sort(wv_add.begin(), wv_add.end(), [](const MfRgItem& x, const MfRgItem& y) {
         return std::max(x.n_a, x.n_b) > std::max(y.n_a, y.n_b);
     });
-    for (int k = 0; e == hipSuccess && k < 2; k++) {
-        std::vector<MfRgItem>& v = k == 0 ? wv_mul : wv_add;
+    for (int k = 0; e == hipSuccess && k < 3; k++) {
+        std::vector<MfRgItem>& v = k == 0 ? wv_mul : k == 1 ? wv_mul_small : wv_add;
         if (v.empty()) continue;
         WvLaunch l;
-        l.mul = k == 0;
+        l.mul = k < 2;
+        l.part = !wv_split ? 0 : k == 1 ? 1 : 2;
         l.n_items = (uint32_t)v.size();
         e = hipMalloc((void**)&l.d_items, v.size() * sizeof(MfRgItem));
         if (e == hipSuccess) e = hipMemcpy(l.d_items, v.data(), v.size() * sizeof(MfRgItem), hipMemcpyHostToDevice);
@@ -447,10 +455,10 @@ midint_status midint_ragged_plan_create(midint_ctx* ctx, const midint_op* ops, s
         if (e != hipSuccess) break;
         void* wargs[] = {(void*)&l.d_items, (void*)&l.n_items, (void*)&p->pow8};
         hipKernelNodeParams kp = {};
-        kp.func = const_cast<void*>(l.mul ? rgw_mul_kernel_ptr() : rgw_add_kernel_ptr());
+        kp.func = const_cast<void*>(l.mul ? rgw_mul_kernel_ptr(l.part) : rgw_add_kernel_ptr());
         kp.gridDim = dim3((l.n_items + rgw_waves() - 1) / rgw_waves());
         kp.blockDim = dim3(64 * rgw_waves());
-        kp.sharedMemBytes = l.mul ? (unsigned)rgw_lds_bytes() : 0u;
+        kp.sharedMemBytes = l.mul ? (unsigned)rgw_lds_bytes(l.part) : 0u;
         kp.kernelParams = wargs;
         kp.extra = nullptr;
         hipGraphNode_t n = nullptr;

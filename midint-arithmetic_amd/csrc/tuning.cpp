@@ -42,6 +42,8 @@
 //                 operand has at most 128 limbs, run one op per wave (16 x 16 x 64 tiles
 //                 for the products); the other multiplications on the matrix-core nodes,
 //                 no persistent kernel. 0: the persistent kernels and ragged.mfma_min
+//   ragged.wave_split 1: the one-per-wave products with a shorter operand of at most
+//                 70 limbs on a kernel of their own (fewer registers), a second node
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -60,7 +62,7 @@ std::mutex g_mu;
 std::unordered_map<std::string, int64_t>& table() {
     static std::unordered_map<std::string, int64_t> t = {
         {"mul.q", 8}, {"mul.prefetch", 0}, {"mul.digits", 0}, {"mul.threads", 256}, {"mul.rpark", 1}, {"mul.mfma", 0}, {"mul.mfma_wpi", 0}, {"mul.mfma_ipb", 0}, {"mul.wave16", 1}, {"mul.m16", 1}, {"mul.m16_waves", 0}, {"add.rows", 2}, {"add.threads", 256}, {"add.nt", 1},
-        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0}, {"ragged.m16_ratio", 2}, {"ragged.m16_waves", 4}, {"ragged.wave", 1},
+        {"poly.fused", 1}, {"ntt.fast", 1}, {"ntt.vals", 0}, {"ragged.threads", 256}, {"ragged.q", 4}, {"ragged.mfma_waves", 4}, {"ragged.mfma_min", 32}, {"ragged.overlap", 1}, {"ragged.m16", 0}, {"ragged.m16_ratio", 2}, {"ragged.m16_waves", 4}, {"ragged.wave", 1}, {"ragged.wave_split", 0},
     };
     return t;
 }

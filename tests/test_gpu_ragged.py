@@ -183,7 +183,7 @@ def test_ragged_config_sweep(mi, ctx, torch_dev, orc, seed):
     ops, a, b, rl = make_ragged(mi.gen_limbs, n_ops=int(rng.integers(50, 500)), seed=100 + seed,
                                 mul_frac=float(rng.choice([0.3, 0.7, 1.0])), lo_bits=lo, hi_bits=hi)
     cfg = {"ragged.mfma_min": int(rng.choice([0, 1, 8, 32, 64])), "ragged.overlap": int(rng.integers(0, 2)),
-           "ragged.wave": int(rng.integers(0, 2))}
+           "ragged.wave": int(rng.integers(0, 2)), "ragged.wave_split": int(rng.integers(0, 2))}
     for c in RAGGED_CLASSES:
         cfg[f"ragged.q@{c}"] = int(rng.choice([4, 8]))
         cfg[f"ragged.threads@{c}"] = int(rng.choice([256, 512]))
@@ -274,7 +274,9 @@ def test_ragged_wave_every_short_size(mi, ctx, torch_dev, orc):
     1..128 limbs (every fragment count K = 1..17), as either operand, against
     longer operands of odd, even, row-multiple and maximal length (the rows
     of a staged through the ring, the carry digit in the next row), with
-    all-ones, 0x7F-byte and random limbs; and additions of the same shapes."""
+    all-ones, 0x7F-byte and random limbs; and additions of the same shapes.
+    The products on one kernel and split by fragment count
+    (ragged.wave_split)."""
     K7F = np.uint64(0x7F7F7F7F7F7F7F7F)
     MASK = np.uint64(0xFFFFFFFFFFFFFFFF)
     longs = [1, 2, 127, 128, 129, 255, 256, 383, 512, 640, 1000, 1023, 1024]
@@ -294,13 +296,14 @@ def test_ragged_wave_every_short_size(mi, ctx, torch_dev, orc):
                     kinds.append(kind)
                     fills.append((None, MASK)[len(fills) % 2])
     ops, a, b, rl = _packed(shapes, kinds, fills, 17)
-    saved = _set_keys(mi, {"ragged.wave": 1})
-    try:
-        got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
-    finally:
-        _set_keys(mi, saved)
     want = ragged_reference(orc, ops, a, b, rl)
-    if not np.array_equal(got, want):
-        bad = int(np.flatnonzero(got != want)[0])
-        op = max(i for i, o in enumerate(ops) if o[5] <= bad)
-        raise AssertionError(f"first mismatch at limb {bad}, op {ops[op][:3]}")
+    for split in (0, 1):
+        saved = _set_keys(mi, {"ragged.wave": 1, "ragged.wave_split": split})
+        try:
+            got = run_plan(mi, ctx, torch_dev, ops, a, b, rl, reps=2)
+        finally:
+            _set_keys(mi, saved)
+        if not np.array_equal(got, want):
+            bad = int(np.flatnonzero(got != want)[0])
+            op = max(i for i, o in enumerate(ops) if o[5] <= bad)
+            raise AssertionError(f"split {split}: first mismatch at limb {bad}, op {ops[op][:3]}")
