@@ -245,16 +245,71 @@ __device__ __forceinline__ uint64_t gl_reduce(uint32_t l0, uint32_t l1, uint32_t
     return join32(x0, x1);
 }
 
+// The same reduction with the low half as one 64-bit value L = l1:l0, the
+// 2^64 term taken by one multiply-add: T = L + hl * (2^32 - 1) (v_mad_u64_u32,
+// 65 bits, carry kT), x = T + kT * eps - hh. T + kT eps cannot overflow (when
+// kT is set, T < hl eps <= 2^64 - 2^33 + 1); a borrow of the subtraction of
+// hh means x + 2^64 was computed, fixed by - eps (= + p - 2^64), which cannot
+// borrow again; then one conditional subtraction of p. 13 VALU against 16.
+// The carry of the multiply-add becomes a VGPR value inside the first asm
+// statement, so no mask crosses statements.
+__device__ __forceinline__ uint64_t gl_reduce_l(uint64_t L, uint32_t hl, uint32_t hh) {
+    uint64_t T, k1;
+    uint32_t e;
+    asm("v_mad_u64_u32 %[T], %[k], %[hl], -1, %[L]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]"
+        : [T] "=&v"(T), [e] "=&v"(e), [k] "=&s"(k1)
+        : [hl] "v"(hl), [L] "v"(L));
+    uint32_t y0, y1, z0, z1, e2;
+    uint64_t k;
+    asm("v_add_co_u32_e64 %[y0], %[k], %[t0], %[e]\n\t"
+        "v_addc_co_u32_e64 %[y1], %[k], %[t1], 0, %[k]\n\t"
+        "v_sub_co_u32_e64 %[y0], %[k], %[y0], %[hh]\n\t"
+        "v_subb_co_u32_e64 %[y1], %[k], %[y1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[e2], 0, -1, %[k]\n\t"
+        "v_sub_co_u32_e64 %[y0], %[k], %[y0], %[e2]\n\t"
+        "v_subb_co_u32_e64 %[y1], %[k], %[y1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[z0], %[k], %[y0], -1\n\t"
+        "v_addc_co_u32_e64 %[z1], %[k], %[y1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[y0], %[y0], %[z0], %[k]\n\t"
+        "v_cndmask_b32_e64 %[y1], %[y1], %[z1], %[k]"
+        : [y0] "=&v"(y0), [y1] "=&v"(y1), [z0] "=&v"(z0), [z1] "=&v"(z1), [e2] "=&v"(e2), [k] "=&s"(k)
+        : [t0] "v"(lo32(T)), [t1] "v"(hi32(T)), [e] "v"(e), [hh] "v"(hh));
+    return join32(y0, y1);
+}
+
+// L + 2^64 hl reduced mod p (canonical): gl_reduce_l with hh = 0, where
+// T + kT eps is the value (no borrow step). 8 VALU.
+__device__ __forceinline__ uint64_t gl_reduce_l0(uint64_t L, uint32_t hl) {
+    uint64_t T, k1;
+    uint32_t e;
+    asm("v_mad_u64_u32 %[T], %[k], %[hl], -1, %[L]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]"
+        : [T] "=&v"(T), [e] "=&v"(e), [k] "=&s"(k1)
+        : [hl] "v"(hl), [L] "v"(L));
+    uint32_t y0, y1, z0, z1;
+    uint64_t k;
+    asm("v_add_co_u32_e64 %[y0], %[k], %[t0], %[e]\n\t"
+        "v_addc_co_u32_e64 %[y1], %[k], %[t1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[z0], %[k], %[y0], -1\n\t"
+        "v_addc_co_u32_e64 %[z1], %[k], %[y1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[y0], %[y0], %[z0], %[k]\n\t"
+        "v_cndmask_b32_e64 %[y1], %[y1], %[z1], %[k]"
+        : [y0] "=&v"(y0), [y1] "=&v"(y1), [z0] "=&v"(z0), [z1] "=&v"(z1), [k] "=&s"(k)
+        : [t0] "v"(lo32(T)), [t1] "v"(hi32(T)), [e] "v"(e));
+    return join32(y0, y1);
+}
+
 // x * 2^s for 0 < s < 64: the 128-bit shift, then the reduction. Meant for
 // compile-time s (the shifts and word selects fold away).
 __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, int s) {
     const uint32_t x0 = lo32(x), x1 = hi32(x);
     if (s < 32) {
-        return gl_reduce(x0 << s, __builtin_amdgcn_alignbit(x1, x0, 32 - s), x1 >> (32 - s), 0u);
+        return gl_reduce_l0(x << s, x1 >> (32 - s));
     } else if (s == 32) {
-        return gl_reduce(0u, x0, x1, 0u);
+        return gl_reduce_l0(x << 32, x1);
     }
-    return gl_reduce(0u, x0 << (s - 32), __builtin_amdgcn_alignbit(x1, x0, 64 - s), x1 >> (64 - s));
+    return gl_reduce_l(x << s, __builtin_amdgcn_alignbit(x1, x0, 64 - s), x1 >> (64 - s));
 }
 
 // 64x64 -> 128 from four v_mad_u64_u32 (no partial sum can overflow 64
@@ -265,7 +320,7 @@ __device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
     const uint64_t t = (uint64_t)ah * bl + hi32(p00);
     const uint64_t u = (uint64_t)al * bh + lo32(t);
     const uint64_t h = (uint64_t)ah * bh + hi32(t) + hi32(u);
-    return gl_reduce(lo32(p00), lo32(u), lo32(h), hi32(h));
+    return gl_reduce_l(join32(lo32(p00), lo32(u)), lo32(h), hi32(h));
 }
 
 }  // namespace midint

@@ -338,7 +338,9 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
     ntt_fwd<LG, V, MODE>(gb, m, x, v, twf);
 #pragma unroll
     for (int k = 0; k < V; k++) v[k] = gl_mul(v[k], ah[k]);
-    __syncthreads();
+    // b's last forward pass read only this wave's block of LDS, which is all
+    // the inverse's first store writes
+    ntt_wave_sync();
     NttInv<LG, V, Pl::NP - 1>::run(x, v, opaque_u32(tid), twi);
 
     if (MODE == 1) {

@@ -32,6 +32,33 @@ __device__ __forceinline__ uint32_t opaque_u32(uint32_t x) {
     return x;
 }
 
+// Group j of thread tid in a pass with G groups per thread: wave-contiguous,
+// g = wave 64 G + 64 j + lane, so that wave w's groups are [64 G w, 64 G (w+1))
+// (the same stride between lanes as tid + T j). With this assignment every
+// pass after the first touches only the elements whose top log2(T / 64)
+// index bits are the wave's index (the first pass's stages span exactly
+// those bits): wave w's block of L / (T / 64) consecutive elements. The LDS
+// exchanges between those passes are wave-local and need no workgroup
+// barrier.
+template <int G>
+__device__ __forceinline__ uint32_t ntt_group(uint32_t tid, int j) {
+    if constexpr (G == 1) {
+        return tid;
+    } else {
+        constexpr int lgG = G == 2 ? 1 : G == 4 ? 2 : G == 8 ? 3 : G == 16 ? 4 : 5;
+        static_assert((1 << lgG) == G, "groups per thread");
+        return ((tid >> 6) << (6 + lgG)) | ((uint32_t)j << 6) | (tid & 63u);
+    }
+}
+
+// Orders a wave's LDS stores before its following LDS loads (LDS executes a
+// wave's instructions in order; this keeps the compiler from moving loads of
+// other lanes' data above the stores).
+__device__ __forceinline__ void ntt_wave_sync() {
+    __builtin_amdgcn_wave_barrier();
+    asm volatile("" ::: "memory");
+}
+
 template <int LG, int V>
 struct NttPlan {
     static constexpr int RM = V == 32 ? 5 : 4;  // stages per full pass
@@ -155,13 +182,12 @@ template <int R, bool INV, int V, int LG, int S0, bool ZHI = false>
 __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, const uint64_t* __restrict__ tw) {
     constexpr int G = V >> R;  // groups per thread
     constexpr int K = 1 << R;  // values per group
-    constexpr uint32_t T = (1u << LG) / V;
     constexpr int lgH = LG - 1 - S0;      // H = 2^lgH
     constexpr int lgmin = lgH - (R - 1);  // h_min
     if constexpr (lgmin > 0) {
 #pragma unroll
         for (int j = 0; j < G; j++) {
-            const uint32_t g = tid + T * j;
+            const uint32_t g = ntt_group<G>(tid, j);
             const uint32_t g_lo = g & ((1u << lgmin) - 1);
             NttUnits<R, INV, V, LG, S0, ZHI, 0>::run(v, j * K, g_lo, tw);
         }
@@ -169,7 +195,7 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
     }
 #pragma unroll
     for (int j = 0; j < G; j++) {
-        const uint32_t g = tid + T * j;
+        const uint32_t g = ntt_group<G>(tid, j);
         const uint32_t g_lo = g & ((1u << lgmin) - 1);
 #pragma unroll
         for (int st = 0; st < R; st++) {
@@ -232,12 +258,11 @@ __device__ __forceinline__ uint32_t ntt_group_index(uint32_t lg, uint32_t s0, ui
 // compile-time offset.
 template <int R, int V, int LG, int S0>
 __device__ __forceinline__ void ntt_load_pass(const uint64_t* x, uint64_t (&v)[V], uint32_t tid) {
-    constexpr uint32_t T = (1u << LG) / V;
     constexpr int lgH = LG - 1 - S0;
     constexpr int lgmin = lgH - (R - 1);
 #pragma unroll
     for (int j = 0; j < (V >> R); j++) {
-        const uint32_t g = tid + T * j;
+        const uint32_t g = ntt_group<(V >> R)>(tid, j);
         const uint64_t* xb = x + ntt_pad(((g >> lgmin) << (lgH + 1)) + (g & ((1u << lgmin) - 1)));
 #pragma unroll
         for (int k = 0; k < (1 << R); k++) {
@@ -249,12 +274,11 @@ __device__ __forceinline__ void ntt_load_pass(const uint64_t* x, uint64_t (&v)[V
 
 template <int R, int V, int LG, int S0>
 __device__ __forceinline__ void ntt_store_pass(uint64_t* x, const uint64_t (&v)[V], uint32_t tid) {
-    constexpr uint32_t T = (1u << LG) / V;
     constexpr int lgH = LG - 1 - S0;
     constexpr int lgmin = lgH - (R - 1);
 #pragma unroll
     for (int j = 0; j < (V >> R); j++) {
-        const uint32_t g = tid + T * j;
+        const uint32_t g = ntt_group<(V >> R)>(tid, j);
         uint64_t* xb = x + ntt_pad(((g >> lgmin) << (lgH + 1)) + (g & ((1u << lgmin) - 1)));
 #pragma unroll
         for (int k = 0; k < (1 << R); k++) {
@@ -276,7 +300,11 @@ struct NttFwd {
         if constexpr (P + 1 < Pl::NP) {
             constexpr int R1 = Pl::radix(P + 1);
             ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
-            __syncthreads();
+            if constexpr (P >= 1) {
+                ntt_wave_sync();  // passes P and P + 1 both within the wave's block
+            } else {
+                __syncthreads();
+            }
             ntt_load_pass<R1, V, LG, (P + 1) * Pl::RM>(x, v, tid);
             NttFwd<LG, V, P + 1>::run(x, v, tid, twf);
         }
@@ -284,7 +312,9 @@ struct NttFwd {
 };
 
 // Inverse passes p .. 0 starting from values of pass p's grouping in v; ends
-// with the natural-order result stored in LDS (padded) and a barrier.
+// with the natural-order result stored in LDS (padded) and a barrier. The
+// passes after the first touch only the wave's own block (ntt_group), so
+// the first store needs no barrier after a forward transform's last pass.
 template <int LG, int V, int P>
 struct NttInv {
     __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
@@ -293,7 +323,11 @@ struct NttInv {
         constexpr int R = Pl::radix(P);
         ntt_pass_regs<R, true, V, LG, P * Pl::RM>(v, tid, twi);
         ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
-        __syncthreads();
+        if constexpr (P >= 2) {
+            ntt_wave_sync();  // passes P and P - 1 both within the wave's block
+        } else {
+            __syncthreads();
+        }
         if constexpr (P > 0) {
             constexpr int R1 = Pl::radix(P - 1);
             ntt_load_pass<R1, V, LG, (P - 1) * Pl::RM>(x, v, tid);
