@@ -389,7 +389,7 @@ __global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, co
     ntt_fwd<LG, V, 0, true>(ga, m, x, P, twf, inv_len);
 #pragma unroll
     for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mul(P[k], P[k]), LG);
-    __syncthreads();
+    ntt_wave_sync();  // the last forward pass and the first inverse store: the wave's own block
     NttInv<LG, V, Pl::NP - 1>::run(x, Q, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gb, m, lds_gp);
@@ -400,7 +400,7 @@ __global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, co
     ntt_fwd<LG, V, 0, true>(gb, m, x, Q, twf, inv_len);
 #pragma unroll
     for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
-    __syncthreads();
+    ntt_wave_sync();
     NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_store_rows<R>(rows, gr, m);
@@ -421,7 +421,7 @@ __global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, co
     ntt_fwd<LG, V, 0, true>(s1, m, x, P, twf, inv_len);
 #pragma unroll
     for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
-    __syncthreads();
+    ntt_wave_sync();
     NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gr, m, lds_gp);
