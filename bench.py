@@ -145,6 +145,16 @@ def measured_traffic(name):
         return None
 
 
+def profiled_valu_per_product(name):
+    """wave64 VALU instructions per product of a kernel (SQ_INSTS_VALU of the
+    committed rocprofv3 summary, profiles/traffic.json), or None."""
+    try:
+        with open(os.path.join(ROOT, "profiles", "traffic.json")) as f:
+            return json.load(f)["kernels"][name]["valu_wave_instr_per_product"]
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def cpu_threads():
     """Host threads for the CPU baselines: the CPUs this process may run on,
     capped at one GPU's share of a measurement box."""
@@ -515,6 +525,19 @@ def main():
     ms_ntt = timed(torch, lambda: ctx.mul_ntt(dan, dbn, drn, mn), max(5, args.steps // 20), 2)
     ntt_pp = nn * 360448 * 4  # radix-2-equivalent modmuls per product (3 transforms + pointwise), 4 pp each
     log(f"mul_ntt {nn} x {mn}: {ms_ntt * 1e3:.1f} us/batch ({ms_ntt * 1e3 / nn:.3f} us per product)")
+    # the NTT's own bound: VALU instruction issue. wave64 instructions per
+    # product from the committed profile, the issue ceiling from the probe
+    # (probe_add32 counts two ops per instruction and lane)
+    ntt_valu = profiled_valu_per_product("mul_ntt_1024x2048")
+    ntt_issue = None
+    if ntt_valu:
+        achieved_wi = ntt_valu * nn / (ms_ntt * 1e-3)
+        peak_wi = add_rate / 2 / 64
+        ntt_issue = {"bound": "valu-issue", "achieved": achieved_wi / 1e9, "peak": peak_wi / 1e9,
+                     "unit": "G wave64 VALU instructions/s", "frac": achieved_wi / peak_wi,
+                     "instr_per_product": ntt_valu,
+                     "source": "instructions per product: profiles/traffic.json (rocprofv3 SQ_INSTS_VALU); peak: "
+                               "midint_probe_valu's 32-bit VALU issue rate on this GPU"}
     del drn
 
     # ---------------------------------------------------------- add (HBM)
@@ -585,7 +608,8 @@ def main():
                                            "unit": "T partial-products/s (360448 radix-2-equivalent modmuls x 4 "
                                                    "per product)",
                                            "frac": ntt_pp / (ms_ntt * 1e-3) / mad_rate,
-                                           "traffic": measured_traffic("mul_ntt_1024x2048")}},
+                                           "traffic": measured_traffic("mul_ntt_1024x2048")},
+                              "valu_issue": ntt_issue},
             "mul_valu_equivalent": valu_roofline,
             "composites": composites,
             "mul_sweep": mul_sweep,
