@@ -254,28 +254,30 @@ __device__ __forceinline__ uint64_t gl_reduce(uint32_t l0, uint32_t l1, uint32_t
 // The carry of the multiply-add becomes a VGPR value inside the first asm
 // statement, so no mask crosses statements.
 __device__ __forceinline__ uint64_t gl_reduce_l(uint64_t L, uint32_t hl, uint32_t hh) {
+    // (the second statement works in place on T's words and e, with one
+    // temporary: the reduction sits inside register-heavy transforms)
     uint64_t T, k1;
     uint32_t e;
     asm("v_mad_u64_u32 %[T], %[k], %[hl], -1, %[L]\n\t"
         "v_cndmask_b32_e64 %[e], 0, -1, %[k]"
-        : [T] "=&v"(T), [e] "=&v"(e), [k] "=&s"(k1)
+        : [T] "=v"(T), [e] "=v"(e), [k] "=&s"(k1)
         : [hl] "v"(hl), [L] "v"(L));
-    uint32_t y0, y1, z0, z1, e2;
+    uint32_t t0 = lo32(T), t1 = hi32(T), z1;
     uint64_t k;
-    asm("v_add_co_u32_e64 %[y0], %[k], %[t0], %[e]\n\t"
-        "v_addc_co_u32_e64 %[y1], %[k], %[t1], 0, %[k]\n\t"
-        "v_sub_co_u32_e64 %[y0], %[k], %[y0], %[hh]\n\t"
-        "v_subb_co_u32_e64 %[y1], %[k], %[y1], 0, %[k]\n\t"
-        "v_cndmask_b32_e64 %[e2], 0, -1, %[k]\n\t"
-        "v_sub_co_u32_e64 %[y0], %[k], %[y0], %[e2]\n\t"
-        "v_subb_co_u32_e64 %[y1], %[k], %[y1], 0, %[k]\n\t"
-        "v_add_co_u32_e64 %[z0], %[k], %[y0], -1\n\t"
-        "v_addc_co_u32_e64 %[z1], %[k], %[y1], 0, %[k]\n\t"
-        "v_cndmask_b32_e64 %[y0], %[y0], %[z0], %[k]\n\t"
-        "v_cndmask_b32_e64 %[y1], %[y1], %[z1], %[k]"
-        : [y0] "=&v"(y0), [y1] "=&v"(y1), [z0] "=&v"(z0), [z1] "=&v"(z1), [e2] "=&v"(e2), [k] "=&s"(k)
-        : [t0] "v"(lo32(T)), [t1] "v"(hi32(T)), [e] "v"(e), [hh] "v"(hh));
-    return join32(y0, y1);
+    asm("v_add_co_u32_e64 %[t0], %[k], %[t0], %[e]\n\t"
+        "v_addc_co_u32_e64 %[t1], %[k], %[t1], 0, %[k]\n\t"
+        "v_sub_co_u32_e64 %[t0], %[k], %[t0], %[hh]\n\t"
+        "v_subb_co_u32_e64 %[t1], %[k], %[t1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[e], 0, -1, %[k]\n\t"
+        "v_sub_co_u32_e64 %[t0], %[k], %[t0], %[e]\n\t"
+        "v_subb_co_u32_e64 %[t1], %[k], %[t1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[e], %[k], %[t0], -1\n\t"
+        "v_addc_co_u32_e64 %[z1], %[k], %[t1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[t0], %[t0], %[e], %[k]\n\t"
+        "v_cndmask_b32_e64 %[t1], %[t1], %[z1], %[k]"
+        : [t0] "+v"(t0), [t1] "+v"(t1), [e] "+v"(e), [z1] "=&v"(z1), [k] "=&s"(k)
+        : [hh] "v"(hh));
+    return join32(t0, t1);
 }
 
 // L + 2^64 hl reduced mod p (canonical): gl_reduce_l with hh = 0, where
@@ -285,19 +287,18 @@ __device__ __forceinline__ uint64_t gl_reduce_l0(uint64_t L, uint32_t hl) {
     uint32_t e;
     asm("v_mad_u64_u32 %[T], %[k], %[hl], -1, %[L]\n\t"
         "v_cndmask_b32_e64 %[e], 0, -1, %[k]"
-        : [T] "=&v"(T), [e] "=&v"(e), [k] "=&s"(k1)
+        : [T] "=v"(T), [e] "=v"(e), [k] "=&s"(k1)
         : [hl] "v"(hl), [L] "v"(L));
-    uint32_t y0, y1, z0, z1;
+    uint32_t t0 = lo32(T), t1 = hi32(T), z1;
     uint64_t k;
-    asm("v_add_co_u32_e64 %[y0], %[k], %[t0], %[e]\n\t"
-        "v_addc_co_u32_e64 %[y1], %[k], %[t1], 0, %[k]\n\t"
-        "v_add_co_u32_e64 %[z0], %[k], %[y0], -1\n\t"
-        "v_addc_co_u32_e64 %[z1], %[k], %[y1], 0, %[k]\n\t"
-        "v_cndmask_b32_e64 %[y0], %[y0], %[z0], %[k]\n\t"
-        "v_cndmask_b32_e64 %[y1], %[y1], %[z1], %[k]"
-        : [y0] "=&v"(y0), [y1] "=&v"(y1), [z0] "=&v"(z0), [z1] "=&v"(z1), [k] "=&s"(k)
-        : [t0] "v"(lo32(T)), [t1] "v"(hi32(T)), [e] "v"(e));
-    return join32(y0, y1);
+    asm("v_add_co_u32_e64 %[t0], %[k], %[t0], %[e]\n\t"
+        "v_addc_co_u32_e64 %[t1], %[k], %[t1], 0, %[k]\n\t"
+        "v_add_co_u32_e64 %[e], %[k], %[t0], -1\n\t"
+        "v_addc_co_u32_e64 %[z1], %[k], %[t1], 0, %[k]\n\t"
+        "v_cndmask_b32_e64 %[t0], %[t0], %[e], %[k]\n\t"
+        "v_cndmask_b32_e64 %[t1], %[t1], %[z1], %[k]"
+        : [t0] "+v"(t0), [t1] "+v"(t1), [e] "+v"(e), [z1] "=&v"(z1), [k] "=&s"(k));
+    return join32(t0, t1);
 }
 
 // x * 2^s for 0 < s < 64: the 128-bit shift, then the reduction. Meant for
@@ -313,14 +314,23 @@ __device__ __forceinline__ uint64_t gl_mul_pow2(uint64_t x, int s) {
 }
 
 // 64x64 -> 128 from four v_mad_u64_u32 (no partial sum can overflow 64
-// bits), then the reduction.
-__device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) {
+// bits), then the reduction: FM = 0 through the multiply-add (gl_reduce_l,
+// fewer instructions), FM = 1 on the carry chain alone (gl_reduce, fewer
+// live registers: for kernels that hold two transforms in registers).
+template <int FM = 0>
+__device__ __forceinline__ uint64_t gl_mulv(uint64_t a, uint64_t b) {
     const uint32_t al = lo32(a), ah = hi32(a), bl = lo32(b), bh = hi32(b);
     const uint64_t p00 = (uint64_t)al * bl;
     const uint64_t t = (uint64_t)ah * bl + hi32(p00);
     const uint64_t u = (uint64_t)al * bh + lo32(t);
     const uint64_t h = (uint64_t)ah * bh + hi32(t) + hi32(u);
-    return gl_reduce_l(join32(lo32(p00), lo32(u)), lo32(h), hi32(h));
+    if constexpr (FM == 1) {
+        return gl_reduce(lo32(p00), lo32(u), lo32(h), hi32(h));
+    } else {
+        return gl_reduce_l(join32(lo32(p00), lo32(u)), lo32(h), hi32(h));
+    }
 }
+
+__device__ __forceinline__ uint64_t gl_mul(uint64_t a, uint64_t b) { return gl_mulv<0>(a, b); }
 
 }  // namespace midint

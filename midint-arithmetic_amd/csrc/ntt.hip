@@ -207,7 +207,7 @@ __device__ __forceinline__ uint64_t ntt_coef(const uint64_t* g, uint32_t i, uint
 // a's nonzero inputs instead of to all L transformed values).
 // SYNC: a barrier between loading the inputs and the first pass's LDS
 // stores (for an input that lives in the transform's own LDS area).
-template <int LG, int V, int MODE, bool SCALE = false, bool SYNC = false>
+template <int LG, int V, int MODE, bool SCALE = false, bool SYNC = false, int FM = 0>
 __device__ __forceinline__ void ntt_fwd(const uint64_t* g, uint32_t m, uint64_t* x, uint64_t (&v)[V],
                                         const uint64_t* __restrict__ twf, uint64_t scale = 0) {
     using Pl = NttPlan<LG, V>;
@@ -224,11 +224,11 @@ __device__ __forceinline__ void ntt_fwd(const uint64_t* g, uint32_t m, uint64_t*
                 v[(j << R0) + k] = 0ull;
             } else {
                 const uint64_t c = ntt_coef<MODE>(g, ntt_group_index<R0>(LG, 0, tid, Pl::T, j, k), m);
-                v[(j << R0) + k] = SCALE ? gl_mul(c, scale) : c;
+                v[(j << R0) + k] = SCALE ? gl_mulv<FM>(c, scale) : c;
             }
         }
     if (SYNC) __syncthreads();
-    NttFwd<LG, V, 0, ZHI>::run(x, v, tid, twf);
+    NttFwd<LG, V, 0, ZHI, FM>::run(x, v, tid, twf);
 }
 
 // Exact recombination of the 16-bit-spaced coefficients of a product,
@@ -354,6 +354,11 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
     ntt_store_rows<V / 8>(rows, r + (size_t)blockIdx.x * rl, rl);  // rl = 2m (full) or m (low half)
 }
 
+// The polynomial kernel holds two transformed operands in registers; its
+// field multiplications reduce on the carry chain alone (gl_mulv<1>), whose
+// fewer live registers keep it within 128 VGPRs without spilling.
+constexpr int kPolyFM = 1;
+
 // The reference's polyFttKer (cuda/ker-fft-mul.cu.h:456-509) in one kernel:
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance (one per workgroup),
 // the steps mod B^m in the reference's order, with eight L-point
@@ -386,43 +391,43 @@ __global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, co
     Rows<R> rows;
 
     // ---- s1 = a^2 + b
-    ntt_fwd<LG, V, 0, true>(ga, m, x, P, twf, inv_len);
+    ntt_fwd<LG, V, 0, true, false, kPolyFM>(ga, m, x, P, twf, inv_len);
 #pragma unroll
-    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mul(P[k], P[k]), LG);
+    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mulv<kPolyFM>(P[k], P[k]), LG);
     ntt_wave_sync();  // the last forward pass and the first inverse store: the wave's own block
-    NttInv<LG, V, Pl::NP - 1>::run(x, Q, opaque_u32(threadIdx.x), twi);
+    NttInv<LG, V, Pl::NP - 1, kPolyFM>::run(x, Q, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gb, m, lds_gp);
     ntt_store_rows<R>(rows, s1, m);
     __syncthreads();
 
     // ---- ab (kept in r)
-    ntt_fwd<LG, V, 0, true>(gb, m, x, Q, twf, inv_len);
+    ntt_fwd<LG, V, 0, true, false, kPolyFM>(gb, m, x, Q, twf, inv_len);
 #pragma unroll
-    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
+    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mulv<kPolyFM>(P[k], Q[k]), LG);
     ntt_wave_sync();
-    NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
+    NttInv<LG, V, Pl::NP - 1, kPolyFM>::run(x, P, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_store_rows<R>(rows, gr, m);
     __syncthreads();
 
     // ---- s2 = b^2 + b, into the transform area
 #pragma unroll
-    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mul(Q[k], Q[k]), LG);
-    NttInv<LG, V, Pl::NP - 1>::run(x, Q, opaque_u32(threadIdx.x), twi);
+    for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mulv<kPolyFM>(Q[k], Q[k]), LG);
+    NttInv<LG, V, Pl::NP - 1, kPolyFM>::run(x, Q, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gb, m, lds_gp);
     ntt_store_rows<R>(rows, x, m);
     __syncthreads();
 
     // ---- r = s1 s2 + ab
-    ntt_fwd<LG, V, 0, true, true>(x, m, x, Q, twf, inv_len);
+    ntt_fwd<LG, V, 0, true, true, kPolyFM>(x, m, x, Q, twf, inv_len);
     __syncthreads();  // every thread is done reading LDS for s2's last pass
-    ntt_fwd<LG, V, 0, true>(s1, m, x, P, twf, inv_len);
+    ntt_fwd<LG, V, 0, true, false, kPolyFM>(s1, m, x, P, twf, inv_len);
 #pragma unroll
-    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mul(P[k], Q[k]), LG);
+    for (int k = 0; k < V; k++) P[k] = gl_mul_pow2(gl_mulv<kPolyFM>(P[k], Q[k]), LG);
     ntt_wave_sync();
-    NttInv<LG, V, Pl::NP - 1>::run(x, P, opaque_u32(threadIdx.x), twi);
+    NttInv<LG, V, Pl::NP - 1, kPolyFM>::run(x, P, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gr, m, lds_gp);
     ntt_store_rows<R>(rows, gr, m);

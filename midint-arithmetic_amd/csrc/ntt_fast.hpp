@@ -82,7 +82,7 @@ struct NttPlan {
 //   c2 = Y2 + Y3, c3 = (Y3 - Y2) 2^48,
 //   x0 = c0 + c2, x2 = c0 - c2, x1 = c1 + c3, x3 = c1 - c3.
 // The twiddle tables hold w^i and w^-i for i < L (3E can exceed L / 2).
-template <bool INV, int LG, bool ZERO_HI>
+template <bool INV, int LG, bool ZERO_HI, int FM = 0>
 __device__ __forceinline__ void ntt_radix4(uint64_t& s0, uint64_t& s1, uint64_t& s2, uint64_t& s3, uint32_t e,
                                            const uint64_t* __restrict__ tw) {
     constexpr uint32_t kMask = (1u << LG) - 1;
@@ -102,11 +102,11 @@ __device__ __forceinline__ void ntt_radix4(uint64_t& s0, uint64_t& s1, uint64_t&
             b3 = gl_mul_pow2(gl_sub(x1, x3), 48);
         }
         s0 = gl_add(b0, b1);
-        s1 = gl_mul(gl_sub(b0, b1), w2);
-        s2 = gl_mul(gl_add(b2, b3), w1);
-        s3 = gl_mul(gl_sub(b2, b3), w3);
+        s1 = gl_mulv<FM>(gl_sub(b0, b1), w2);
+        s2 = gl_mulv<FM>(gl_add(b2, b3), w1);
+        s3 = gl_mulv<FM>(gl_sub(b2, b3), w3);
     } else {
-        const uint64_t y0 = s0, Y1 = gl_mul(s1, w2), Y2 = gl_mul(s2, w1), Y3 = gl_mul(s3, w3);
+        const uint64_t y0 = s0, Y1 = gl_mulv<FM>(s1, w2), Y2 = gl_mulv<FM>(s2, w1), Y3 = gl_mulv<FM>(s3, w3);
         const uint64_t c0 = gl_add(y0, Y1), c1 = gl_sub(y0, Y1);
         const uint64_t c2 = gl_add(Y2, Y3), c3 = gl_mul_pow2(gl_sub(Y3, Y2), 48);
         s0 = gl_add(c0, c2);
@@ -121,7 +121,7 @@ __device__ __forceinline__ void ntt_radix4(uint64_t& s0, uint64_t& s1, uint64_t&
 // jj = R-1, R-3, ..., then the span-1 stage alone if R is odd; the inverse
 // runs the mirror order. U counts stage positions; the recursion keeps every
 // index a compile-time constant.
-template <int R, bool INV, int V, int LG, int S0, bool ZHI, int U>
+template <int R, bool INV, int V, int LG, int S0, bool ZHI, int FM, int U>
 struct NttUnits {
     static __device__ __forceinline__ void run(uint64_t (&v)[V], int base, uint32_t g_lo,
                                                const uint64_t* __restrict__ tw) {
@@ -141,9 +141,9 @@ struct NttUnits {
                 if constexpr (!INV) {
                     const uint64_t x = a, y = b;
                     a = gl_add(x, y);
-                    b = gl_mul(gl_sub(x, y), wt);
+                    b = gl_mulv<FM>(gl_sub(x, y), wt);
                 } else {
-                    const uint64_t x = a, y = gl_mul(b, wt);
+                    const uint64_t x = a, y = gl_mulv<FM>(b, wt);
                     a = gl_add(x, y);
                     b = gl_sub(x, y);
                 }
@@ -158,14 +158,14 @@ struct NttUnits {
                 if (k & (J | J2)) continue;
                 const uint32_t klo = (uint32_t)(k & (J2 - 1));
                 const uint32_t e = (g_lo + (klo << lgmin)) << s;
-                ntt_radix4<INV, LG, zhi>(v[base + k], v[base + k + J2], v[base + k + J], v[base + k + J + J2], e, tw);
+                ntt_radix4<INV, LG, zhi, FM>(v[base + k], v[base + k + J2], v[base + k + J], v[base + k + J + J2], e, tw);
             }
         }
-        NttUnits<R, INV, V, LG, S0, ZHI, U + 1>::run(v, base, g_lo, tw);
+        NttUnits<R, INV, V, LG, S0, ZHI, FM, U + 1>::run(v, base, g_lo, tw);
     }
 };
-template <int R, bool INV, int V, int LG, int S0, bool ZHI>
-struct NttUnits<R, INV, V, LG, S0, ZHI, R> {
+template <int R, bool INV, int V, int LG, int S0, bool ZHI, int FM>
+struct NttUnits<R, INV, V, LG, S0, ZHI, FM, R> {
     static __device__ __forceinline__ void run(uint64_t (&)[V], int, uint32_t, const uint64_t* __restrict__) {}
 };
 
@@ -178,7 +178,7 @@ struct NttUnits<R, INV, V, LG, S0, ZHI, R> {
 // stage of span 1 last; the inverse mirrors that order.
 // ZHI (forward, first pass): the upper half of the input is zero, so the
 // first stage is a = u, b = u w.
-template <int R, bool INV, int V, int LG, int S0, bool ZHI = false>
+template <int R, bool INV, int V, int LG, int S0, bool ZHI = false, int FM = 0>
 __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, const uint64_t* __restrict__ tw) {
     constexpr int G = V >> R;  // groups per thread
     constexpr int K = 1 << R;  // values per group
@@ -189,7 +189,7 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
         for (int j = 0; j < G; j++) {
             const uint32_t g = ntt_group<G>(tid, j);
             const uint32_t g_lo = g & ((1u << lgmin) - 1);
-            NttUnits<R, INV, V, LG, S0, ZHI, 0>::run(v, j * K, g_lo, tw);
+            NttUnits<R, INV, V, LG, S0, ZHI, FM, 0>::run(v, j * K, g_lo, tw);
         }
         return;
     }
@@ -221,14 +221,14 @@ __device__ __forceinline__ void ntt_pass_regs(uint64_t (&v)[V], uint32_t tid, co
                     const uint64_t d = zero_x ? u : gl_sub(u, x);
                     b = unit           ? d
                         : sh > 0 && sh < 64 ? gl_mul_pow2(d, sh)
-                                            : gl_mul(d, tw[(g_lo + (klo << lgmin)) << s]);
+                                            : gl_mulv<FM>(d, tw[(g_lo + (klo << lgmin)) << s]);
                 } else if (sh > 32) {
                     // w^-e = 2^(192 - sh) = -2^(96 - sh): swap the add and the sub
                     const uint64_t u = a, y = gl_mul_pow2(b, 96 - sh);
                     a = gl_sub(u, y);
                     b = gl_add(u, y);
                 } else {
-                    const uint64_t u = a, x = unit ? b : gl_mul(b, tw[(g_lo + (klo << lgmin)) << s]);
+                    const uint64_t u = a, x = unit ? b : gl_mulv<FM>(b, tw[(g_lo + (klo << lgmin)) << s]);
                     a = gl_add(u, x);
                     b = gl_sub(u, x);
                 }
@@ -290,13 +290,13 @@ __device__ __forceinline__ void ntt_store_pass(uint64_t* x, const uint64_t (&v)[
 
 // Forward passes p .. NP-1 starting from values of pass p's grouping in v;
 // returns with the last pass's values in v.
-template <int LG, int V, int P, bool ZHI = false>
+template <int LG, int V, int P, bool ZHI = false, int FM = 0>
 struct NttFwd {
     __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
                                                const uint64_t* __restrict__ twf) {
         using Pl = NttPlan<LG, V>;
         constexpr int R = Pl::radix(P);
-        ntt_pass_regs<R, false, V, LG, P * Pl::RM, ZHI && P == 0>(v, tid, twf);
+        ntt_pass_regs<R, false, V, LG, P * Pl::RM, ZHI && P == 0, FM>(v, tid, twf);
         if constexpr (P + 1 < Pl::NP) {
             constexpr int R1 = Pl::radix(P + 1);
             ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
@@ -306,7 +306,7 @@ struct NttFwd {
                 __syncthreads();
             }
             ntt_load_pass<R1, V, LG, (P + 1) * Pl::RM>(x, v, tid);
-            NttFwd<LG, V, P + 1>::run(x, v, tid, twf);
+            NttFwd<LG, V, P + 1, false, FM>::run(x, v, tid, twf);
         }
     }
 };
@@ -315,13 +315,13 @@ struct NttFwd {
 // with the natural-order result stored in LDS (padded) and a barrier. The
 // passes after the first touch only the wave's own block (ntt_group), so
 // the first store needs no barrier after a forward transform's last pass.
-template <int LG, int V, int P>
+template <int LG, int V, int P, int FM = 0>
 struct NttInv {
     __device__ static __forceinline__ void run(uint64_t* x, uint64_t (&v)[V], uint32_t tid,
                                                const uint64_t* __restrict__ twi) {
         using Pl = NttPlan<LG, V>;
         constexpr int R = Pl::radix(P);
-        ntt_pass_regs<R, true, V, LG, P * Pl::RM>(v, tid, twi);
+        ntt_pass_regs<R, true, V, LG, P * Pl::RM, false, FM>(v, tid, twi);
         ntt_store_pass<R, V, LG, P * Pl::RM>(x, v, tid);
         if constexpr (P >= 2) {
             ntt_wave_sync();  // passes P and P - 1 both within the wave's block
@@ -331,7 +331,7 @@ struct NttInv {
         if constexpr (P > 0) {
             constexpr int R1 = Pl::radix(P - 1);
             ntt_load_pass<R1, V, LG, (P - 1) * Pl::RM>(x, v, tid);
-            NttInv<LG, V, P - 1>::run(x, v, tid, twi);
+            NttInv<LG, V, P - 1, FM>::run(x, v, tid, twi);
         }
     }
 };
