@@ -414,24 +414,28 @@ def bench_serving(mi, ctx):
 
     reqs = make_requests(mi.gen_limbs, 3000)
     serving = []
-    for qps in (1000, 16000, 64000, float("inf")):
+    inf = float("inf")
+    for qps, batched in ((1000, False), (16000, False), (64000, False), (inf, False), (64000, True), (inf, True)):
         srv = mi.Server(ctx, hbm_budget=4 << 30, streams=4)
-        res = mi.serve_poisson(srv, reqs, qps=qps, seed=int(qps) if qps != float("inf") else 1,
-                               keep_results=False)
+        res = mi.serve_poisson(srv, reqs, qps=qps, seed=int(qps) if qps != inf else 1,
+                               keep_results=False, batched=batched)
         st = srv.stats()
         srv.close()
         res.pop("results")
         res["batches"] = st["batches_launched"]
         res["mean_batch_ops"] = st["ops_in_batches"] / max(1, st["batches_launched"])
         res["deferred_admissions"] = st["deferred_admissions"]
+        res["client"] = "batched (submit_many / result_many)" if batched else "one call per request"
         serving.append(res)
-        log(f"serving {'burst' if qps == float('inf') else f'qps {qps}'}: p50 {res['p50_ms']:.2f} ms, "
-            f"p99 {res['p99_ms']:.2f} ms, {res['ops_per_s']:.0f} ops/s, {res['batches']} batches, "
-            f"{res['mean_batch_ops']:.1f} ops/batch")
+        log(f"serving {'burst' if qps == inf else f'qps {qps}'}{' batched' if batched else ''}: "
+            f"p50 {res['p50_ms']:.2f} ms, p99 {res['p99_ms']:.2f} ms, {res['ops_per_s']:.0f} ops/s, "
+            f"{res['batches']} batches, {res['mean_batch_ops']:.1f} ops/batch")
     return {"requests": len(reqs), "mix": "70% mul / 30% add, operands 8..4096 limbs log-uniform",
             "points": serving,
             "note": "offered_qps null = every request submitted at time 0 (saturation); latency = completion "
-                    "- scheduled arrival on the host clock"}
+                    "- scheduled arrival on the host clock; client 'one call per request' drives the server "
+                    "with one Python call per submit and per result, 'batched' with one call for the "
+                    "requests due at once and one per poll for the completed ones"}
 
 
 def main():

@@ -298,6 +298,15 @@ midint_status midint_server_create(midint_ctx* ctx, const midint_server_config* 
 /* Copies the operands and queues the request; *id identifies it. */
 midint_status midint_server_submit(midint_server* s, uint32_t kind, const uint64_t* a, uint32_t n_a,
                                    const uint64_t* b, uint32_t n_b, uint64_t* id);
+/* Queues n requests in one call, as n calls of midint_server_submit would:
+ * request i is kinds[i] on a[i] (n_a[i] limbs) and b[i] (n_b[i] limbs); ids[i]
+ * receives its id. Every request is checked before any is queued: on an
+ * error nothing is queued. No reference counterpart: the batched form of
+ * midint_server_submit, for drivers that submit bursts (one call instead of
+ * n from an interpreted or remote client). */
+midint_status midint_server_submit_many(midint_server* s, size_t n, const uint32_t* kinds,
+                                        const uint64_t* const* a, const uint32_t* n_a,
+                                        const uint64_t* const* b, const uint32_t* n_b, uint64_t* ids);
 /* Drives the scheduler once. Up to `cap` ids of requests completed in this
  * call go to done_ids; *n_done receives their number. */
 midint_status midint_server_poll(midint_server* s, uint64_t* done_ids, size_t cap, size_t* n_done);
@@ -306,6 +315,16 @@ midint_status midint_server_poll(midint_server* s, uint64_t* done_ids, size_t ca
  * minus submission time. MIDINT_EBUSY while it is still in flight. */
 midint_status midint_server_result(midint_server* s, uint64_t id, uint64_t* r, size_t cap_limbs, size_t* n_limbs,
                                    double* latency_s);
+/* Results of n completed requests in one call, as n calls of
+ * midint_server_result would: request ids[i]'s limbs at r + offsets[i]
+ * (offsets[i] plus its length at most cap_limbs), its length in n_limbs[i]
+ * and its latency in latency_s[i] (either array may be NULL). Every id is
+ * checked first (ENOTFOUND, EBUSY, EINVAL for a result that does not fit):
+ * on an error nothing is copied or released. No reference counterpart: the
+ * batched form of midint_server_result. */
+midint_status midint_server_result_many(midint_server* s, size_t n, const uint64_t* ids, uint64_t* r,
+                                        const uint64_t* offsets, size_t cap_limbs, size_t* n_limbs,
+                                        double* latency_s);
 midint_status midint_server_stats(const midint_server* s, midint_server_counters* out);
 midint_status midint_server_destroy(midint_server* s);
 

@@ -90,6 +90,8 @@ _SIGS = {
     "midint_server_poll": (_int, [_vp, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]),
     "midint_server_result": (_int, [_vp, ctypes.c_uint64, _vp, ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t),
                                     _dblp]),
+    "midint_server_submit_many": (_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "midint_server_result_many": (_int, [_vp, ctypes.c_size_t, _vp, _vp, _vp, ctypes.c_size_t, _vp, _vp]),
     "midint_server_stats": (_int, [_vp, _vp]),
     "midint_server_destroy": (_int, [_vp]),
     "midint_tune_set": (_int, [ctypes.c_char_p, ctypes.c_int64]),
@@ -365,6 +367,37 @@ class Server:
               "midint_server_submit")
         return rid.value
 
+    def submit_many(self, reqs):
+        """Queues [(kind, a, b)] in one call; returns their ids."""
+        return self.submit_prepared(_Batch(reqs), 0, len(reqs))
+
+    def submit_prepared(self, batch, lo, hi):
+        """Queues requests lo..hi-1 of a _Batch in one call; returns their ids."""
+        import numpy as np
+
+        n = hi - lo
+        ids = np.empty(n, dtype=np.uint64)
+        if n:
+            check(lib().midint_server_submit_many(self._h, n, batch.kinds[lo:].ctypes.data,
+                                                  batch.a_ptr[lo:].ctypes.data, batch.n_a[lo:].ctypes.data,
+                                                  batch.b_ptr[lo:].ctypes.data, batch.n_b[lo:].ctypes.data,
+                                                  ids.ctypes.data), "midint_server_submit_many")
+        return ids
+
+    def result_many(self, ids, out, offsets):
+        """Copies the results of `ids` (uint64 array) into the uint64 array
+        `out` at `offsets`; returns their latencies (seconds)."""
+        import numpy as np
+
+        ids = np.ascontiguousarray(ids, dtype=np.uint64)
+        offsets = np.ascontiguousarray(offsets, dtype=np.uint64)
+        lat = np.empty(len(ids), dtype=np.float64)
+        if len(ids):
+            check(lib().midint_server_result_many(self._h, len(ids), ids.ctypes.data, out.ctypes.data,
+                                                  offsets.ctypes.data, out.size, None, lat.ctypes.data),
+                  "midint_server_result_many")
+        return lat
+
     def poll(self):
         n = ctypes.c_size_t()
         check(lib().midint_server_poll(self._h, self._ids, len(self._ids), ctypes.byref(n)), "midint_server_poll")
@@ -397,13 +430,33 @@ class Server:
             pass
 
 
-def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s=120.0):
+class _Batch:
+    """Requests [(kind, a, b)] as the arrays midint_server_submit_many takes
+    (the operand arrays must stay alive while it is in use)."""
+
+    def __init__(self, reqs):
+        import numpy as np
+
+        self.reqs = reqs
+        self.kinds = np.array([k for k, _, _ in reqs], dtype=np.uint32)
+        self.a_ptr = np.array([a.ctypes.data for _, a, _ in reqs], dtype=np.uint64)
+        self.b_ptr = np.array([b.ctypes.data for _, _, b in reqs], dtype=np.uint64)
+        self.n_a = np.array([a.size for _, a, _ in reqs], dtype=np.uint32)
+        self.n_b = np.array([b.size for _, _, b in reqs], dtype=np.uint32)
+
+
+def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s=120.0, batched=False):
     """Drives `server` with `requests` [(kind, a, b)] arriving as a Poisson
     stream of rate `qps` (qps = inf: every request arrives at time 0, a
     burst that saturates the server). Latency of a request = completion -
     scheduled arrival (host monotonic clock; includes time waiting in the
     queue). Returns dict(p50_ms, p99_ms, mean_ms, ops_per_s, offered_qps
-    (None for a burst), results{index: limbs})."""
+    (None for a burst), results{index: limbs}).
+    batched: the requests due at once go in with one submit_many call and
+    the completed ones come back with one result_many call per poll (the
+    operand pointer arrays and the result buffer are prepared before the
+    clock starts), so a burst measures the server rather than one
+    interpreted call per request."""
     import math
     import time
 
@@ -416,6 +469,11 @@ def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s
     out_len = [(len(a) + len(b)) if k == 1 else max(len(a), len(b)) + 1 for k, a, b in requests]
     id2idx, lat, results = {}, np.zeros(n), {}
     submit_late = np.zeros(n)
+    if batched:
+        batch = _Batch(requests)
+        out_off = np.concatenate([[0], np.cumsum(out_len)[:-1]]).astype(np.uint64)
+        out_buf = np.empty(int(sum(out_len)), dtype=np.uint64)
+        out_buf.fill(0)  # touch the pages before the clock starts
     i = done = 0
     t0 = time.perf_counter()
     last = t0
@@ -423,6 +481,25 @@ def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s
         now = time.perf_counter() - t0
         if now > time_limit_s:
             raise RuntimeError(f"serve_poisson: {done}/{n} done after {time_limit_s} s")
+        if batched:
+            j = i
+            while j < n and arrivals[j] <= now:
+                j += 1
+            if j > i:
+                ids = server.submit_prepared(batch, i, j)
+                t_sub = time.perf_counter() - t0
+                for q, rid in enumerate(ids.tolist()):
+                    id2idx[rid] = i + q
+                submit_late[i:j] = t_sub - arrivals[i:j]
+                i = j
+            got = server.poll()
+            if got:
+                idx = np.array([id2idx.pop(rid) for rid in got], dtype=np.int64)
+                l_c = server.result_many(np.array(got, dtype=np.uint64), out_buf, out_off[idx])
+                lat[idx] = submit_late[idx] + l_c
+                done += len(got)
+                last = time.perf_counter()
+            continue
         while i < n and arrivals[i] <= now:
             k, a, b = requests[i]
             rid = server.submit(k, a, b)
@@ -437,6 +514,8 @@ def serve_poisson(server, requests, qps, seed=0, keep_results=True, time_limit_s
                 results[j] = r
             done += 1
             last = time.perf_counter()
+    if batched and keep_results:
+        results = {q: out_buf[int(out_off[q]):int(out_off[q]) + out_len[q]].copy() for q in range(n)}
     span = last - (t0 + arrivals[0])
     return {"p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
             "mean_ms": float(lat.mean() * 1e3), "ops_per_s": n / span,

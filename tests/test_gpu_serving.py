@@ -52,6 +52,59 @@ def test_saturated_burst_batches_and_matches(mi, ctx, orc):
         assert np.array_equal(alone, out["results"][j])
 
 
+@pytest.mark.parametrize("qps", [float("inf"), 30000.0], ids=["burst", "stream"])
+def test_batched_submit_and_results(mi, ctx, orc, qps):
+    """The batched entry points (midint_server_submit_many / _result_many):
+    a burst and a Poisson stream driven through them give every request the
+    oracle's result and the result of the same request served alone through
+    the single-request calls."""
+    reqs = make_requests(mi.gen_limbs, 500, seed=33)
+    srv = mi.Server(ctx, hbm_budget=512 << 20, streams=4)
+    out = mi.serve_poisson(srv, reqs, qps=qps, seed=11, batched=True)
+    st = srv.stats()
+    srv.close()
+    assert st["completed"] == len(reqs) and st["queued"] == 0
+    for j, (k, a, b) in enumerate(reqs):
+        assert np.array_equal(out["results"][j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
+    for j in (0, 250, 499):
+        solo = mi.Server(ctx, streams=1)
+        alone = mi.serve_poisson(solo, [reqs[j]], qps=1000, seed=3)["results"][0]
+        solo.close()
+        assert np.array_equal(alone, out["results"][j])
+
+
+def test_batched_entry_points_reject_without_side_effects(mi, ctx):
+    """submit_many checks every request before queueing any; result_many
+    rejects unknown, unfinished and repeated ids before copying or releasing
+    any result."""
+    import ctypes
+
+    srv = mi.Server(ctx, streams=2)
+    good = (1, np.arange(1, 9, dtype=np.uint64), np.arange(1, 5, dtype=np.uint64))
+    bad = (1, np.ones(5000, dtype=np.uint64), np.ones(4, dtype=np.uint64))  # above the 4096-limb limit
+    with pytest.raises(mi.MidintError):
+        srv.submit_many([good, bad])
+    assert srv.stats()["submitted"] == 0
+    ids = srv.submit_many([good, good])
+    done = []
+    while len(done) < 2:
+        done += srv.poll()
+    out = np.zeros(64, dtype=np.uint64)
+    with pytest.raises(mi.MidintError):  # the same id twice
+        srv.result_many(np.array([ids[0], ids[0]], dtype=np.uint64), out, [0, 16])
+    with pytest.raises(mi.MidintError):  # an unknown id
+        srv.result_many(np.array([ids[0], 999999], dtype=np.uint64), out, [0, 16])
+    lat = srv.result_many(np.array(ids, dtype=np.uint64), out, [0, 16])  # both still there
+    assert len(lat) == 2 and (lat >= 0).all()
+    want = int("".join(f"{int(x):016x}" for x in good[1][::-1]), 16) * int(
+        "".join(f"{int(x):016x}" for x in good[2][::-1]), 16)
+    for off in (0, 16):
+        got = sum(int(out[off + q]) << (64 * q) for q in range(12))
+        assert got == want
+    srv.close()
+    del ctypes
+
+
 def test_alone_equals_mixed(mi, ctx, orc):
     reqs = make_requests(mi.gen_limbs, 120, seed=9)
     srv = mi.Server(ctx, streams=3)
