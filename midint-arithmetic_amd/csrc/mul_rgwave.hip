@@ -35,7 +35,7 @@
 //    column of the rows computed, one B more in column 0, B 2^(64 P) = 0
 //    modulo the output length for any P >= n_a + n_b limbs.
 //
-// Per wave 7280 bytes of LDS: the ring (4096), b's digits reversed (1120),
+// Per wave 7280 bytes of LDS: b's digits reversed (1120), the ring (4096),
 // the words V of one row after a 16-byte prefix (2064). Each wave owns its
 // region and one op at a time, so waves never synchronise.
 //
@@ -54,11 +54,11 @@ constexpr int kRgwDL = 48;        // DL
 constexpr uint32_t kRgwWaves = 4;  // waves (independent ops) per workgroup
 constexpr int kRgwSmallK = 9;      // the small-fragment kernel: K <= 9 (n_b <= 70)
 
-// Per-wave LDS layout of the kernel for fragment counts K <= KMAX: the ring
-// of a's digit rows (row R's windows reach back at most 64 KMAX - 16
-// digits: two slots for KMAX <= 9, four up to 17), b's digits reversed
-// (digit j at byte X0 - j, X0 = 15 mod 16, bytes [X0 - 64 KMAX + 1, X0 + 20)
-// read), the words V of one row after a 16-byte prefix.
+// Per-wave LDS layout of the kernel for fragment counts K <= KMAX: b's
+// digits reversed (digit j at byte X0 - j, X0 = 15 mod 16, bytes
+// [X0 - 64 KMAX + 1, X0 + 20) read), the ring of a's digit rows (row R's
+// windows reach back at most 64 KMAX - 16 digits: two slots for KMAX <= 9,
+// four up to 17), the words V of one row after a 16-byte prefix.
 template <int KMAX>
 struct RgwCfg {
     static constexpr uint32_t RING = KMAX <= 9 ? 2048 : 4096;
@@ -97,8 +97,8 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     using G = RgwCfg<KMAX>;
     constexpr uint32_t kRing = G::RING;
     constexpr int32_t kX0 = G::X0;
-    uint8_t* ring = base;
-    uint8_t* breg = base + kRing;
+    uint8_t* breg = base;  // first, so the fragment reads' offsets fit ds_read2_b32's offset fields
+    uint8_t* ring = base + G::BR;
     uint8_t* vreg = base + G::VOFF;
     const uint32_t na = it.n_a, nb = it.n_b, pl = na + nb;
     const uint32_t ln = lane & 15, lg = lane >> 4;

@@ -124,9 +124,11 @@ struct W16 {
     static constexpr int AOFF = kW16Delta - 64 * P.umin;
     static constexpr int A_END0 = AOFF + 16 * 18 + 64 * P.umax - kW16Delta + 16;
     static constexpr int A_BYTES = w16_round16(A_END0 > AOFF + 8 * M + 8 ? A_END0 : AOFF + 8 * M + 8);
-    // br region (after the a region): b's digit j at byte X0 - j; the A
-    // fragments read five dwords from dword (X0 - 64k - DL - r + 16g) / 4;
-    // X0 = 15 mod 16 puts b's limb pairs on 16-byte boundaries
+    // br region (first in the wave's region, so that the A fragments' reads
+    // of step k sit within the 1020-byte offset fields of ds_read2_b32 from
+    // one base register): b's digit j at byte X0 - j; the A fragments read
+    // five dwords from dword (X0 - 64k - DL - r + 16g) / 4; X0 = 15 mod 16
+    // puts b's limb pairs on 16-byte boundaries
     static constexpr int X0_MIN0 = 64 * P.khi + kW16Delta + 15;
     static constexpr int X0_MIN = X0_MIN0 > 8 * M + 7 ? X0_MIN0 : 8 * M + 7;
     static constexpr int X0 = ((X0_MIN - 15 + 15) & ~15) + 15;
@@ -167,8 +169,8 @@ struct W16Wave {
     // (written once: the products only overwrite digit bytes and words).
     __device__ __forceinline__ void init(uint8_t* base, uint32_t pow8) {
         lane = threadIdx.x & 63;
-        areg = base;
-        breg = base + G::A_BYTES;
+        breg = base;
+        areg = base + G::BR_BYTES;
         vreg = base + G::V_OFF;
         for (uint32_t x = 16 * lane; x < (uint32_t)G::V_OFF; x += 1024)
             *reinterpret_cast<uint4*>(base + x) = make_uint4(0u, 0u, 0u, 0u);
