@@ -354,10 +354,13 @@ __global__ __launch_bounds__((1u << LG) / V) void ntt_fast_kernel(uint32_t m, ui
     ntt_store_rows<V / 8>(rows, r + (size_t)blockIdx.x * rl, rl);  // rl = 2m (full) or m (low half)
 }
 
-// The polynomial kernel holds two transformed operands in registers; its
-// field multiplications reduce on the carry chain alone (gl_mulv<1>), whose
-// fewer live registers keep it within 128 VGPRs without spilling.
-constexpr int kPolyFM = 1;
+// The polynomial kernel's field multiplications reduce through the
+// multiply-add (gl_mulv<0>), except in the inverse transform of a^2, which
+// runs while NTT(a) is held for ab: there the carry-chain reduction
+// (gl_mulv<1>), with fewer live registers, keeps the kernel within 128
+// VGPRs without spilling (126; the multiply-add form everywhere spills 6).
+constexpr int kPolyFM = 0;
+constexpr int kPolyHeldFM = 1;
 
 // The reference's polyFttKer (cuda/ker-fft-mul.cu.h:456-509) in one kernel:
 // r = ((a^2 + b)(b^2 + b) + ab) mod B^m per instance (one per workgroup),
@@ -395,7 +398,7 @@ __global__ __launch_bounds__((1u << LG) / V) void poly_ntt_kernel(uint32_t m, co
 #pragma unroll
     for (int k = 0; k < V; k++) Q[k] = gl_mul_pow2(gl_mulv<kPolyFM>(P[k], P[k]), LG);
     ntt_wave_sync();  // the last forward pass and the first inverse store: the wave's own block
-    NttInv<LG, V, Pl::NP - 1, kPolyFM>::run(x, Q, opaque_u32(threadIdx.x), twi);
+    NttInv<LG, V, Pl::NP - 1, kPolyHeldFM>::run(x, Q, opaque_u32(threadIdx.x), twi);
     ntt_recombine_rows<LG, V>(x, m, rows, lds_gp);
     ntt_rows_add<R>(rows, gb, m, lds_gp);
     ntt_store_rows<R>(rows, s1, m);
