@@ -276,8 +276,10 @@ midint_status midint_ragged_plan_destroy(midint_ragged_plan* plan);
  * that do not fit wait; none is dropped), and launches them on free streams,
  * so later requests overlap on the device with earlier ones. Results are
  * exact (full sums and products) and independent of batching. One thread
- * drives a server. No reference counterpart (the reference has no serving
- * path). */
+ * drives a server; inside a call that copies more than about 1 MB of
+ * operands or results (a large micro-batch, submit_many, result_many) the
+ * library splits the host copies over up to 8 short-lived threads. No
+ * reference counterpart (the reference has no serving path). */
 typedef struct {
     uint64_t hbm_budget_bytes; /* device pool for in-flight batches (0: 1 GiB) */
     uint64_t max_batch_bytes;  /* device bytes of one micro-batch (0: 64 MiB) */
