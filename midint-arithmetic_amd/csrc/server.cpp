@@ -25,9 +25,12 @@
 // Several micro-batches are in flight on different streams, so work of
 // requests admitted later overlaps on the device with work admitted earlier.
 // Kernels per micro-batch: for ops whose operands are <= 1024 limbs, the
-// matrix-core kernel (one product per workgroup) for multiplications whose
+// ragged plans' kernels (ragged.wave = 1, the default: the additions and
+// the products whose shorter operand has at most 128 limbs one op per wave,
+// the other products on matrix-core kernels with tiles and waves per size
+// class; ragged.wave = 0: the matrix-core kernel for multiplications whose
 // shorter operand has at least ragged.mfma_min limbs and the ragged
-// persistent kernel for the rest; the add kernel for larger additions (padded
+// persistent kernel for the rest); the add kernel for larger additions (padded
 // to a 2048- or 4096-limb class, carry-out kept); the matrix-core kernel for
 // larger products (one workgroup each up to 2048 limbs, or four full
 // 2048-limb quarter products plus the combine kernel above 2048 limbs; the
@@ -349,44 +352,111 @@ midint_status launch_batch(midint_server* s, size_t si) {
     out += n_ntt * 2 * kNttM;
     const size_t carry_bytes = align_up(add2k.size() + add4k.size(), 8);
     const size_t items_off = in;  // limbs
-    // ragged multiplications whose shorter operand has at least
-    // ragged.mfma_min limbs go to the matrix-core kernel (one product per
-    // workgroup), the rest to the persistent kernel
-    const int64_t mf_min = tune_get("ragged.mfma_min", 32);
+    // ragged ops, as the ragged plans run them: with ragged.wave = 1 (the
+    // default) the additions and the products whose shorter operand has at
+    // most rgw_max_short() limbs one op per wave, the other products on
+    // matrix-core kernels (tiles and waves per size class: ragged.m16@C,
+    // ragged.m16_waves@C, ragged.mfma_waves@C; one launch per tile shape);
+    // with ragged.wave = 0 the products whose shorter operand has at least
+    // ragged.mfma_min limbs on one matrix-core launch and the rest on the
+    // persistent kernel
+    const bool by_wave = tune_get("ragged.wave", 1) != 0;
+    const int64_t mf_min = by_wave ? (int64_t)rgw_max_short() + 1 : tune_get("ragged.mfma_min", 32);
     std::vector<midint_op> pack_ops;
-    std::vector<size_t> mf_idx;
-    uint32_t mf_na = 0, mf_nb = 0;
+    std::vector<size_t> wv_add_idx, wv_mul_idx;
+    struct MfGroup {
+        std::vector<size_t> idx;
+        uint32_t na = 0, nb = 0, waves = kMfRgWaves;
+        MfRgConfig cfg = {};
+        size_t meta_off = 0;
+    };
+    MfGroup mfg[2];  // by_wave: [0] 32 x 32 x 32 tiles, [1] 16 x 16 x 64; else [0] only
     for (size_t k = 0; k < rops.size(); k++) {
         const midint_op& o = rops[k];
-        if (o.kind == MIDINT_OP_MUL && mf_min > 0 && std::min(o.n_a, o.n_b) >= (uint64_t)mf_min) {
-            mf_idx.push_back(k);
-            mf_na = std::max(mf_na, (uint32_t)std::max(o.n_a, o.n_b));
-            mf_nb = std::max(mf_nb, (uint32_t)std::min(o.n_a, o.n_b));
+        const uint32_t nl = (uint32_t)std::max(o.n_a, o.n_b), ns = (uint32_t)std::min(o.n_a, o.n_b);
+        if (by_wave && o.kind == MIDINT_OP_ADD) {
+            wv_add_idx.push_back(k);
+        } else if (by_wave && o.kind == MIDINT_OP_MUL && ns <= rgw_max_short()) {
+            wv_mul_idx.push_back(k);
+        } else if (o.kind == MIDINT_OP_MUL && mf_min > 0 && ns >= (uint64_t)mf_min) {
+            int g = 0;
+            if (by_wave) {
+                const int64_t mode = tune_get_class("ragged.m16", nl, 0);
+                const int64_t ratio = std::max<int64_t>(1, tune_get_class("ragged.m16_ratio", nl, 2));
+                g = (mode == 1 || (mode == 2 && (uint64_t)ns * ratio >= nl)) ? 1 : 0;
+                // waves: the largest class's setting
+                if (nl > mfg[g].na)
+                    mfg[g].waves = (uint32_t)tune_get_class(g ? "ragged.m16_waves" : "ragged.mfma_waves", nl,
+                                                            kMfRgWaves);
+            }
+            mfg[g].idx.push_back(k);
+            mfg[g].na = std::max(mfg[g].na, nl);
+            mfg[g].nb = std::max(mfg[g].nb, ns);
         } else {
             pack_ops.push_back(o);
         }
     }
-    MfRgConfig mf = {};
-    if (!mf_idx.empty() && !mfma_rg_config(mf_na, mf_nb, mf_na + mf_nb, &mf)) {
-        for (size_t k : mf_idx) pack_ops.push_back(rops[k]);  // no fitting configuration
-        mf_idx.clear();
+    for (int g = 0; g < 2; g++) {
+        MfGroup& G = mfg[g];
+        if (G.idx.empty()) continue;
+        bool ok;
+        if (!by_wave) {
+            ok = mfma_rg_config(G.na, G.nb, G.na + G.nb, &G.cfg);
+        } else {
+            auto cfg = [&](uint32_t w) {
+                return g ? mfma16_rg_config(G.na, G.nb, G.na + G.nb, &G.cfg, w)
+                         : mfma32_rg_config(G.na, G.nb, G.na + G.nb, &G.cfg, w);
+            };
+            ok = (G.waves == 1 || G.waves == 2 || G.waves == 4 || G.waves == 8) && cfg(G.waves);
+            if (!ok) ok = cfg(kMfRgWaves);
+        }
+        if (!ok) {  // no fitting configuration: the persistent kernel takes them
+            for (size_t k : G.idx) pack_ops.push_back(rops[k]);
+            G.idx.clear();
+            continue;
+        }
+        std::stable_sort(G.idx.begin(), G.idx.end(), [&](size_t x, size_t y) {
+            return (uint64_t)rops[x].n_a * rops[x].n_b > (uint64_t)rops[y].n_a * rops[y].n_b;
+        });
     }
-    std::stable_sort(mf_idx.begin(), mf_idx.end(), [&](size_t x, size_t y) {
-        return (uint64_t)rops[x].n_a * rops[x].n_b > (uint64_t)rops[y].n_a * rops[y].n_b;
+    // one-op-per-wave items, the costliest first (as in the ragged plans)
+    auto wv_cost = [&](size_t k) {
+        const uint64_t nl = std::max(rops[k].n_a, rops[k].n_b), ns = std::min(rops[k].n_a, rops[k].n_b);
+        return ((nl + ns + 127) / 128) * ((8 * ns + 15) / 64 + 1);
+    };
+    std::stable_sort(wv_mul_idx.begin(), wv_mul_idx.end(), [&](size_t x, size_t y) { return wv_cost(x) > wv_cost(y); });
+    std::stable_sort(wv_add_idx.begin(), wv_add_idx.end(), [&](size_t x, size_t y) {
+        return std::max(rops[x].n_a, rops[x].n_b) > std::max(rops[y].n_a, rops[y].n_b);
     });
     midint_status ps = ragged_pack(pack_ops.data(), pack_ops.size(), 0, 0, 0, items, rgops, &conv_cap);
     if (ps != MIDINT_OK) return ps;
     // products above the ragged limit: matrix-core items over the padded
     // 2048-limb operand slots (one per 2048-class product, four full-size
-    // quarter products per split product), with the NTT kernel as the
+    // quarter products per split product) on the tiles and waves the
+    // uniform 2048-limb batches use (mul.m16@2048, default 16 x 16 x 64;
+    // mul.m16_waves@2048, default 8 waves), with the NTT kernel as the
     // fallback when no configuration fits
     MfRgConfig bigmf = {};
-    const bool big_mf = n_ntt > 0 && mfma_rg_config(kNttM, kNttM, 2 * kNttM, &bigmf);
-    // metadata: [RgItem | RgOp | MfRgItem (16-byte aligned) | big MfRgItem | counter limb]
-    const size_t mf_meta_off = align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 16);
-    const size_t big_meta_off = mf_meta_off + mf_idx.size() * sizeof(MfRgItem);
+    bool big_mf = false;
+    if (n_ntt > 0) {
+        if (tune_get_class("mul.m16", kNttM, 1) != 0) {
+            const int64_t w = tune_get_class("mul.m16_waves", kNttM, 0);
+            big_mf = mfma16_rg_config(kNttM, kNttM, 2 * kNttM, &bigmf, w > 0 && w <= 16 ? (uint32_t)w : 8u);
+        }
+        if (!big_mf) big_mf = mfma_rg_config(kNttM, kNttM, 2 * kNttM, &bigmf);
+    }
+    // metadata: [RgItem | RgOp | MfRgItem groups (16-byte aligned) | big MfRgItem | wave additions | wave
+    // products | counter limb]
+    size_t meta_at = align_up(items.size() * sizeof(RgItem) + rgops.size() * sizeof(RgOp), 16);
+    for (MfGroup& G : mfg) {
+        G.meta_off = meta_at;
+        meta_at += G.idx.size() * sizeof(MfRgItem);
+    }
+    const size_t big_meta_off = meta_at;
+    const size_t wv_add_off = big_meta_off + (big_mf ? n_ntt : 0) * sizeof(MfRgItem);
+    const size_t wv_mul_off = wv_add_off + wv_add_idx.size() * sizeof(MfRgItem);
     const size_t meta_limbs =
-        align_up(big_meta_off + (big_mf ? n_ntt : 0) * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
+        align_up(wv_mul_off + wv_mul_idx.size() * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
     in += meta_limbs;
     in = align_up(in, 32);
     const size_t in_bytes = in * 8;
@@ -479,16 +549,33 @@ midint_status launch_batch(midint_server* s, size_t si) {
     memcpy(meta, items.data(), items.size() * sizeof(RgItem));
     memcpy(meta + items.size() * sizeof(RgItem), rgops.data(), rgops.size() * sizeof(RgOp));
     {
-        // matrix-core items: device addresses inside this batch's region,
-        // the longer operand first
-        MfRgItem* mfi = reinterpret_cast<MfRgItem*>(meta + mf_meta_off);
-        for (size_t q = 0; q < mf_idx.size(); q++) {
-            const midint_op& o = rops[mf_idx[q]];
+        // matrix-core and one-per-wave product items: device addresses inside
+        // this batch's region, the longer operand first
+        auto product_item = [&](size_t k) {
+            const midint_op& o = rops[k];
             const bool sw = o.n_a < o.n_b;
             const uint64_t* x = sw ? d_in + rg_b + o.off_b : d_in + rg_a + o.off_a;
             const uint64_t* y = sw ? d_in + rg_a + o.off_a : d_in + rg_b + o.off_b;
-            mfi[q] = mfma_rg_item(x, y, d_out + rg_r + o.off_r, (uint32_t)(sw ? o.n_b : o.n_a),
-                                  (uint32_t)(sw ? o.n_a : o.n_b));
+            return mfma_rg_item(x, y, d_out + rg_r + o.off_r, (uint32_t)(sw ? o.n_b : o.n_a),
+                                (uint32_t)(sw ? o.n_a : o.n_b));
+        };
+        for (const MfGroup& G : mfg) {
+            MfRgItem* mfi = reinterpret_cast<MfRgItem*>(meta + G.meta_off);
+            for (size_t q = 0; q < G.idx.size(); q++) mfi[q] = product_item(G.idx[q]);
+        }
+        MfRgItem* wvm = reinterpret_cast<MfRgItem*>(meta + wv_mul_off);
+        for (size_t q = 0; q < wv_mul_idx.size(); q++) wvm[q] = product_item(wv_mul_idx[q]);
+        // additions in the op's order of operands
+        MfRgItem* wva = reinterpret_cast<MfRgItem*>(meta + wv_add_off);
+        for (size_t q = 0; q < wv_add_idx.size(); q++) {
+            const midint_op& o = rops[wv_add_idx[q]];
+            MfRgItem it = {};
+            it.a = d_in + rg_a + o.off_a;
+            it.b = d_in + rg_b + o.off_b;
+            it.r = d_out + rg_r + o.off_r;
+            it.n_a = (uint32_t)o.n_a;
+            it.n_b = (uint32_t)o.n_b;
+            wva[q] = it;
         }
         if (big_mf) {
             // instance inst: operands at X / Y + inst kNttM (zero-padded),
@@ -532,14 +619,26 @@ midint_status launch_batch(midint_server* s, size_t si) {
         if (e == hipSuccess)
             e = hipLaunchKernel(ragged_kernel_ptr(256), dim3(grid), dim3(256), args, lds, st);
     }
-    if (e == hipSuccess && !mf_idx.empty()) {
-        MfRgItem* d_mf = reinterpret_cast<MfRgItem*>(reinterpret_cast<uint8_t*>(d_in + items_off) + mf_meta_off);
-        uint32_t pow8 = mf.pow8;
+    uint8_t* d_meta = reinterpret_cast<uint8_t*>(d_in + items_off);
+    for (const MfGroup& G : mfg) {
+        if (e != hipSuccess || G.idx.empty()) continue;
+        MfRgItem* d_mf = reinterpret_cast<MfRgItem*>(d_meta + G.meta_off);
+        uint32_t pow8 = G.cfg.pow8;
         void* args[] = {(void*)&d_mf, (void*)&pow8};
-        if (mf.lds > 64 * 1024)
-            e = hipFuncSetAttribute(mf.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)mf.lds);
+        if (G.cfg.lds > 64 * 1024)
+            e = hipFuncSetAttribute(G.cfg.func, hipFuncAttributeMaxDynamicSharedMemorySize, (int)G.cfg.lds);
         if (e == hipSuccess)
-            e = hipLaunchKernel(mf.func, dim3((uint32_t)mf_idx.size()), dim3(mf.threads), args, mf.lds, st);
+            e = hipLaunchKernel(G.cfg.func, dim3((uint32_t)G.idx.size()), dim3(G.cfg.threads), args, G.cfg.lds, st);
+    }
+    for (int k = 0; k < 2 && e == hipSuccess; k++) {  // one op per wave: products, additions
+        const bool mul = k == 0;
+        uint32_t n_wv = (uint32_t)(mul ? wv_mul_idx.size() : wv_add_idx.size());
+        if (n_wv == 0) continue;
+        MfRgItem* d_wv = reinterpret_cast<MfRgItem*>(d_meta + (mul ? wv_mul_off : wv_add_off));
+        uint32_t pow8 = 256;
+        void* args[] = {(void*)&d_wv, (void*)&n_wv, (void*)&pow8};
+        e = hipLaunchKernel(mul ? rgw_mul_kernel_ptr(0) : rgw_add_kernel_ptr(), dim3((n_wv + rgw_waves() - 1) / rgw_waves()),
+                            dim3(64 * rgw_waves()), args, mul ? rgw_lds_bytes(0) : 0, st);
     }
     if (e == hipSuccess && !add2k.empty())
         e = launch_add((uint32_t)add2k.size(), 2048, d_in + a2k, d_in + a2k + add2k.size() * 2048, d_out + a2k_r,
@@ -660,8 +759,9 @@ void retire(midint_server* s, Slot& sl, double t, std::vector<uint64_t>& done) {
 // afterwards, so a fresh server reports only live traffic.
 midint_status warm_up(midint_server* s) {
     const uint32_t shapes[][3] = {{MIDINT_OP_MUL, 8, 8},       {MIDINT_OP_MUL, 64, 64},  {MIDINT_OP_MUL, 1024, 1024},
-                                  {MIDINT_OP_ADD, 8, 8},       {MIDINT_OP_ADD, 2048, 8}, {MIDINT_OP_ADD, 4096, 8},
-                                  {MIDINT_OP_MUL, 2048, 8},    {MIDINT_OP_MUL, 4096, 8}};
+                                  {MIDINT_OP_MUL, 200, 150},   {MIDINT_OP_MUL, 300, 200}, {MIDINT_OP_ADD, 8, 8},
+                                  {MIDINT_OP_ADD, 2048, 8},    {MIDINT_OP_ADD, 4096, 8},  {MIDINT_OP_MUL, 2048, 8},
+                                  {MIDINT_OP_MUL, 4096, 8}};
     std::vector<uint64_t> ones(kServeMaxLimbs, 1);
     std::vector<uint64_t> done;
     for (size_t si = 0; si < s->slots.size(); si++) {

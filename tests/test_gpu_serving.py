@@ -153,19 +153,34 @@ def test_budget_defers_not_drops(mi, ctx, orc):
         assert np.array_equal(out["results"][j], request_reference(orc, k, a, b))
 
 
-@pytest.mark.parametrize("mfma_min", [0, 1])
-def test_matrix_core_route_threshold(mi, ctx, orc, mfma_min):
-    """Multiplications up to 1024 limbs whose shorter operand has at least
-    ragged.mfma_min limbs run on the matrix-core kernel inside the micro-batch
-    (0: none, 1: all); results equal the oracle's either way."""
+@pytest.mark.parametrize("wave,mfma_min,m16", [(0, 0, 0), (0, 1, 0), (1, 32, 0), (1, 32, 1), (1, 32, 2)])
+def test_ragged_route_kernels(mi, ctx, orc, wave, mfma_min, m16):
+    """Requests with both operands up to 1024 limbs run inside the
+    micro-batch on the ragged plans' kernels: with ragged.wave = 1 the
+    additions and the products whose shorter operand has at most 128 limbs
+    one op per wave, the other products on the matrix-core kernels whose
+    tiles ragged.m16@C picks per size class (0: 32 x 32 x 32, 1: 16 x 16 x
+    64, 2: by shape); with ragged.wave = 0 the products whose shorter operand
+    has at least ragged.mfma_min limbs on the matrix-core kernel (0: none,
+    1: all) and the rest on the persistent kernel. Results equal the
+    oracle's every way."""
     reqs = make_requests(mi.gen_limbs, 150, seed=17)
-    old = mi.tune_get("ragged.mfma_min")
-    mi.tune_set("ragged.mfma_min", mfma_min)
+    keys = {"ragged.wave": wave, "ragged.mfma_min": mfma_min}
+    for c in (256, 512, 1024):
+        keys[f"ragged.m16@{c}"] = m16
+    saved = {}
+    for k, v in keys.items():
+        try:
+            saved[k] = mi.tune_get(k)
+        except Exception:
+            saved[k] = mi.tune_get(k.split("@")[0])
+        mi.tune_set(k, v)
     try:
         srv = mi.Server(ctx, streams=2)
         out = mi.serve_poisson(srv, reqs, qps=20000, seed=6)["results"]
         srv.close()
     finally:
-        mi.tune_set("ragged.mfma_min", old)
+        for k, v in saved.items():
+            mi.tune_set(k, v)
     for j, (k, a, b) in enumerate(reqs):
         assert np.array_equal(out[j], request_reference(orc, k, a, b)), (j, k, a.size, b.size)
