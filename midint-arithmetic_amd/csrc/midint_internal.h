@@ -208,6 +208,23 @@ const void* ragged_reset_ptr();
 hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s);
 // Copies `bytes` (a multiple of 8; both pointers 16-byte aligned) with a kernel on `s`.
 hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// Serving loop: zero-padded slots built on the device from packed operands
+// (dst[0, n) = src[0, n), zeros up to size), and results gathered from
+// their padded outputs into a packed area (carry: the top limb is that byte
+// when set); one workgroup per slot or result.
+struct ServeSlot {
+    const uint64_t* src;
+    uint64_t* dst;
+    uint32_t n, size;
+};
+struct ServeGather {
+    const uint64_t* src;
+    uint64_t* dst;
+    const uint8_t* carry;
+    uint32_t n, pad;
+};
+hipError_t launch_serve_expand(uint32_t n, const ServeSlot* slots, hipStream_t s);
+hipError_t launch_serve_gather(uint32_t n, const ServeGather* items, hipStream_t s);
 
 // Autotuned launch parameter (see tuning.cpp), or dflt for unknown keys.
 int64_t tune_get(const char* key, int64_t dflt);

@@ -9,6 +9,12 @@
 // additions on register-resident rows (6144 limbs = 16 waves x 3 rows).
 //
 // stage_copy: the batch's host<->HBM transfers (below).
+//
+// serve_expand / serve_gather: the operands of requests above the ragged
+// size limit cross the bus packed (their limbs only) and are spread into
+// their zero-padded class slots on the device; their results are gathered
+// from the padded outputs into one packed area, which is all that goes
+// back.
 #include <algorithm>
 
 #include "add_core.hpp"
@@ -92,6 +98,34 @@ hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream
     const size_t blocks = std::min<size_t>(1024, std::max<size_t>(1, (n16 + 1023) / 1024));
     stage_copy_kernel<<<(uint32_t)blocks, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16,
                                                         (uint32_t)(bytes % 16 != 0));
+    return hipGetLastError();
+}
+
+
+// One workgroup per slot: dst[0, n) = src[0, n), dst[n, size) = 0.
+__global__ __launch_bounds__(256) void serve_expand_kernel(const ServeSlot* __restrict__ slots) {
+    const ServeSlot sl = slots[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < sl.size; i += 256) sl.dst[i] = i < sl.n ? sl.src[i] : 0ull;
+}
+
+// One workgroup per result: dst[0, n) = src[0, n), except that with a carry
+// byte the top limb dst[n - 1] is that byte.
+__global__ __launch_bounds__(256) void serve_gather_kernel(const ServeGather* __restrict__ items) {
+    const ServeGather it = items[blockIdx.x];
+    const uint32_t m = it.carry ? it.n - 1 : it.n;
+    for (uint32_t i = threadIdx.x; i < m; i += 256) it.dst[i] = it.src[i];
+    if (it.carry && threadIdx.x == 0) it.dst[m] = *it.carry;
+}
+
+hipError_t launch_serve_expand(uint32_t n, const ServeSlot* slots, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    serve_expand_kernel<<<n, 256, 0, s>>>(slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_serve_gather(uint32_t n, const ServeGather* items, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    serve_gather_kernel<<<n, 256, 0, s>>>(items);
     return hipGetLastError();
 }
 

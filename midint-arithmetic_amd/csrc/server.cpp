@@ -121,13 +121,12 @@ struct Request {
     bool done = false;
 };
 
-// Where a request's result sits in the batch's output staging.
+// Where a request's result sits in the batch's output staging (the packed
+// results that come back).
 struct ResultRef {
     uint64_t id;
     size_t out_off;  // limbs from the start of the output area
     uint32_t len;    // limbs of the result
-    int carry_idx;   // >= 0: result limb len-1 is this carry byte (big adds at the class size)
-    uint32_t carry_class;
 };
 
 enum class Route { Ragged, AddClass2048, AddClass4096, Ntt2048, Ntt4096 };
@@ -139,16 +138,19 @@ Route route_of(uint32_t kind, uint32_t na, uint32_t nb) {
     return mx <= kNttM ? Route::Ntt2048 : Route::Ntt4096;
 }
 
-// Device bytes a request needs inside a batch region (inputs + outputs,
-// upper bound including its share of the ragged metadata).
+// Device bytes a request needs inside a batch region (upper bound including
+// its share of the metadata): ragged ops their packed operands and result;
+// the others their packed operands, their zero-padded class slots, the
+// class-size outputs, the packed result and their slot / result entries.
 size_t request_bytes(uint32_t kind, uint32_t na, uint32_t nb) {
+    constexpr size_t kBigMeta = 8 * sizeof(ServeSlot) + sizeof(ServeGather) + sizeof(MfRgItem);
     switch (route_of(kind, na, nb)) {
         case Route::Ragged:
             return 8ull * (na + nb + (kind == MIDINT_OP_MUL ? na + nb : std::max(na, nb) + 1)) + 2 * sizeof(RgOp);
-        case Route::AddClass2048: return 8ull * 3 * 2048 + 1;
-        case Route::AddClass4096: return 8ull * 3 * 4096 + 1;
-        case Route::Ntt2048: return 8ull * (2 * kNttM + 2 * kNttM);
-        case Route::Ntt4096: return 4 * 8ull * (2 * kNttM + 2 * kNttM);
+        case Route::AddClass2048: return 8ull * (6 * 2048 + 1) + 1 + kBigMeta;
+        case Route::AddClass4096: return 8ull * (6 * 4096 + 1) + 1 + kBigMeta;
+        case Route::Ntt2048: return 8ull * (2 * kNttM + 2 * kNttM + 2 * kNttM + 2 * kNttM) + kBigMeta;
+        case Route::Ntt4096: return 8ull * (4 * kNttM + 4 * 2 * kNttM + 4 * 2 * kNttM + 4 * kNttM) + 4 * kBigMeta;
     }
     return 0;
 }
@@ -211,7 +213,7 @@ struct Slot {
     size_t pinned_bytes = 0;
     bool busy = false;
     // in-flight batch
-    size_t region_off = 0, region_bytes = 0, out_off_bytes = 0, out_bytes = 0, carry_off_bytes = 0;
+    size_t region_off = 0, region_bytes = 0, out_off_bytes = 0, out_bytes = 0;
     std::vector<ResultRef> refs;
     std::vector<uint64_t> ids;
 };
@@ -325,13 +327,33 @@ midint_status launch_batch(midint_server* s, size_t si) {
         rops[k].off_b = in - rg_b;
         in += s->reqs[rids[k]].nb;
     }
-    const size_t a2k = in;
-    in += (size_t)add2k.size() * 2048 * 2;
-    const size_t a4k = in;
-    in += (size_t)add4k.size() * 4096 * 2;
-    const size_t nttx = in;
+    // the requests above the ragged limit cross the bus packed: their a's,
+    // then their b's (in the order add2k, add4k, ntt2k, ntt4k)
+    std::vector<uint64_t> big;
+    big.reserve(add2k.size() + add4k.size() + ntt2k.size() + ntt4k.size());
+    big.insert(big.end(), add2k.begin(), add2k.end());
+    big.insert(big.end(), add4k.begin(), add4k.end());
+    big.insert(big.end(), ntt2k.begin(), ntt2k.end());
+    big.insert(big.end(), ntt4k.begin(), ntt4k.end());
+    std::vector<size_t> big_a(big.size()), big_b(big.size()), big_r(big.size());
+    for (size_t i = 0; i < big.size(); i++) {
+        big_a[i] = in;
+        in += s->reqs[big[i]].na;
+    }
+    for (size_t i = 0; i < big.size(); i++) {
+        big_b[i] = in;
+        in += s->reqs[big[i]].nb;
+    }
+    // their zero-padded class slots, on the device only (limbs from the
+    // start of the padded area), built there from the packed operands
+    size_t pad = 0;
+    const size_t a2k = pad;
+    pad += (size_t)add2k.size() * 2048 * 2;
+    const size_t a4k = pad;
+    pad += (size_t)add4k.size() * 4096 * 2;
+    const size_t nttx = pad;
     const size_t n_ntt = ntt2k.size() + 4 * ntt4k.size();
-    in += n_ntt * kNttM * 2;
+    pad += n_ntt * kNttM * 2;
     // ragged metadata (as limbs)
     std::vector<RgItem> items;
     std::vector<RgOp> rgops;
@@ -344,6 +366,14 @@ midint_status launch_batch(midint_server* s, size_t si) {
         rops[k].off_r = out - rg_r;
         out += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
     }
+    // the big requests' results, gathered on the device from the padded
+    // outputs below: [ragged results | these] is what comes back
+    for (size_t i = 0; i < big.size(); i++) {
+        const Request& r = s->reqs[big[i]];
+        big_r[i] = out;
+        out += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
+    }
+    const size_t out_copy = out;
     const size_t a2k_r = out;
     out += add2k.size() * 2048;
     const size_t a4k_r = out;
@@ -455,16 +485,23 @@ midint_status launch_batch(midint_server* s, size_t si) {
     const size_t big_meta_off = meta_at;
     const size_t wv_add_off = big_meta_off + (big_mf ? n_ntt : 0) * sizeof(MfRgItem);
     const size_t wv_mul_off = wv_add_off + wv_add_idx.size() * sizeof(MfRgItem);
-    const size_t meta_limbs =
-        align_up(wv_mul_off + wv_mul_idx.size() * sizeof(MfRgItem), 8) / 8 + 1 /* counter */;
+    // the big requests' slot and result tables
+    const size_t n_slots = 2 * (add2k.size() + add4k.size() + n_ntt);
+    const size_t slot_meta_off = align_up(wv_mul_off + wv_mul_idx.size() * sizeof(MfRgItem), 16);
+    const size_t gather_meta_off = align_up(slot_meta_off + n_slots * sizeof(ServeSlot), 16);
+    const size_t meta_limbs = align_up(gather_meta_off + big.size() * sizeof(ServeGather), 8) / 8 + 1 /* counter */;
     in += meta_limbs;
     in = align_up(in, 32);
+    // device region: [staged inputs | padded slots | outputs | carry bytes];
+    // the staging buffer holds the staged inputs, then the packed results
     const size_t in_bytes = in * 8;
+    const size_t pad_bytes = align_up(pad, 32) * 8;
     const size_t out_bytes = out * 8 + carry_bytes;
-    const size_t region = in_bytes + out_bytes;
-    if (region > sl.pinned_bytes) {
+    const size_t region = in_bytes + pad_bytes + out_bytes;
+    const size_t staging = in_bytes + out_copy * 8;
+    if (staging > sl.pinned_bytes) {
         // rare: staging too small for this batch; grow it
-        if (alloc_staging(sl, region) != hipSuccess) {
+        if (alloc_staging(sl, staging) != hipSuccess) {
             for (auto it = take.rbegin(); it != take.rend(); ++it) s->queue.push_front(*it);
             return MIDINT_ENOMEM;
         }
@@ -476,74 +513,36 @@ midint_status launch_batch(midint_server* s, size_t si) {
         return MIDINT_EBUSY;
     }
 
-    // ---- fill the staging inputs: every operand slot takes its limbs and
-    // zeros up to the slot's size (padded classes), the slots in parallel;
-    // the metadata area after them starts zeroed
+    // ---- fill the staging inputs: the packed operands, in parallel; the
+    // metadata area after them starts zeroed
     uint64_t* h = reinterpret_cast<uint64_t*>(sl.pinned);
     struct Fill {
         uint64_t* dst;
         const uint64_t* src;
-        size_t n, size;  // limbs copied, limbs of the slot
+        size_t n;  // limbs
     };
     std::vector<Fill> fills;
-    fills.reserve(2 * (rids.size() + add2k.size() + add4k.size() + n_ntt));
+    fills.reserve(2 * (rids.size() + big.size()));
     for (size_t k = 0; k < rids.size(); k++) {
         const Request& r = s->reqs[rids[k]];
-        fills.push_back({h + rg_a + rops[k].off_a, r.a.data(), r.na, r.na});
-        fills.push_back({h + rg_b + rops[k].off_b, r.b.data(), r.nb, r.nb});
+        fills.push_back({h + rg_a + rops[k].off_a, r.a.data(), r.na});
+        fills.push_back({h + rg_b + rops[k].off_b, r.b.data(), r.nb});
     }
-    for (size_t k = 0; k < add2k.size(); k++) {
-        const Request& r = s->reqs[add2k[k]];
-        fills.push_back({h + a2k + k * 2048, r.a.data(), r.na, 2048});
-        fills.push_back({h + a2k + (add2k.size() + k) * 2048, r.b.data(), r.nb, 2048});
-    }
-    for (size_t k = 0; k < add4k.size(); k++) {
-        const Request& r = s->reqs[add4k[k]];
-        fills.push_back({h + a4k + k * 4096, r.a.data(), r.na, 4096});
-        fills.push_back({h + a4k + (add4k.size() + k) * 4096, r.b.data(), r.nb, 4096});
-    }
-    {
-        uint64_t* X = h + nttx;
-        uint64_t* Y = h + nttx + n_ntt * kNttM;
-        size_t inst = 0;
-        for (uint64_t id : ntt2k) {
-            const Request& r = s->reqs[id];
-            fills.push_back({X + inst * kNttM, r.a.data(), r.na, kNttM});
-            fills.push_back({Y + inst * kNttM, r.b.data(), r.nb, kNttM});
-            inst++;
-        }
-        for (uint64_t id : ntt4k) {
-            const Request& r = s->reqs[id];
-            // halves: a = a1 X + a0, b = b1 X + b0; instances a0b0, a1b1, a0b1, a1b0
-            const uint32_t a0 = std::min(r.na, kNttM), a1 = r.na > kNttM ? r.na - kNttM : 0;
-            const uint32_t b0 = std::min(r.nb, kNttM), b1 = r.nb > kNttM ? r.nb - kNttM : 0;
-            const uint64_t* pa0 = r.a.data();
-            const uint64_t* pa1 = r.a.data() + a0;
-            const uint64_t* pb0 = r.b.data();
-            const uint64_t* pb1 = r.b.data() + b0;
-            const uint64_t* xs[4] = {pa0, pa1, pa0, pa1};
-            const uint32_t xl[4] = {a0, a1, a0, a1};
-            const uint64_t* ys[4] = {pb0, pb1, pb1, pb0};
-            const uint32_t yl[4] = {b0, b1, b1, b0};
-            for (int q = 0; q < 4; q++) {
-                fills.push_back({X + inst * kNttM, xs[q], xl[q], kNttM});
-                fills.push_back({Y + inst * kNttM, ys[q], yl[q], kNttM});
-                inst++;
-            }
-        }
+    for (size_t i = 0; i < big.size(); i++) {
+        const Request& r = s->reqs[big[i]];
+        fills.push_back({h + big_a[i], r.a.data(), r.na});
+        fills.push_back({h + big_b[i], r.b.data(), r.nb});
     }
     parallel_items(
-        fills.size(), [&](size_t i) { return 8 * fills[i].size; },
-        [&](size_t i) {
-            const Fill& f = fills[i];
-            if (f.n) memcpy(f.dst, f.src, 8 * f.n);
-            if (f.size > f.n) memset(f.dst + f.n, 0, 8 * (f.size - f.n));
-        });
+        fills.size(), [&](size_t i) { return 8 * fills[i].n; },
+        [&](size_t i) { memcpy(fills[i].dst, fills[i].src, 8 * fills[i].n); });
     memset(h + items_off, 0, (in - items_off) * 8);
     // ---- device pointers
     uint8_t* dreg = s->pool.base + off;
     uint64_t* d_in = reinterpret_cast<uint64_t*>(dreg);
-    uint64_t* d_out = reinterpret_cast<uint64_t*>(dreg + in_bytes);
+    uint64_t* d_pad = reinterpret_cast<uint64_t*>(dreg + in_bytes);
+    uint64_t* d_out = reinterpret_cast<uint64_t*>(dreg + in_bytes + pad_bytes);
+    uint8_t* d_carry = dreg + in_bytes + pad_bytes + out * 8;
 
     uint8_t* meta = reinterpret_cast<uint8_t*>(h + items_off);
     memcpy(meta, items.data(), items.size() * sizeof(RgItem));
@@ -581,8 +580,8 @@ midint_status launch_batch(midint_server* s, size_t si) {
             // instance inst: operands at X / Y + inst kNttM (zero-padded),
             // product at ntt_r + inst 2 kNttM
             MfRgItem* bi = reinterpret_cast<MfRgItem*>(meta + big_meta_off);
-            const uint64_t* X = d_in + nttx;
-            const uint64_t* Y = d_in + nttx + n_ntt * kNttM;
+            const uint64_t* X = d_pad + nttx;
+            const uint64_t* Y = d_pad + nttx + n_ntt * kNttM;
             uint64_t* Z = d_out + ntt_r;
             size_t inst = 0;
             for (uint64_t id : ntt2k) {
@@ -596,8 +595,63 @@ midint_status launch_batch(midint_server* s, size_t si) {
             for (; inst < n_ntt; inst++)  // the split's quarters: all 2 kNttM limbs feed the combine
                 bi[inst] = mfma_rg_item(X + inst * kNttM, Y + inst * kNttM, Z + inst * 2 * kNttM, kNttM, kNttM);
         }
+        // the big requests' padded slots from their packed operands
+        ServeSlot* sv = reinterpret_cast<ServeSlot*>(meta + slot_meta_off);
+        size_t q = 0;
+        auto slot = [&](size_t i, bool b_op, uint32_t skip, uint32_t n, uint64_t* dst, uint32_t size) {
+            sv[q++] = ServeSlot{d_in + (b_op ? big_b[i] : big_a[i]) + skip, dst, n, size};
+        };
+        size_t i = 0;
+        for (size_t k = 0; k < add2k.size(); k++, i++) {
+            const Request& r = s->reqs[big[i]];
+            slot(i, false, 0, r.na, d_pad + a2k + k * 2048, 2048);
+            slot(i, true, 0, r.nb, d_pad + a2k + (add2k.size() + k) * 2048, 2048);
+        }
+        for (size_t k = 0; k < add4k.size(); k++, i++) {
+            const Request& r = s->reqs[big[i]];
+            slot(i, false, 0, r.na, d_pad + a4k + k * 4096, 4096);
+            slot(i, true, 0, r.nb, d_pad + a4k + (add4k.size() + k) * 4096, 4096);
+        }
+        uint64_t* PX = d_pad + nttx;
+        uint64_t* PY = d_pad + nttx + n_ntt * kNttM;
+        size_t inst = 0;
+        for (size_t k = 0; k < ntt2k.size(); k++, i++, inst++) {
+            const Request& r = s->reqs[big[i]];
+            slot(i, false, 0, r.na, PX + inst * kNttM, kNttM);
+            slot(i, true, 0, r.nb, PY + inst * kNttM, kNttM);
+        }
+        for (size_t k = 0; k < ntt4k.size(); k++, i++) {
+            const Request& r = s->reqs[big[i]];
+            // halves: a = a1 X + a0, b = b1 X + b0; instances a0b0, a1b1, a0b1, a1b0
+            const uint32_t a0 = std::min(r.na, kNttM), a1 = r.na > kNttM ? r.na - kNttM : 0;
+            const uint32_t b0 = std::min(r.nb, kNttM), b1 = r.nb > kNttM ? r.nb - kNttM : 0;
+            const uint32_t xo[4] = {0, a0, 0, a0}, xl[4] = {a0, a1, a0, a1};
+            const uint32_t yo[4] = {0, b0, b0, 0}, yl[4] = {b0, b1, b1, b0};
+            for (int u = 0; u < 4; u++, inst++) {
+                slot(i, false, xo[u], xl[u], PX + inst * kNttM, kNttM);
+                slot(i, true, yo[u], yl[u], PY + inst * kNttM, kNttM);
+            }
+        }
+        // and their results, packed after the ragged results
+        ServeGather* gv = reinterpret_cast<ServeGather*>(meta + gather_meta_off);
+        i = 0;
+        for (size_t k = 0; k < add2k.size() + add4k.size(); k++, i++) {
+            const Request& r = s->reqs[big[i]];
+            const bool c4 = k >= add2k.size();
+            const uint32_t C = c4 ? 4096 : 2048;
+            const size_t kk = c4 ? k - add2k.size() : k;
+            const uint32_t len = std::max(r.na, r.nb) + 1;
+            // the full-width sum at the class size: its top limb is the kernel's carry-out
+            gv[i] = ServeGather{d_out + (c4 ? a4k_r : a2k_r) + kk * C, d_out + big_r[i], len - 1 == C ? d_carry + k : nullptr,
+                                len, 0};
+        }
+        inst = 0;
+        for (size_t k = 0; k < ntt2k.size() + ntt4k.size(); k++, i++) {
+            const Request& r = s->reqs[big[i]];
+            gv[i] = ServeGather{d_out + ntt_r + inst * 2 * kNttM, d_out + big_r[i], nullptr, r.na + r.nb, 0};
+            inst += k < ntt2k.size() ? 1 : 4;
+        }
     }
-    uint8_t* d_carry = dreg + in_bytes + out * 8;
     RgItem* d_items = reinterpret_cast<RgItem*>(d_in + items_off);
     RgOp* d_rgops = reinterpret_cast<RgOp*>(reinterpret_cast<uint8_t*>(d_items) + items.size() * sizeof(RgItem));
     uint32_t* d_counter = reinterpret_cast<uint32_t*>(d_in + items_off + meta_limbs - 1);
@@ -606,6 +660,9 @@ midint_status launch_batch(midint_server* s, size_t si) {
     // the memset of the staging above, so the input copy clears it
     hipStream_t st = sl.stream;
     hipError_t e = launch_stage_copy(sl.pinned_dev, d_in, in_bytes, st);
+    uint8_t* d_meta = reinterpret_cast<uint8_t*>(d_in + items_off);
+    if (e == hipSuccess)
+        e = launch_serve_expand((uint32_t)n_slots, reinterpret_cast<const ServeSlot*>(d_meta + slot_meta_off), st);
     if (e == hipSuccess && !items.empty()) {
         const uint32_t n_items = (uint32_t)items.size();
         const uint64_t* dA = d_in + rg_a;
@@ -619,7 +676,6 @@ midint_status launch_batch(midint_server* s, size_t si) {
         if (e == hipSuccess)
             e = hipLaunchKernel(ragged_kernel_ptr(256), dim3(grid), dim3(256), args, lds, st);
     }
-    uint8_t* d_meta = reinterpret_cast<uint8_t*>(d_in + items_off);
     for (const MfGroup& G : mfg) {
         if (e != hipSuccess || G.idx.empty()) continue;
         MfRgItem* d_mf = reinterpret_cast<MfRgItem*>(d_meta + G.meta_off);
@@ -641,13 +697,13 @@ midint_status launch_batch(midint_server* s, size_t si) {
                             dim3(64 * rgw_waves()), args, mul ? rgw_lds_bytes(0) : 0, st);
     }
     if (e == hipSuccess && !add2k.empty())
-        e = launch_add((uint32_t)add2k.size(), 2048, d_in + a2k, d_in + a2k + add2k.size() * 2048, d_out + a2k_r,
+        e = launch_add((uint32_t)add2k.size(), 2048, d_pad + a2k, d_pad + a2k + add2k.size() * 2048, d_out + a2k_r,
                        d_carry, false, st);
     if (e == hipSuccess && !add4k.empty())
-        e = launch_add((uint32_t)add4k.size(), 4096, d_in + a4k, d_in + a4k + add4k.size() * 4096, d_out + a4k_r,
+        e = launch_add((uint32_t)add4k.size(), 4096, d_pad + a4k, d_pad + a4k + add4k.size() * 4096, d_out + a4k_r,
                        d_carry + add2k.size(), false, st);
     if (e == hipSuccess && n_ntt > 0 && big_mf) {
-        MfRgItem* d_big = reinterpret_cast<MfRgItem*>(reinterpret_cast<uint8_t*>(d_in + items_off) + big_meta_off);
+        MfRgItem* d_big = reinterpret_cast<MfRgItem*>(d_meta + big_meta_off);
         uint32_t pow8 = bigmf.pow8;
         void* args[] = {(void*)&d_big, (void*)&pow8};
         if (bigmf.lds > 64 * 1024)
@@ -659,12 +715,14 @@ midint_status launch_batch(midint_server* s, size_t si) {
         midint_status ts = ctx_ntt_tables(s->ctx, ntt_lg_for_limbs(kNttM), &t);
         if (ts != MIDINT_OK) e = hipErrorUnknown;
         if (e == hipSuccess)
-            e = launch_mul_ntt((uint32_t)n_ntt, kNttM, d_in + nttx, d_in + nttx + n_ntt * kNttM, d_out + ntt_r, t,
+            e = launch_mul_ntt((uint32_t)n_ntt, kNttM, d_pad + nttx, d_pad + nttx + n_ntt * kNttM, d_out + ntt_r, t,
                                st);
     }
     if (e == hipSuccess && !ntt4k.empty())
         e = launch_bigmul_combine((uint32_t)ntt4k.size(), d_out + ntt_r + ntt2k.size() * 2 * kNttM, st);
-    if (e == hipSuccess) e = launch_stage_copy(d_out, sl.pinned_dev + in_bytes, out_bytes, st);
+    if (e == hipSuccess)
+        e = launch_serve_gather((uint32_t)big.size(), reinterpret_cast<const ServeGather*>(d_meta + gather_meta_off), st);
+    if (e == hipSuccess) e = launch_stage_copy(d_out, sl.pinned_dev + in_bytes, out_copy * 8, st);
     if (e == hipSuccess) e = hipEventRecord(sl.ev, st);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
@@ -678,35 +736,18 @@ midint_status launch_batch(midint_server* s, size_t si) {
     sl.region_off = off;
     sl.region_bytes = region;
     sl.out_off_bytes = in_bytes;
-    sl.out_bytes = out_bytes;
-    sl.carry_off_bytes = in_bytes + out * 8;
+    sl.out_bytes = out_copy * 8;
     sl.refs.clear();
     sl.ids = take;
     for (size_t k = 0; k < rids.size(); k++) {
         Request& r = s->reqs[rids[k]];
         const uint32_t len = r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
-        sl.refs.push_back({rids[k], rg_r + rops[k].off_r, len, -1, 0});
+        sl.refs.push_back({rids[k], rg_r + rops[k].off_r, len});
     }
-    for (size_t k = 0; k < add2k.size(); k++) {
-        Request& r = s->reqs[add2k[k]];
-        const uint32_t len = std::max(r.na, r.nb) + 1;
-        sl.refs.push_back({add2k[k], a2k_r + k * 2048, len, len - 1 == 2048 ? (int)k : -1, 2048});
-    }
-    for (size_t k = 0; k < add4k.size(); k++) {
-        Request& r = s->reqs[add4k[k]];
-        const uint32_t len = std::max(r.na, r.nb) + 1;
-        sl.refs.push_back({add4k[k], a4k_r + k * 4096, len, len - 1 == 4096 ? (int)(add2k.size() + k) : -1, 4096});
-    }
-    size_t inst = 0;
-    for (uint64_t id : ntt2k) {
-        Request& r = s->reqs[id];
-        sl.refs.push_back({id, ntt_r + inst * 2 * kNttM, r.na + r.nb, -1, 0});
-        inst++;
-    }
-    for (uint64_t id : ntt4k) {
-        Request& r = s->reqs[id];
-        sl.refs.push_back({id, ntt_r + inst * 2 * kNttM, r.na + r.nb, -1, 0});
-        inst += 4;
+    for (size_t i = 0; i < big.size(); i++) {
+        Request& r = s->reqs[big[i]];
+        const uint32_t len = r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
+        sl.refs.push_back({big[i], big_r[i], len});
     }
     for (uint64_t id : take) {
         Request& r = s->reqs[id];
@@ -723,7 +764,6 @@ midint_status launch_batch(midint_server* s, size_t si) {
 
 void retire(midint_server* s, Slot& sl, double t, std::vector<uint64_t>& done) {
     const uint64_t* outp = reinterpret_cast<const uint64_t*>(sl.pinned + sl.out_off_bytes);
-    const uint8_t* carries = sl.pinned + sl.carry_off_bytes;
     std::vector<Request*> req(sl.refs.size());
     for (size_t i = 0; i < sl.refs.size(); i++) req[i] = &s->reqs[sl.refs[i].id];
     // the result copies in parallel (each request's own vector)
@@ -731,14 +771,7 @@ void retire(midint_server* s, Slot& sl, double t, std::vector<uint64_t>& done) {
         sl.refs.size(), [&](size_t i) { return 8 * (size_t)sl.refs[i].len; },
         [&](size_t i) {
             const ResultRef& rr = sl.refs[i];
-            Request& r = *req[i];
-            if (rr.carry_idx >= 0) {
-                // full-width sum at its class size: the top limb is the kernel's carry-out
-                r.result.assign(outp + rr.out_off, outp + rr.out_off + rr.len - 1);
-                r.result.push_back(carries[rr.carry_idx]);
-            } else {
-                r.result.assign(outp + rr.out_off, outp + rr.out_off + rr.len);
-            }
+            req[i]->result.assign(outp + rr.out_off, outp + rr.out_off + rr.len);
         });
     for (size_t i = 0; i < sl.refs.size(); i++) {
         req[i]->done = true;
