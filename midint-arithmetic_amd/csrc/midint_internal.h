@@ -206,12 +206,13 @@ const void* ragged_reset_ptr();
 // 2048 x 2048 NTT products [P00 | P11 | P01 | P10] (4096 limbs each); the
 // combine kernel adds P01 + P10 into limbs [2048, 8192) of P00 | P11.
 hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s);
-// Copies `bytes` (a multiple of 8; both pointers 16-byte aligned) with a kernel on `s`.
-hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
-// Serving loop: zero-padded slots built on the device from packed operands
-// (dst[0, n) = src[0, n), zeros up to size), and results gathered from
-// their padded outputs into a packed area (carry: the top limb is that byte
-// when set); one workgroup per slot or result.
+// Serving loop transfers between the pinned staging buffer and HBM, kernels
+// on `s`: `bytes` (a multiple of 8; both pointers 16-byte aligned) copied,
+// plus per entry of `slots` a zero-padded device slot filled from packed
+// limbs (dst[0, n) = src[0, n), zeros up to size) on the way in, and per
+// entry of `items` a result gathered from its padded output (carry: the top
+// limb is that byte when set) on the way out; the tables are read in place
+// (host-mapped staging).
 struct ServeSlot {
     const uint64_t* src;
     uint64_t* dst;
@@ -223,8 +224,10 @@ struct ServeGather {
     const uint8_t* carry;
     uint32_t n, pad;
 };
-hipError_t launch_serve_expand(uint32_t n, const ServeSlot* slots, hipStream_t s);
-hipError_t launch_serve_gather(uint32_t n, const ServeGather* items, hipStream_t s);
+hipError_t launch_stage_in(const void* src, void* dst, size_t bytes, const ServeSlot* slots, uint32_t n_slots,
+                           hipStream_t s);
+hipError_t launch_stage_out(const void* src, void* dst, size_t bytes, const ServeGather* items, uint32_t n_items,
+                            hipStream_t s);
 
 // Autotuned launch parameter (see tuning.cpp), or dflt for unknown keys.
 int64_t tune_get(const char* key, int64_t dflt);

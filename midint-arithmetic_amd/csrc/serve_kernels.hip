@@ -8,13 +8,11 @@
 // adds P01 and then P10 into limbs [2048, 8192): two chained carry-resolved
 // additions on register-resident rows (6144 limbs = 16 waves x 3 rows).
 //
-// stage_copy: the batch's host<->HBM transfers (below).
-//
-// serve_expand / serve_gather: the operands of requests above the ragged
-// size limit cross the bus packed (their limbs only) and are spread into
-// their zero-padded class slots on the device; their results are gathered
-// from the padded outputs into one packed area, which is all that goes
-// back.
+// stage_in / stage_out: the batch's host<->HBM transfers (below). The
+// operands of requests above the ragged size limit cross the bus packed
+// (their limbs only) and are spread into their zero-padded class slots on
+// the way in; their results are gathered from the padded outputs into
+// their packed places on the way out.
 #include <algorithm>
 
 #include "add_core.hpp"
@@ -68,15 +66,19 @@ hipError_t launch_bigmul_combine(uint32_t n, uint64_t* blocks16k, hipStream_t s)
 
 namespace midint {
 
-// Staging copy between pinned host memory and HBM for the serving loop:
-// moves `n16` 16-byte words, then one trailing 8-byte word when `tail8` is
-// set. It runs on the batch's own stream in place of a copy-engine transfer;
-// each thread keeps 4 loads in flight so that host reads over the bus
-// overlap.
-__global__ __launch_bounds__(256) void stage_copy_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
-                                                         size_t n16, uint32_t tail8) {
+// The serving loop's transfers between its pinned staging buffer and HBM,
+// on the batch's own stream in place of copy-engine transfers. Blocks [0,
+// copy_blocks) move `n16` 16-byte words, then one trailing 8-byte word when
+// `tail8` is set (each thread keeps 4 loads in flight so that reads over the
+// bus overlap); each further block handles one entry of `slots` (inbound:
+// packed limbs read from the staging buffer into a zero-padded device slot)
+// or `items` (outbound: a result read from its padded device output into
+// its packed place in the staging buffer). The tables are read where the
+// host wrote them, in the staging buffer.
+__device__ __forceinline__ void stage_copy_part(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n16,
+                                                uint32_t tail8, uint32_t copy_blocks) {
     constexpr int U = 4;
-    const size_t stride = (size_t)gridDim.x * 256;
+    const size_t stride = (size_t)copy_blocks * 256;
     size_t i = (size_t)blockIdx.x * 256 + threadIdx.x;
     for (; i + (U - 1) * stride < n16; i += U * stride) {
         uint4 v[U];
@@ -90,42 +92,91 @@ __global__ __launch_bounds__(256) void stage_copy_kernel(const uint4* __restrict
         reinterpret_cast<uint64_t*>(dst + n16)[0] = reinterpret_cast<const uint64_t*>(src + n16)[0];
 }
 
-hipError_t launch_stage_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
-    if (bytes == 0) return hipSuccess;
-    if (bytes % 8 || reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(dst) % 16)
-        return hipErrorInvalidValue;
-    const size_t n16 = bytes / 16;
-    const size_t blocks = std::min<size_t>(1024, std::max<size_t>(1, (n16 + 1023) / 1024));
-    stage_copy_kernel<<<(uint32_t)blocks, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16,
-                                                        (uint32_t)(bytes % 16 != 0));
-    return hipGetLastError();
+__global__ __launch_bounds__(256) void stage_in_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                       size_t n16, uint32_t tail8, uint32_t copy_blocks,
+                                                       const ServeSlot* __restrict__ slots) {
+    if (blockIdx.x < copy_blocks) {
+        stage_copy_part(src, dst, n16, tail8, copy_blocks);
+        return;
+    }
+    // dst[0, n) = src[0, n), dst[n, size) = 0, in chunks of 16 limbs per
+    // thread whose reads over the bus are all in flight at once
+    const ServeSlot sl = slots[blockIdx.x - copy_blocks];
+    constexpr uint32_t U = 16;
+    for (uint32_t base = 0; base < sl.size; base += U * 256) {
+        uint64_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = base + u * 256 + threadIdx.x;
+            v[u] = i < sl.n ? sl.src[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = base + u * 256 + threadIdx.x;
+            if (i < sl.size) sl.dst[i] = v[u];
+        }
+    }
 }
 
-
-// One workgroup per slot: dst[0, n) = src[0, n), dst[n, size) = 0.
-__global__ __launch_bounds__(256) void serve_expand_kernel(const ServeSlot* __restrict__ slots) {
-    const ServeSlot sl = slots[blockIdx.x];
-    for (uint32_t i = threadIdx.x; i < sl.size; i += 256) sl.dst[i] = i < sl.n ? sl.src[i] : 0ull;
-}
-
-// One workgroup per result: dst[0, n) = src[0, n), except that with a carry
-// byte the top limb dst[n - 1] is that byte.
-__global__ __launch_bounds__(256) void serve_gather_kernel(const ServeGather* __restrict__ items) {
-    const ServeGather it = items[blockIdx.x];
+__global__ __launch_bounds__(256) void stage_out_kernel(const uint4* __restrict__ src, uint4* __restrict__ dst,
+                                                        size_t n16, uint32_t tail8, uint32_t copy_blocks,
+                                                        const ServeGather* __restrict__ items) {
+    if (blockIdx.x < copy_blocks) {
+        stage_copy_part(src, dst, n16, tail8, copy_blocks);
+        return;
+    }
+    // dst[0, n) = src[0, n), except that with a carry byte the top limb is
+    // that byte; 16 limbs per thread at once
+    const ServeGather it = items[blockIdx.x - copy_blocks];
     const uint32_t m = it.carry ? it.n - 1 : it.n;
-    for (uint32_t i = threadIdx.x; i < m; i += 256) it.dst[i] = it.src[i];
+    constexpr uint32_t U = 16;
+    for (uint32_t base = 0; base < m; base += U * 256) {
+        uint64_t v[U];
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = base + u * 256 + threadIdx.x;
+            v[u] = i < m ? it.src[i] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; u++) {
+            const uint32_t i = base + u * 256 + threadIdx.x;
+            if (i < m) it.dst[i] = v[u];
+        }
+    }
     if (it.carry && threadIdx.x == 0) it.dst[m] = *it.carry;
 }
 
-hipError_t launch_serve_expand(uint32_t n, const ServeSlot* slots, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    serve_expand_kernel<<<n, 256, 0, s>>>(slots);
+namespace {
+
+hipError_t stage_geometry(const void* src, void* dst, size_t bytes, size_t* n16, uint32_t* blocks) {
+    if (bytes % 8 || reinterpret_cast<uintptr_t>(src) % 16 || reinterpret_cast<uintptr_t>(dst) % 16)
+        return hipErrorInvalidValue;
+    *n16 = bytes / 16;
+    *blocks = bytes == 0 ? 0u : (uint32_t)std::min<size_t>(1024, std::max<size_t>(1, (*n16 + 1023) / 1024));
+    return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t launch_stage_in(const void* src, void* dst, size_t bytes, const ServeSlot* slots, uint32_t n_slots,
+                           hipStream_t s) {
+    size_t n16 = 0;
+    uint32_t blocks = 0;
+    hipError_t e = stage_geometry(src, dst, bytes, &n16, &blocks);
+    if (e != hipSuccess || blocks + n_slots == 0) return e;
+    stage_in_kernel<<<blocks + n_slots, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16,
+                                                     (uint32_t)(bytes % 16 != 0), blocks, slots);
     return hipGetLastError();
 }
 
-hipError_t launch_serve_gather(uint32_t n, const ServeGather* items, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    serve_gather_kernel<<<n, 256, 0, s>>>(items);
+hipError_t launch_stage_out(const void* src, void* dst, size_t bytes, const ServeGather* items, uint32_t n_items,
+                            hipStream_t s) {
+    size_t n16 = 0;
+    uint32_t blocks = 0;
+    hipError_t e = stage_geometry(src, dst, bytes, &n16, &blocks);
+    if (e != hipSuccess || blocks + n_items == 0) return e;
+    stage_out_kernel<<<blocks + n_items, 256, 0, s>>>(static_cast<const uint4*>(src), static_cast<uint4*>(dst), n16,
+                                                      (uint32_t)(bytes % 16 != 0), blocks, items);
     return hipGetLastError();
 }
 

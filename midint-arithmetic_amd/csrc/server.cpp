@@ -139,18 +139,19 @@ Route route_of(uint32_t kind, uint32_t na, uint32_t nb) {
 }
 
 // Device bytes a request needs inside a batch region (upper bound including
-// its share of the metadata): ragged ops their packed operands and result;
-// the others their packed operands, their zero-padded class slots, the
-// class-size outputs, the packed result and their slot / result entries.
+// its share of the metadata): ragged ops their operands and result; the
+// others their zero-padded class slots, the class-size outputs and their
+// slot / result entries (their packed operands and results only pass
+// through the staging buffer, which is never larger than the region).
 size_t request_bytes(uint32_t kind, uint32_t na, uint32_t nb) {
     constexpr size_t kBigMeta = 8 * sizeof(ServeSlot) + sizeof(ServeGather) + sizeof(MfRgItem);
     switch (route_of(kind, na, nb)) {
         case Route::Ragged:
             return 8ull * (na + nb + (kind == MIDINT_OP_MUL ? na + nb : std::max(na, nb) + 1)) + 2 * sizeof(RgOp);
-        case Route::AddClass2048: return 8ull * (6 * 2048 + 1) + 1 + kBigMeta;
-        case Route::AddClass4096: return 8ull * (6 * 4096 + 1) + 1 + kBigMeta;
-        case Route::Ntt2048: return 8ull * (2 * kNttM + 2 * kNttM + 2 * kNttM + 2 * kNttM) + kBigMeta;
-        case Route::Ntt4096: return 8ull * (4 * kNttM + 4 * 2 * kNttM + 4 * 2 * kNttM + 4 * kNttM) + 4 * kBigMeta;
+        case Route::AddClass2048: return 8ull * 3 * 2048 + 1 + kBigMeta;
+        case Route::AddClass4096: return 8ull * 3 * 4096 + 1 + kBigMeta;
+        case Route::Ntt2048: return 8ull * (2 * kNttM + 2 * kNttM) + kBigMeta;
+        case Route::Ntt4096: return 4 * 8ull * (2 * kNttM + 2 * kNttM) + 4 * kBigMeta;
     }
     return 0;
 }
@@ -327,8 +328,9 @@ midint_status launch_batch(midint_server* s, size_t si) {
         rops[k].off_b = in - rg_b;
         in += s->reqs[rids[k]].nb;
     }
-    // the requests above the ragged limit cross the bus packed: their a's,
-    // then their b's (in the order add2k, add4k, ntt2k, ntt4k)
+    // the requests above the ragged limit (in the order add2k, add4k,
+    // ntt2k, ntt4k) cross the bus packed; they stay in the staging buffer
+    // after the inputs that go to the device (offsets below)
     std::vector<uint64_t> big;
     big.reserve(add2k.size() + add4k.size() + ntt2k.size() + ntt4k.size());
     big.insert(big.end(), add2k.begin(), add2k.end());
@@ -336,16 +338,9 @@ midint_status launch_batch(midint_server* s, size_t si) {
     big.insert(big.end(), ntt2k.begin(), ntt2k.end());
     big.insert(big.end(), ntt4k.begin(), ntt4k.end());
     std::vector<size_t> big_a(big.size()), big_b(big.size()), big_r(big.size());
-    for (size_t i = 0; i < big.size(); i++) {
-        big_a[i] = in;
-        in += s->reqs[big[i]].na;
-    }
-    for (size_t i = 0; i < big.size(); i++) {
-        big_b[i] = in;
-        in += s->reqs[big[i]].nb;
-    }
     // their zero-padded class slots, on the device only (limbs from the
-    // start of the padded area), built there from the packed operands
+    // start of the padded area), filled from the packed operands by the
+    // inbound transfer
     size_t pad = 0;
     const size_t a2k = pad;
     pad += (size_t)add2k.size() * 2048 * 2;
@@ -366,14 +361,7 @@ midint_status launch_batch(midint_server* s, size_t si) {
         rops[k].off_r = out - rg_r;
         out += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
     }
-    // the big requests' results, gathered on the device from the padded
-    // outputs below: [ragged results | these] is what comes back
-    for (size_t i = 0; i < big.size(); i++) {
-        const Request& r = s->reqs[big[i]];
-        big_r[i] = out;
-        out += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
-    }
-    const size_t out_copy = out;
+    const size_t rg_out = out;
     const size_t a2k_r = out;
     out += add2k.size() * 2048;
     const size_t a4k_r = out;
@@ -492,13 +480,31 @@ midint_status launch_batch(midint_server* s, size_t si) {
     const size_t meta_limbs = align_up(gather_meta_off + big.size() * sizeof(ServeGather), 8) / 8 + 1 /* counter */;
     in += meta_limbs;
     in = align_up(in, 32);
-    // device region: [staged inputs | padded slots | outputs | carry bytes];
-    // the staging buffer holds the staged inputs, then the packed results
-    const size_t in_bytes = in * 8;
+    const size_t in_copy = in;  // limbs that go to the device
+    // staging only: the big requests' packed operands (their a's, then their
+    // b's), then the results that come back: the ragged results as they lie
+    // on the device, then the big requests' results, packed
+    for (size_t i = 0; i < big.size(); i++) {
+        big_a[i] = in;
+        in += s->reqs[big[i]].na;
+    }
+    for (size_t i = 0; i < big.size(); i++) {
+        big_b[i] = in;
+        in += s->reqs[big[i]].nb;
+    }
+    const size_t host_in = align_up(in, 32);
+    size_t out_copy = rg_out;
+    for (size_t i = 0; i < big.size(); i++) {
+        const Request& r = s->reqs[big[i]];
+        big_r[i] = out_copy;
+        out_copy += r.kind == MIDINT_OP_MUL ? r.na + r.nb : std::max(r.na, r.nb) + 1;
+    }
+    // device region: [inputs | padded slots | outputs | carry bytes]
+    const size_t in_bytes = in_copy * 8;
     const size_t pad_bytes = align_up(pad, 32) * 8;
     const size_t out_bytes = out * 8 + carry_bytes;
     const size_t region = in_bytes + pad_bytes + out_bytes;
-    const size_t staging = in_bytes + out_copy * 8;
+    const size_t staging = host_in * 8 + out_copy * 8;
     if (staging > sl.pinned_bytes) {
         // rare: staging too small for this batch; grow it
         if (alloc_staging(sl, staging) != hipSuccess) {
@@ -536,7 +542,7 @@ midint_status launch_batch(midint_server* s, size_t si) {
     parallel_items(
         fills.size(), [&](size_t i) { return 8 * fills[i].n; },
         [&](size_t i) { memcpy(fills[i].dst, fills[i].src, 8 * fills[i].n); });
-    memset(h + items_off, 0, (in - items_off) * 8);
+    memset(h + items_off, 0, (in_copy - items_off) * 8);
     // ---- device pointers
     uint8_t* dreg = s->pool.base + off;
     uint64_t* d_in = reinterpret_cast<uint64_t*>(dreg);
@@ -595,11 +601,13 @@ midint_status launch_batch(midint_server* s, size_t si) {
             for (; inst < n_ntt; inst++)  // the split's quarters: all 2 kNttM limbs feed the combine
                 bi[inst] = mfma_rg_item(X + inst * kNttM, Y + inst * kNttM, Z + inst * 2 * kNttM, kNttM, kNttM);
         }
-        // the big requests' padded slots from their packed operands
+        // the big requests' padded slots, filled from their packed operands
+        // in the staging buffer (its device-mapped address)
+        const uint64_t* hdev = reinterpret_cast<const uint64_t*>(sl.pinned_dev);
         ServeSlot* sv = reinterpret_cast<ServeSlot*>(meta + slot_meta_off);
         size_t q = 0;
         auto slot = [&](size_t i, bool b_op, uint32_t skip, uint32_t n, uint64_t* dst, uint32_t size) {
-            sv[q++] = ServeSlot{d_in + (b_op ? big_b[i] : big_a[i]) + skip, dst, n, size};
+            sv[q++] = ServeSlot{hdev + (b_op ? big_b[i] : big_a[i]) + skip, dst, n, size};
         };
         size_t i = 0;
         for (size_t k = 0; k < add2k.size(); k++, i++) {
@@ -632,7 +640,8 @@ midint_status launch_batch(midint_server* s, size_t si) {
                 slot(i, true, yo[u], yl[u], PY + inst * kNttM, kNttM);
             }
         }
-        // and their results, packed after the ragged results
+        // and their results, into their packed places in the staging buffer
+        uint64_t* hout = reinterpret_cast<uint64_t*>(sl.pinned_dev) + host_in;
         ServeGather* gv = reinterpret_cast<ServeGather*>(meta + gather_meta_off);
         i = 0;
         for (size_t k = 0; k < add2k.size() + add4k.size(); k++, i++) {
@@ -642,13 +651,13 @@ midint_status launch_batch(midint_server* s, size_t si) {
             const size_t kk = c4 ? k - add2k.size() : k;
             const uint32_t len = std::max(r.na, r.nb) + 1;
             // the full-width sum at the class size: its top limb is the kernel's carry-out
-            gv[i] = ServeGather{d_out + (c4 ? a4k_r : a2k_r) + kk * C, d_out + big_r[i], len - 1 == C ? d_carry + k : nullptr,
+            gv[i] = ServeGather{d_out + (c4 ? a4k_r : a2k_r) + kk * C, hout + big_r[i], len - 1 == C ? d_carry + k : nullptr,
                                 len, 0};
         }
         inst = 0;
         for (size_t k = 0; k < ntt2k.size() + ntt4k.size(); k++, i++) {
             const Request& r = s->reqs[big[i]];
-            gv[i] = ServeGather{d_out + ntt_r + inst * 2 * kNttM, d_out + big_r[i], nullptr, r.na + r.nb, 0};
+            gv[i] = ServeGather{d_out + ntt_r + inst * 2 * kNttM, hout + big_r[i], nullptr, r.na + r.nb, 0};
             inst += k < ntt2k.size() ? 1 : 4;
         }
     }
@@ -659,10 +668,11 @@ midint_status launch_batch(midint_server* s, size_t si) {
     // the ragged kernel's work counter is the last metadata limb, zeroed by
     // the memset of the staging above, so the input copy clears it
     hipStream_t st = sl.stream;
-    hipError_t e = launch_stage_copy(sl.pinned_dev, d_in, in_bytes, st);
+    // (the slot and result tables are read from the staging buffer)
+    const uint8_t* h_meta = sl.pinned_dev + items_off * 8;
+    hipError_t e = launch_stage_in(sl.pinned_dev, d_in, in_bytes,
+                                   reinterpret_cast<const ServeSlot*>(h_meta + slot_meta_off), (uint32_t)n_slots, st);
     uint8_t* d_meta = reinterpret_cast<uint8_t*>(d_in + items_off);
-    if (e == hipSuccess)
-        e = launch_serve_expand((uint32_t)n_slots, reinterpret_cast<const ServeSlot*>(d_meta + slot_meta_off), st);
     if (e == hipSuccess && !items.empty()) {
         const uint32_t n_items = (uint32_t)items.size();
         const uint64_t* dA = d_in + rg_a;
@@ -721,8 +731,8 @@ midint_status launch_batch(midint_server* s, size_t si) {
     if (e == hipSuccess && !ntt4k.empty())
         e = launch_bigmul_combine((uint32_t)ntt4k.size(), d_out + ntt_r + ntt2k.size() * 2 * kNttM, st);
     if (e == hipSuccess)
-        e = launch_serve_gather((uint32_t)big.size(), reinterpret_cast<const ServeGather*>(d_meta + gather_meta_off), st);
-    if (e == hipSuccess) e = launch_stage_copy(d_out, sl.pinned_dev + in_bytes, out_copy * 8, st);
+        e = launch_stage_out(d_out, sl.pinned_dev + host_in * 8, rg_out * 8,
+                             reinterpret_cast<const ServeGather*>(h_meta + gather_meta_off), (uint32_t)big.size(), st);
     if (e == hipSuccess) e = hipEventRecord(sl.ev, st);
     if (e != hipSuccess) {
         (void)hipStreamSynchronize(st);
@@ -735,7 +745,7 @@ midint_status launch_batch(midint_server* s, size_t si) {
     sl.busy = true;
     sl.region_off = off;
     sl.region_bytes = region;
-    sl.out_off_bytes = in_bytes;
+    sl.out_off_bytes = host_in * 8;
     sl.out_bytes = out_copy * 8;
     sl.refs.clear();
     sl.ids = take;
