@@ -11,9 +11,12 @@
 //     device region fits the free part of the HBM pool (the admission
 //     budget) and the staging limits; a request that does not fit waits
 //     (deferred, never dropped);
-//  3. launch: the batch's operands go to HBM with one copy kernel from its
-//     pinned staging buffer, its kernels run on the slot's stream, results
-//     come back with one copy kernel, and an event marks completion.
+//  3. launch: the batch's operands go to HBM with one transfer kernel from
+//     its pinned staging buffer (the operands of requests above 1024 limbs
+//     packed there and spread into their zero-padded class slots on the
+//     way), its kernels run on the slot's stream, results come back with
+//     one transfer kernel (the big requests' results gathered packed from
+//     their padded outputs), and an event marks completion.
 //
 // The transfers are kernels reading and writing the mapped staging memory,
 // not copy-engine transfers: hipMemcpyAsync hands copies above a few tens of
