@@ -87,22 +87,28 @@ __host__ __device__ inline M16Geom m16_geom(uint32_t na, uint32_t nb, uint32_t P
     g.T = (g.nblk + 15) / 16;
     g.Da = 8 * (int32_t)na + 1;
     g.Db = 8 * (int32_t)nb + 1;
-    // window bytes: AOFF + 16 (n + g) + 64 u - DL, u = 4t - k over the used (t, k)
-    int32_t umin = 1 << 30, umax = -(1 << 30), khi = 0;
-    for (uint32_t t = 0; t < g.T; t++) {
-        int32_t lo, hi;
-        m16_krange(g, t, &lo, &hi);
-        if (lo > hi) continue;
-        umin = std::min<int32_t>(umin, 4 * (int32_t)t - hi);
-        umax = std::max<int32_t>(umax, 4 * (int32_t)t - lo);
-        khi = std::max<int32_t>(khi, hi);
-    }
+    // window bytes: AOFF + 16 (n + g) + 64 u - DL, u = 4t - k over the used
+    // (t, k). Both ends of a tile's step range grow by at most 4 from one
+    // tile to the next (its columns move on by 16 blocks, 256 digits, 4
+    // steps, and the clamps to the operands only slow that), so 4t - kmax(t)
+    // and 4t - kmin(t) never decrease: the first tile gives the smallest u
+    // and the last tile the largest (every tile has a nonempty range). This
+    // is O(1) work per product instead of a loop over every tile, which
+    // made this the largest share of the kernel's scalar instructions for
+    // ragged products (checked against that loop for every pair of operand
+    // sizes up to 2048 limbs and every uniform size up to 4096).
+    int32_t lo0, hi0, loL, hiL;
+    m16_krange(g, 0, &lo0, &hi0);
+    m16_krange(g, g.T - 1, &loL, &hiL);
+    const int32_t umin = -hi0;
+    const int32_t umax = 4 * (int32_t)(g.T - 1) - loL;
+    (void)lo0;
+    (void)hiL;
     g.aoff = (uint32_t)std::max<int32_t>(0, kM16Delta - 64 * umin);
     g.aoff = m16_round16(g.aoff);
     // (the window of step umax + 1 is read, not used: one more step of bytes)
     const uint32_t a_end = (uint32_t)((int32_t)g.aoff + 16 * 18 + 64 * (umax + 1) - kM16Delta + 16);
     g.a_bytes = m16_round16(std::max<uint32_t>(a_end, g.aoff + 8 * na + 8));
-    (void)khi;
     // T_k (0 <= k <= kcap, the last k with a digit of b) reads five dwords
     // from byte (x0 - 64k - DL - r + 16g) & ~3, r < 16, g < 4; x0 = 15 mod 16
     // puts b's limb pairs on 16-byte boundaries
