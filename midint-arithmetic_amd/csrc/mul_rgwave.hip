@@ -92,6 +92,27 @@ __device__ __forceinline__ void rgw_digits(uint64_t x0, uint64_t x1, uint32_t n,
     w1 ^= h1;
 }
 
+// The window fragments j < K + 12 that a row's MFMAs use: tile i takes
+// fragment j at step k = 4 i + K - 1 - j, so every j when K >= 4 and only
+// the j with j mod 4 < K below (4, 8 or 12 of the K + 12).
+struct RgwSteps {
+    int n;
+    int j[32];
+};
+template <int K>
+constexpr RgwSteps rgw_steps() {
+    RgwSteps st{};
+    for (int j = 0; j < K + 12; j++) {
+        bool used = false;
+        for (int i = 0; i < 4; i++) {
+            const int k = 4 * i + K - 1 - j;
+            used = used || (k >= 0 && k < K);
+        }
+        if (used) st.j[st.n++] = j;
+    }
+    return st;
+}
+
 template <int K, int KMAX>
 __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, uint32_t lane, const MfPow& pw) {
     using G = RgwCfg<KMAX>;
@@ -166,8 +187,9 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
         if (R + 1 < rows) rgw_load_row(it.a, na, R + 1, lane, nx0, nx1);
         asm volatile("" ::: "memory");
 
-        // 4 K MFMAs over K + 12 window fragments, two fragments ahead
-        constexpr int NJ = K + 12;
+        // 4 K MFMAs over the K + 12 window fragments (those used), two
+        // fragments ahead
+        constexpr RgwSteps ST = rgw_steps<K>();
         const int32_t fb = (int32_t)(16 * (ln + lg)) - kRgwDL + 1024 * (int32_t)R - 64 * (K - 1);
         auto frag = [&](int j) {
             return *reinterpret_cast<const mf_v4i*>(ring + ((uint32_t)(fb + 64 * j) & (kRing - 1)));
@@ -176,16 +198,17 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
 #pragma unroll
         for (int i = 0; i < 4; i++) acc[i] = mf_v4i{0, 0, 0, 0};
         mf_v4i bq[3];
-        bq[0] = frag(0);
-        bq[1] = frag(1);
+        bq[0] = frag(ST.j[0]);
+        bq[1] = frag(ST.j[1]);
 #pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            if (j + 2 < NJ) bq[(j + 2) % 3] = frag(j + 2);
+        for (int q = 0; q < ST.n; q++) {
+            const int j = ST.j[q];
+            if (q + 2 < ST.n) bq[(q + 2) % 3] = frag(ST.j[q + 2]);
             __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 const int k = 4 * i + K - 1 - j;
-                if (k >= 0 && k < K) acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[k], bq[j % 3], acc[i], 0, 0, 0);
+                if (k >= 0 && k < K) acc[i] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A[k], bq[q % 3], acc[i], 0, 0, 0);
             }
             __builtin_amdgcn_sched_barrier(0);
         }
