@@ -37,17 +37,18 @@
 // Column sums to limbs. Lane (n, g) of tile t holds the sums of output
 // digits 16I + 4g + v (v < 4), I = 16t + n: word w = 4I + g of the product,
 // V_w = sum_v (c + 255 B) 2^(8v) < 2^55 (four v_mad_i64_i32; one B more in
-// column 0), written to LDS. Limb L of the product is W_L = V_{2L} +
-// V_{2L+1} 2^32 < 2^88: its low 64 bits, plus the bits above 64 of W_{L-1},
-// one pass of the row adder over the wave's rows.
+// column 0), kept in registers: cross-lane swaps move the four words of
+// lane l's two limbs into lane l (no LDS round trip). Limb L of the product
+// is W_L = V_{2L} + V_{2L+1} 2^32 < 2^88: its low 64 bits, plus the bits
+// above 64 of W_{L-1}, one pass of the row adder over the wave's rows.
 //
 // The same machinery runs the reference's polyKerQ composite (four low-half
 // products and three additions per instance, poly_wave16_kernel) with the
 // intermediates in registers.
 //
 // No barrier: every wave owns its LDS region (digits of a, reversed digits
-// of b, the words V) and one product at a time; LDS operations of a wave
-// execute in order.
+// of b) and one product at a time; LDS operations of a wave execute in
+// order.
 #include "add_core.hpp"
 #include "mfma_digits.hpp"
 #include "midint_internal.h"
@@ -133,10 +134,7 @@ struct W16 {
     static constexpr int X0_MIN = X0_MIN0 > 8 * M + 7 ? X0_MIN0 : 8 * M + 7;
     static constexpr int X0 = ((X0_MIN - 15 + 15) & ~15) + 15;
     static constexpr int BR_BYTES = w16_round16((X0 & ~3) + 20);
-    // V words after a zero pad of two words (V_{-2}, V_{-1})
-    static constexpr int V_OFF = A_BYTES + BR_BYTES + 16;
-    static constexpr int V_BYTES = ROWS_OUT * 128 * 2 * 8;
-    static constexpr int WAVE_BYTES = V_OFF + V_BYTES;
+    static constexpr int WAVE_BYTES = A_BYTES + BR_BYTES;
     static_assert(T >= 1 && T <= kW16MaxTiles, "tile count");
     static_assert(AOFF % 16 == 0 && AOFF >= 0, "a region alignment");
     static_assert(X0 - 64 * P.khi - kW16Delta - 15 >= 0, "br region start");
@@ -158,10 +156,8 @@ struct W16Wave {
     static constexpr W16Plan P = G::P;
     uint8_t* areg;
     uint8_t* breg;
-    uint8_t* vreg;
     const uint8_t* pa;  // A fragment of step k: dwords from pa + 64 (khi - k)
     const uint8_t* pb;  // B fragment u: 16 bytes at pb + 64 (u - umin)
-    uint8_t* pvw;       // word w = 64 t + 4 n + g of tile t at pvw + 512 t
     uint32_t lane, sh;
     MfPow pw;
 
@@ -171,8 +167,7 @@ struct W16Wave {
         lane = threadIdx.x & 63;
         breg = base;
         areg = base + G::BR_BYTES;
-        vreg = base + G::V_OFF;
-        for (uint32_t x = 16 * lane; x < (uint32_t)G::V_OFF; x += 1024)
+        for (uint32_t x = 16 * lane; x < (uint32_t)G::WAVE_BYTES; x += 1024)
             *reinterpret_cast<uint4*>(base + x) = make_uint4(0u, 0u, 0u, 0u);
         const uint32_t ln = lane & 15, lg = lane >> 4;
         const uint32_t yb = (uint32_t)(G::X0 - 64 * P.khi - kW16Delta) - ln + 16 * lg;
@@ -180,7 +175,6 @@ struct W16Wave {
         pa = breg + (yb & ~3u);
         // AOFF + 16 (n + g) + 64 u - DL = 16 (n + g) + 64 (u - umin)
         pb = areg + 16 * (ln + lg);
-        pvw = vreg + 8 * (4 * ln + lg);
         pw = mf_pow(pow8);
     }
 
@@ -225,8 +219,9 @@ struct W16Wave {
         asm volatile("" ::: "memory");
     }
 
-    // The matrix-core phase of the staged operands and the words V_w.
-    __device__ __forceinline__ void product() {
+    // The matrix-core phase of the staged operands and the words V_w: lane
+    // (n, g) of tile t ends with word w = 64 t + 4 n + g in V[t].
+    __device__ __forceinline__ void product(int64_t (&Vt)[G::T]) {
         mf_v4i A[P.khi + 1];
 #pragma unroll
         for (int k = 0; k <= P.khi; k++) {
@@ -267,21 +262,62 @@ struct W16Wave {
             V += (int64_t)acc[t][1] * (int32_t)pw.p8;
             V += (int64_t)acc[t][2] * (int32_t)pw.p16;
             V += (int64_t)acc[t][3] * (int32_t)pw.p24;
-            *reinterpret_cast<int64_t*>(pvw + 512 * t) = V;
+            Vt[t] = V;
         }
-        asm volatile("" ::: "memory");
     }
 
     // Limbs of the product from the words: x_L = W_L mod 2^64, y_L = W_{L-1}
     // >> 64 (W_L = V_{2L} + V_{2L+1} 2^32), x + y, as rows (x0, x1; limbs at
-    // or above PL are not defined).
-    __device__ __forceinline__ void finish(Rows<G::ROWS_OUT>& rows) {
+    // or above PL are not defined). Row ro's words 256 ro + 4 l + j (j < 4)
+    // sit in lane (l mod 16) + 16 j of tile 4 ro + l / 16: a transpose of
+    // 16-lane rows across the row's four tiles (permlane32 then permlane16
+    // swaps) puts them in lane l; the two words before them are lane l - 1's
+    // last two (a shift across the wave, lane 0 taking the previous row's
+    // lane 63, or the zero words V_-2, V_-1).
+    __device__ __forceinline__ void finish(const int64_t (&Vt)[G::T], Rows<G::ROWS_OUT>& rows) {
+        uint32_t pl2 = 0, ph2 = 0, pl3 = 0, ph3 = 0;  // previous row's lane 63 words 254, 255
 #pragma unroll
         for (int ro = 0; ro < G::ROWS_OUT; ro++) {
-            const uint8_t* pv = vreg + 32 * (64 * ro + lane);  // word 4 (64 ro + lane)
-            const ulonglong2 v01 = *reinterpret_cast<const ulonglong2*>(pv - 16);
-            ulonglong2 v23 = *reinterpret_cast<const ulonglong2*>(pv);
-            const ulonglong2 v45 = *reinterpret_cast<const ulonglong2*>(pv + 16);
+            uint32_t lo[4], hi[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int t = 4 * ro + q;
+                const uint64_t v = t < G::T ? (uint64_t)Vt[t] : 0ull;
+                lo[q] = (uint32_t)v;
+                hi[q] = (uint32_t)(v >> 32);
+            }
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                const auto a = __builtin_amdgcn_permlane32_swap(lo[q], lo[q + 2], false, false);
+                lo[q] = a[0];
+                lo[q + 2] = a[1];
+                const auto b = __builtin_amdgcn_permlane32_swap(hi[q], hi[q + 2], false, false);
+                hi[q] = b[0];
+                hi[q + 2] = b[1];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; q += 2) {
+                const auto a = __builtin_amdgcn_permlane16_swap(lo[q], lo[q + 1], false, false);
+                lo[q] = a[0];
+                lo[q + 1] = a[1];
+                const auto b = __builtin_amdgcn_permlane16_swap(hi[q], hi[q + 1], false, false);
+                hi[q] = b[0];
+                hi[q + 1] = b[1];
+            }
+            // lane l - 1's words 4 l - 2, 4 l - 1 (wave_shr:1; lane 0 keeps the old value)
+            const uint32_t m2l = (uint32_t)__builtin_amdgcn_update_dpp((int)pl2, (int)lo[2], 0x138, 0xf, 0xf, false);
+            const uint32_t m2h = (uint32_t)__builtin_amdgcn_update_dpp((int)ph2, (int)hi[2], 0x138, 0xf, 0xf, false);
+            const uint32_t m3l = (uint32_t)__builtin_amdgcn_update_dpp((int)pl3, (int)lo[3], 0x138, 0xf, 0xf, false);
+            const uint32_t m3h = (uint32_t)__builtin_amdgcn_update_dpp((int)ph3, (int)hi[3], 0x138, 0xf, 0xf, false);
+            if (ro + 1 < G::ROWS_OUT) {
+                pl2 = __builtin_amdgcn_readlane(lo[2], 63);
+                ph2 = __builtin_amdgcn_readlane(hi[2], 63);
+                pl3 = __builtin_amdgcn_readlane(lo[3], 63);
+                ph3 = __builtin_amdgcn_readlane(hi[3], 63);
+            }
+            const ulonglong2 v01 = make_ulonglong2(((uint64_t)m2h << 32) | m2l, ((uint64_t)m3h << 32) | m3l);
+            ulonglong2 v23 = make_ulonglong2(((uint64_t)hi[0] << 32) | lo[0], ((uint64_t)hi[1] << 32) | lo[1]);
+            const ulonglong2 v45 = make_ulonglong2(((uint64_t)hi[2] << 32) | lo[2], ((uint64_t)hi[3] << 32) | lo[3]);
             if (ro == 0 && lane == 0) v23.x += kMfBias;  // V_0: column 0 takes 256 B
             const uint64_t sp = v01.y + (v01.x >> 32);
             const uint64_t s0 = v23.y + (v23.x >> 32);
@@ -326,9 +362,10 @@ __global__ __launch_bounds__(64 * kW16Waves, kW16MinBlocks) void mul_wave16_kern
         w16_load<M, G::ROWS_IN>(a + inst * sa, xa);
         w16_load<M, G::ROWS_IN>(b + inst * sb, xb);
         w.stage(xa, xb);
-        w.product();
+        int64_t V[G::T];
+        w.product(V);
         Rows<G::ROWS_OUT> rows;
-        w.finish(rows);
+        w.finish(V, rows);
         uint64_t* gr = r + inst * sr;
 #pragma unroll
         for (int ro = 0; ro < G::ROWS_OUT; ro++) {
@@ -381,9 +418,10 @@ __global__ __launch_bounds__(64 * kW16Waves, kW16MinBlocks) void poly_wave16_ker
     };
     auto mul = [&](const uint64_t (&x)[R][2], const uint64_t (&y)[R][2], uint64_t (&z)[R][2]) {
         w.stage(x, y);
-        w.product();
+        int64_t V[G::T];
+        w.product(V);
         Rows<R> t;
-        w.finish(t);
+        w.finish(V, t);
 #pragma unroll
         for (int ro = 0; ro < R; ro++) {
             z[ro][0] = t.x0[ro];
