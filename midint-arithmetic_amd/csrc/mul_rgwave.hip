@@ -27,17 +27,18 @@
 //    below 0;
 //  * row R's 4 K MFMAs take K + 12 window fragments from the ring (tile i at
 //    fragment j uses k = 4i + K - 1 - j), the tiles are folded into the
-//    row's 256 words V, and the row is normalised at once: limb L of the
+//    row's 256 words V (in registers, moved to the row adder's lanes by
+//    cross-lane swaps), and the row is normalised at once: limb L of the
 //    product is (W_L mod 2^64) + (W_{L-1} >> 64), W_L = V_{2L} + V_{2L+1}
 //    2^32, added by the wave's row adder with the carry from row R - 1 in a
-//    scalar (the last two words of row R - 1 are kept in front of the row's
-//    words). The column bias telescopes as in mul_wave16.hip: 255 B in every
+//    scalar (the last two words of row R - 1 kept in scalars for lane 0 of
+//    row R). The column bias telescopes as in mul_wave16.hip: 255 B in every
 //    column of the rows computed, one B more in column 0, B 2^(64 P) = 0
 //    modulo the output length for any P >= n_a + n_b limbs.
 //
-// Per wave 7280 bytes of LDS: b's digits reversed (1120), the ring (4096),
-// the words V of one row after a 16-byte prefix (2064). Each wave owns its
-// region and one op at a time, so waves never synchronise.
+// Per wave 5216 bytes of LDS: b's digits reversed (1120) and the ring
+// (4096). Each wave owns its region and one op at a time, so waves never
+// synchronise.
 //
 // Additions (add_rgw_kernel): one full sum (max(n_a, n_b) + 1 limbs) per
 // wave, rows of 128 limbs with the carry from row to row in a scalar, the
@@ -58,14 +59,13 @@ constexpr int kRgwSmallK = 9;      // the small-fragment kernel: K <= 9 (n_b <= 
 // digits reversed (digit j at byte X0 - j, X0 = 15 mod 16, bytes
 // [X0 - 64 KMAX + 1, X0 + 20) read), the ring of a's digit rows (row R's
 // windows reach back at most 64 KMAX - 16 digits: two slots for KMAX <= 9,
-// four up to 17), the words V of one row after a 16-byte prefix.
+// four up to 17).
 template <int KMAX>
 struct RgwCfg {
     static constexpr uint32_t RING = KMAX <= 9 ? 2048 : 4096;
     static constexpr int32_t X0 = 64 * KMAX - 1;
     static constexpr uint32_t BR = ((uint32_t)X0 + 21 + 15) / 16 * 16;
-    static constexpr uint32_t VOFF = RING + BR + 16;
-    static constexpr uint32_t WAVE_BYTES = VOFF + 256 * 8;
+    static constexpr uint32_t WAVE_BYTES = RING + BR;
     static_assert(X0 % 16 == 15 && X0 >= 64 * KMAX - 1, "b region origin");
     static_assert(64 * KMAX - 16 <= (int)RING - 1024, "ring look-back");
 };
@@ -120,7 +120,6 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     constexpr int32_t kX0 = G::X0;
     uint8_t* breg = base;  // first, so the fragment reads' offsets fit ds_read2_b32's offset fields
     uint8_t* ring = base + G::BR;
-    uint8_t* vreg = base + G::VOFF;
     const uint32_t na = it.n_a, nb = it.n_b, pl = na + nb;
     const uint32_t ln = lane & 15, lg = lane >> 4;
 
@@ -135,7 +134,6 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
         *reinterpret_cast<uint4*>(ring + x + 16 * lane) = make_uint4(0u, 0u, 0u, 0u);
     for (uint32_t x = 16 * lane; x < G::BR; x += 1024)
         *reinterpret_cast<uint4*>(breg + x) = make_uint4(0u, 0u, 0u, 0u);
-    if (lane == 0) *reinterpret_cast<uint4*>(vreg - 16) = make_uint4(0u, 0u, 0u, 0u);
     asm volatile("" ::: "memory");
 
     // ---- b's digits reversed: digit j at byte X0 - j (n_b + 1 limbs: one
@@ -174,8 +172,7 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
     // ---- rows of the product
     const uint32_t rows = (pl + 127) / 128;
     uint32_t ca = 0, cn = 0;
-    uint8_t* pvw = vreg + 8 * (4 * ln + lg);  // word 64 i + 4 n + g of tile i at pvw + 512 i
-    const uint8_t* pv = vreg + 32 * lane;     // words 4 lane .. 4 lane + 3
+    uint32_t prev[4] = {0u, 0u, 0u, 0u};  // the previous row's words 254, 255 (V_-2, V_-1 before row 0)
     for (uint32_t R = 0; R < rows; R++) {
         // stage a's row R into slot R mod 4
         {
@@ -213,7 +210,9 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
             __builtin_amdgcn_sched_barrier(0);
         }
 
-        // fold the tiles' column sums into the row's words
+        // fold the tiles' column sums into the row's words, moved to the row
+        // adder's lanes through cross-lane swaps (mf_row_words)
+        uint64_t S[4], w[4], p[2];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             int64_t V = (int64_t)kMfFoldBias;
@@ -221,22 +220,17 @@ __device__ __forceinline__ void rgw_product(const MfRgItem& it, uint8_t* base, u
             V += (int64_t)acc[i][1] * (int32_t)pw.p8;
             V += (int64_t)acc[i][2] * (int32_t)pw.p16;
             V += (int64_t)acc[i][3] * (int32_t)pw.p24;
-            *reinterpret_cast<int64_t*>(pvw + 512 * i) = V;
+            S[i] = (uint64_t)V;
         }
-        asm volatile("" ::: "memory");
+        mf_row_words(S, w, p, prev);
 
         // limbs of row R: x_L = W_L mod 2^64, y_L = W_{L-1} >> 64, x + y
-        const ulonglong2 v01 = *reinterpret_cast<const ulonglong2*>(pv - 16);
-        ulonglong2 v23 = *reinterpret_cast<const ulonglong2*>(pv);
-        const ulonglong2 v45 = *reinterpret_cast<const ulonglong2*>(pv + 16);
-        asm volatile("" ::: "memory");
-        if (lane == 63) *reinterpret_cast<ulonglong2*>(vreg - 16) = v45;  // words 254, 255 for row R + 1
-        if (R == 0 && lane == 0) v23.x += kMfBias;                         // V_0: column 0 takes 256 B
-        const uint64_t sp = v01.y + (v01.x >> 32);
-        const uint64_t s0 = v23.y + (v23.x >> 32);
-        const uint64_t s1 = v45.y + (v45.x >> 32);
-        const RowState st = row_build((v23.x & 0xffffffffull) | (s0 << 32), sp >> 32,
-                                      (v45.x & 0xffffffffull) | (s1 << 32), s0 >> 32, false, false);
+        if (R == 0 && lane == 0) w[0] += kMfBias;  // V_0: column 0 takes 256 B
+        const uint64_t sp = p[1] + (p[0] >> 32);
+        const uint64_t s0 = w[1] + (w[0] >> 32);
+        const uint64_t s1 = w[3] + (w[2] >> 32);
+        const RowState st = row_build((w[0] & 0xffffffffull) | (s0 << 32), sp >> 32,
+                                      (w[2] & 0xffffffffull) | (s1 << 32), s0 >> 32, false, false);
         const uint64_t C = row_chain(st, &cn);
         uint64_t z0, z1;
         row_apply(st, C, &z0, &z1);

@@ -268,62 +268,23 @@ struct W16Wave {
 
     // Limbs of the product from the words: x_L = W_L mod 2^64, y_L = W_{L-1}
     // >> 64 (W_L = V_{2L} + V_{2L+1} 2^32), x + y, as rows (x0, x1; limbs at
-    // or above PL are not defined). Row ro's words 256 ro + 4 l + j (j < 4)
-    // sit in lane (l mod 16) + 16 j of tile 4 ro + l / 16: a transpose of
-    // 16-lane rows across the row's four tiles (permlane32 then permlane16
-    // swaps) puts them in lane l; the two words before them are lane l - 1's
-    // last two (a shift across the wave, lane 0 taking the previous row's
-    // lane 63, or the zero words V_-2, V_-1).
+    // or above PL are not defined); row ro's words come from its four tiles
+    // through cross-lane swaps (mf_row_words), the zero words V_-2, V_-1
+    // before row 0.
     __device__ __forceinline__ void finish(const int64_t (&Vt)[G::T], Rows<G::ROWS_OUT>& rows) {
-        uint32_t pl2 = 0, ph2 = 0, pl3 = 0, ph3 = 0;  // previous row's lane 63 words 254, 255
+        uint32_t prev[4] = {0u, 0u, 0u, 0u};  // words V_-2, V_-1 before row 0
 #pragma unroll
         for (int ro = 0; ro < G::ROWS_OUT; ro++) {
-            uint32_t lo[4], hi[4];
+            uint64_t S[4], w[4], p[2];
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int t = 4 * ro + q;
-                const uint64_t v = t < G::T ? (uint64_t)Vt[t] : 0ull;
-                lo[q] = (uint32_t)v;
-                hi[q] = (uint32_t)(v >> 32);
-            }
-#pragma unroll
-            for (int q = 0; q < 2; q++) {
-                const auto a = __builtin_amdgcn_permlane32_swap(lo[q], lo[q + 2], false, false);
-                lo[q] = a[0];
-                lo[q + 2] = a[1];
-                const auto b = __builtin_amdgcn_permlane32_swap(hi[q], hi[q + 2], false, false);
-                hi[q] = b[0];
-                hi[q + 2] = b[1];
-            }
-#pragma unroll
-            for (int q = 0; q < 4; q += 2) {
-                const auto a = __builtin_amdgcn_permlane16_swap(lo[q], lo[q + 1], false, false);
-                lo[q] = a[0];
-                lo[q + 1] = a[1];
-                const auto b = __builtin_amdgcn_permlane16_swap(hi[q], hi[q + 1], false, false);
-                hi[q] = b[0];
-                hi[q + 1] = b[1];
-            }
-            // lane l - 1's words 4 l - 2, 4 l - 1 (wave_shr:1; lane 0 keeps the old value)
-            const uint32_t m2l = (uint32_t)__builtin_amdgcn_update_dpp((int)pl2, (int)lo[2], 0x138, 0xf, 0xf, false);
-            const uint32_t m2h = (uint32_t)__builtin_amdgcn_update_dpp((int)ph2, (int)hi[2], 0x138, 0xf, 0xf, false);
-            const uint32_t m3l = (uint32_t)__builtin_amdgcn_update_dpp((int)pl3, (int)lo[3], 0x138, 0xf, 0xf, false);
-            const uint32_t m3h = (uint32_t)__builtin_amdgcn_update_dpp((int)ph3, (int)hi[3], 0x138, 0xf, 0xf, false);
-            if (ro + 1 < G::ROWS_OUT) {
-                pl2 = __builtin_amdgcn_readlane(lo[2], 63);
-                ph2 = __builtin_amdgcn_readlane(hi[2], 63);
-                pl3 = __builtin_amdgcn_readlane(lo[3], 63);
-                ph3 = __builtin_amdgcn_readlane(hi[3], 63);
-            }
-            const ulonglong2 v01 = make_ulonglong2(((uint64_t)m2h << 32) | m2l, ((uint64_t)m3h << 32) | m3l);
-            ulonglong2 v23 = make_ulonglong2(((uint64_t)hi[0] << 32) | lo[0], ((uint64_t)hi[1] << 32) | lo[1]);
-            const ulonglong2 v45 = make_ulonglong2(((uint64_t)hi[2] << 32) | lo[2], ((uint64_t)hi[3] << 32) | lo[3]);
-            if (ro == 0 && lane == 0) v23.x += kMfBias;  // V_0: column 0 takes 256 B
-            const uint64_t sp = v01.y + (v01.x >> 32);
-            const uint64_t s0 = v23.y + (v23.x >> 32);
-            const uint64_t s1 = v45.y + (v45.x >> 32);
-            rows.x0[ro] = (v23.x & 0xffffffffull) | (s0 << 32);
-            rows.x1[ro] = (v45.x & 0xffffffffull) | (s1 << 32);
+            for (int q = 0; q < 4; q++) S[q] = 4 * ro + q < G::T ? (uint64_t)Vt[4 * ro + q] : 0ull;
+            mf_row_words(S, w, p, prev);
+            if (ro == 0 && lane == 0) w[0] += kMfBias;  // V_0: column 0 takes 256 B
+            const uint64_t sp = p[1] + (p[0] >> 32);
+            const uint64_t s0 = w[1] + (w[0] >> 32);
+            const uint64_t s1 = w[3] + (w[2] >> 32);
+            rows.x0[ro] = (w[0] & 0xffffffffull) | (s0 << 32);
+            rows.x1[ro] = (w[2] & 0xffffffffull) | (s1 << 32);
             rows.y0[ro] = sp >> 32;
             rows.y1[ro] = s0 >> 32;
             rows.s0[ro] = false;
