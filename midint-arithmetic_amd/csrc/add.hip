@@ -3,8 +3,8 @@
 // add_kernel  : r = (a + b) mod B^m per instance    (reference baddKer,
 //               cuda/ker-addition.cu.h:227-260)
 // add6_kernel : r = (6a + 10b) mod B^m per instance (reference a6pb10Ker,
-//               cuda/ker-addition.cu.h:266-328), the same chain of six
-//               register-resident additions.
+//               cuda/ker-addition.cu.h:266-328), the value of its chain of
+//               six additions in one register-resident addition.
 //
 // A block owns `ipb` whole instances (a window of ipb*m limbs); the window is
 // split into rows of 128 limbs, R rows per wave. Loads and stores are 16-byte
@@ -171,7 +171,22 @@ __global__ __launch_bounds__(1024) void add_kernel(uint32_t n_inst, uint32_t m, 
     }
 }
 
-template <int R, bool VEC, bool STRIDED>
+// 6a + 10b of one limb pair: the low 64 bits, and the high part (< 16) in hi.
+__device__ __forceinline__ uint64_t a6b10(uint64_t a, uint64_t b, uint32_t& hi) {
+    const unsigned __int128 s = (unsigned __int128)a * 6u + (unsigned __int128)b * 10u;
+    hi = (uint32_t)(s >> 64);
+    return (uint64_t)s;
+}
+
+// r = (6a + 10b) mod B^m in one carry-resolved addition. The reference's
+// chain of six additions, each mod B^m, ends at the same value, since the
+// reduction mod B^m commutes with every step. Per limb 6a_i + 10b_i = lo_i
+// + hi_i B (hi_i < 16), so 6a + 10b = L + H with L_i = lo_i and H_i =
+// hi_{i-1}: the high part moves one limb up (lane l's first limb takes lane
+// l - 1's, through a wave_shr DPP move; lane 0 takes the previous row's lane
+// 63, or, for a wave's first row, the limb before it, read again from
+// memory), and is dropped at every instance start.
+template <int R, bool VEC, bool STRIDED, bool NT = false>
 __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m, uint32_t ipb, uint32_t step_q,
                                                     uint32_t step_r, const uint64_t* __restrict__ a,
                                                     const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
@@ -188,61 +203,39 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
         load_window<R, false, false, true>(t, a + inst_base * sa, b + inst_base * sb, L, m, step_q, step_r, off0,
                                            inst0, sa, sb);
     else
-        load_window<R, VEC>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
-    uint64_t b0[R], b1[R];
+        load_window<R, VEC, NT>(t, a + base, b + base, L, m, step_q, step_r, off0, inst0);
+    const uint32_t lane = lane_id();
+    const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane;
+    uint32_t h0[R], h1[R];
 #pragma unroll
     for (int v = 0; v < R; v++) {
-        b0[v] = t.y0[v];
-        b1[v] = t.y1[v];
+        t.x0[v] = a6b10(t.x0[v], t.y0[v], h0[v]);
+        t.x1[v] = a6b10(t.x1[v], t.y1[v], h1[v]);
     }
-    uint64_t u0[R], u1[R];
-    // t1 = a + b
-    block_add_rows<R>(t, lds_gp);
-    // t2 = t1 + t1
+    // the high part of the limb before the wave's first one (in the same
+    // instance: off0[0] > 0, so k_first > 0)
+    uint32_t before = 0;
+    if (lane == 0 && k_first < L && off0[0] != 0) {
+        const uint64_t* pa = STRIDED ? a + (inst_base + inst0[0]) * sa + off0[0] - 1 : a + base + k_first - 1;
+        const uint64_t* pb = STRIDED ? b + (inst_base + inst0[0]) * sb + off0[0] - 1 : b + base + k_first - 1;
+        (void)a6b10(*pa, *pb, before);
+    }
 #pragma unroll
     for (int v = 0; v < R; v++) {
-        t.y0[v] = t.x0[v];
-        t.y1[v] = t.x1[v];
+        const uint32_t in0 = (uint32_t)__builtin_amdgcn_update_dpp((int)before, (int)h1[v], 0x138, 0xf, 0xf, false);
+        t.y0[v] = t.s0[v] ? 0ull : (uint64_t)in0;
+        t.y1[v] = t.s1[v] ? 0ull : (uint64_t)h0[v];
+        before = (uint32_t)__builtin_amdgcn_readlane((int)h1[v], 63);
     }
     block_add_rows<R>(t, lds_gp);
-    // t3 = t2 + b
-#pragma unroll
-    for (int v = 0; v < R; v++) {
-        t.y0[v] = b0[v];
-        t.y1[v] = b1[v];
-    }
-    block_add_rows<R>(t, lds_gp);
-    // t4 = t3 + t3 (keep t3)
-#pragma unroll
-    for (int v = 0; v < R; v++) {
-        u0[v] = t.x0[v];
-        u1[v] = t.x1[v];
-        t.y0[v] = t.x0[v];
-        t.y1[v] = t.x1[v];
-    }
-    block_add_rows<R>(t, lds_gp);
-    // t5 = t4 + t3
-#pragma unroll
-    for (int v = 0; v < R; v++) {
-        t.y0[v] = u0[v];
-        t.y1[v] = u1[v];
-    }
-    block_add_rows<R>(t, lds_gp);
-    // r = t5 + b
-#pragma unroll
-    for (int v = 0; v < R; v++) {
-        t.y0[v] = b0[v];
-        t.y1[v] = b1[v];
-    }
-    block_add_rows<R>(t, lds_gp);
-    store_window<R, VEC, false, STRIDED>(STRIDED ? r + inst_base * sr : r + base, t, L, m, off0, inst0, sr);
+    store_window<R, VEC, NT, STRIDED>(STRIDED ? r + inst_base * sr : r + base, t, L, m, off0, inst0, sr);
 }
 
 // Instances longer than one workgroup's rows (m > kAddRowMax limbs): one
 // instance per workgroup, walked in chunks of (waves * R * 128) limbs. Each
 // addition of the chain keeps its carry between chunks in a scalar, so any
-// length runs with the registers of R rows. SIX: the six-add chain of
-// add6_kernel, chunk by chunk (each of its additions carries separately).
+// length runs with the registers of R rows. SIX: the reference's six-add
+// chain, chunk by chunk (each of its additions carries separately).
 template <int R, bool VEC, bool SIX>
 __global__ __launch_bounds__(1024) void add_long_kernel(uint32_t m, const uint64_t* __restrict__ a,
                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
@@ -369,6 +362,8 @@ static hipError_t launch_add_r(const AddGeom& g, bool vec, bool six, uint32_t n,
     if (six) {
         if (strided)
             add6_kernel<R, false, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
+        else if (vec && nt)
+            add6_kernel<R, true, false, true><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
         else if (vec)
             add6_kernel<R, true, false><<<blocks, g.threads, 0, s>>>(n, m, g.ipb, sq, sr, a, b, r, sa, sb, sr_);
         else

@@ -133,15 +133,28 @@ def test_add_in_place_and_unaligned_vectors(ctx, torch_dev, orc, m):
     assert np.array_equal(host(big_r)[1:], want)
 
 
-@pytest.mark.parametrize("m", [1, 2, 7, 64, 300, 2048])
+@pytest.mark.parametrize("m", ADD_SIZES)
 def test_add6(ctx, torch_dev, orc, m):
+    """(6a + 10b) mod B^m in one addition: each limb's high part moves up one
+    limb, across lanes, rows and waves, and stops at instance starts."""
     torch = torch_dev
-    n = 21
-    a, b = operands(m * 7, n, m)
-    dr = empty(torch, n * m)
-    ctx.add6(dev(torch, a), dev(torch, b), dr, m)
-    torch.cuda.synchronize()
-    assert np.array_equal(host(dr), orc.add6(a, b, m))
+    for n in (1, 5, 21):
+        a, b = operands(m * 7 + n, n, m)
+        want = orc.add6(a, b, m)
+        da = dev(torch, a)
+        dr = empty(torch, n * m)
+        ctx.add6(da, dev(torch, b), dr, m)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(dr), want), (m, n)
+        # in place, and 8-byte aligned views (scalar loads and stores)
+        ctx.add6(da, dev(torch, b), da, m)
+        big_a = dev(torch, np.concatenate([[np.uint64(0)], a]))
+        big_b = dev(torch, np.concatenate([[np.uint64(0)], b]))
+        big_r = empty(torch, n * m + 1)
+        ctx.add6(big_a[1:], big_b[1:], big_r[1:], m)
+        torch.cuda.synchronize()
+        assert np.array_equal(host(da), want), (m, n)
+        assert np.array_equal(host(big_r)[1:], want), (m, n)
 
 
 # Instances longer than one workgroup's rows (4096 limbs) run chunk by chunk
