@@ -234,8 +234,8 @@ __global__ __launch_bounds__(1024) void add6_kernel(uint32_t n_inst, uint32_t m,
 // Instances longer than one workgroup's rows (m > kAddRowMax limbs): one
 // instance per workgroup, walked in chunks of (waves * R * 128) limbs. Each
 // addition of the chain keeps its carry between chunks in a scalar, so any
-// length runs with the registers of R rows. SIX: the reference's six-add
-// chain, chunk by chunk (each of its additions carries separately).
+// length runs with the registers of R rows. SIX: the value of the
+// reference's six-add chain, (6a + 10b) mod B^m, one addition per chunk.
 template <int R, bool VEC, bool SIX>
 __global__ __launch_bounds__(1024) void add_long_kernel(uint32_t m, const uint64_t* __restrict__ a,
                                                         const uint64_t* __restrict__ b, uint64_t* __restrict__ r,
@@ -248,9 +248,8 @@ __global__ __launch_bounds__(1024) void add_long_kernel(uint32_t m, const uint64
     uint64_t* pr = r + inst * sr;
     const uint32_t chunk = (blockDim.x / kWave) * R * kRowLimbs;
     const uint32_t k_first = (threadIdx.x / kWave) * R * kRowLimbs + 2 * lane_id();
-    uint32_t c[SIX ? 6 : 1];
-#pragma unroll
-    for (int q = 0; q < (SIX ? 6 : 1); q++) c[q] = 0;
+    __shared__ uint32_t lds_hi[2];
+    uint32_t c[1] = {0u};
     for (uint32_t base = 0; base < m; base += chunk) {
         Rows<R> t;
 #pragma unroll
@@ -276,43 +275,39 @@ __global__ __launch_bounds__(1024) void add_long_kernel(uint32_t m, const uint64
                     carry[inst] = (uint8_t)limb_cout(ax1[v], t.x1[v] - t.c1[v], t.c1[v]);
             }
         } else {
-            uint64_t b0[R], b1[R], u0[R], u1[R];
+            // (6a + 10b) mod B^m in one addition per chunk (as add6_kernel):
+            // each limb's high part moves one limb up. A wave's first limb
+            // takes the high part of the limb before it, read again from
+            // memory (same chunk: not yet stored), except in wave 0, where
+            // that limb is the previous chunk's last, already stored (and
+            // overwritten when r = a): the block's last wave hands its high
+            // part over in an LDS slot per chunk parity (written before this
+            // chunk's barrier, read after it, rewritten only after the next)
+            const uint32_t lane = lane_id();
+            const uint32_t wave = threadIdx.x / kWave;
+            const uint32_t par = (base / chunk) & 1u;
+            uint32_t h0[R], h1[R];
 #pragma unroll
             for (int v = 0; v < R; v++) {
-                b0[v] = t.y0[v];
-                b1[v] = t.y1[v];
+                t.x0[v] = a6b10(t.x0[v], t.y0[v], h0[v]);
+                t.x1[v] = a6b10(t.x1[v], t.y1[v], h1[v]);
             }
-            block_add_rows_carry<R>(t, lds_gp, &c[0]);  // t1 = a + b
+            uint32_t before = 0;
+            if (wave == 0) {
+                before = base == 0 ? 0u : lds_hi[par ^ 1u];
+            } else if (lane == 0 && base + k_first < m) {
+                (void)a6b10(pa[base + k_first - 1], pb[base + k_first - 1], before);
+            }
 #pragma unroll
             for (int v = 0; v < R; v++) {
-                t.y0[v] = t.x0[v];
-                t.y1[v] = t.x1[v];
+                const uint32_t in0 =
+                    (uint32_t)__builtin_amdgcn_update_dpp((int)before, (int)h1[v], 0x138, 0xf, 0xf, false);
+                t.y0[v] = (uint64_t)in0;
+                t.y1[v] = (uint64_t)h0[v];
+                before = (uint32_t)__builtin_amdgcn_readlane((int)h1[v], 63);
             }
-            block_add_rows_carry<R>(t, lds_gp, &c[1]);  // t2 = t1 + t1
-#pragma unroll
-            for (int v = 0; v < R; v++) {
-                t.y0[v] = b0[v];
-                t.y1[v] = b1[v];
-            }
-            block_add_rows_carry<R>(t, lds_gp, &c[2]);  // t3 = t2 + b
-#pragma unroll
-            for (int v = 0; v < R; v++) {
-                u0[v] = t.y0[v] = t.x0[v];
-                u1[v] = t.y1[v] = t.x1[v];
-            }
-            block_add_rows_carry<R>(t, lds_gp, &c[3]);  // t4 = t3 + t3
-#pragma unroll
-            for (int v = 0; v < R; v++) {
-                t.y0[v] = u0[v];
-                t.y1[v] = u1[v];
-            }
-            block_add_rows_carry<R>(t, lds_gp, &c[4]);  // t5 = t4 + t3
-#pragma unroll
-            for (int v = 0; v < R; v++) {
-                t.y0[v] = b0[v];
-                t.y1[v] = b1[v];
-            }
-            block_add_rows_carry<R>(t, lds_gp, &c[5]);  // r = t5 + b
+            if (wave == blockDim.x / kWave - 1 && lane == 0) lds_hi[par] = before;
+            block_add_rows_carry<R>(t, lds_gp, &c[0]);
 #pragma unroll
             for (int v = 0; v < R; v++)
                 store_pair<VEC, true>(pr, base + k_first + v * kRowLimbs, m, t.x0[v], t.x1[v]);

@@ -193,6 +193,11 @@ def test_add_long_instances(ctx, torch_dev, orc, m):
     ctx.add(big_a[1:], big_b[1:], big_a[1:], m)
     torch.cuda.synchronize()
     assert np.array_equal(host(big_a)[1:], want), m
+    # add6 in place: each chunk's first limb takes the high part of the
+    # previous chunk's last limb, which the previous chunk has overwritten
+    ctx.add6(da, db, da, m)
+    torch.cuda.synchronize()
+    assert np.array_equal(host(da), orc.add6(a, b, m)), m
 
 
 def test_add_long_strided(mi, ctx, torch_dev, orc):
